@@ -1,0 +1,323 @@
+/*
+ * ns_amd.h — C ABI of the MI355X per-tick brain update (libns_amd.so).
+ *
+ * Plain C: pointers, sizes and POD records only.  Every entry point returns
+ * 0 / non-null on success and -1 / NULL on failure, with the reason kept in a
+ * thread-local string read by ns_error().  No exception crosses this ABI.
+ *
+ * Two surfaces replace the reference's two entry surfaces for the same
+ * arithmetic (reference paths are relative to the neural-selection repo):
+ *
+ *   1. Batched episodes — drop-in for gym_adapter/native.cpp (genome handles)
+ *      and gym_adapter/gpu.cu (ns_gpu_*), bound from Python by ctypes in
+ *      gym_adapter/gpu.py:35-38 and gym_adapter/adapter.py:31-39.
+ *   2. World pool — the neural part of src/main.cu's step loop
+ *      (main.cu:2634-2768) over checkpointable neuron/synapse records
+ *      (main.cu:164-177), replacing the kernels synapse_delivery,
+ *      neuron_step, synapse_plasticity(_delivery), homeostasis_step.
+ *
+ * A third surface, serving, has no reference counterpart: it is the
+ * continuous-batching episode loop that gym_adapter/batch_evaluate.py
+ * approximates with lock-step masks.
+ */
+#ifndef NS_AMD_H
+#define NS_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Record layouts.  Byte-identical to the reference's structs so that state */
+/* moves between the reference and this library without conversion.        */
+/* ------------------------------------------------------------------------ */
+
+/* src/physiology.cuh:29-45 NeuronGene (124 B). */
+typedef struct ns_neuron_gene {
+    float capacitance_pf, leak_ns, rest_mv, threshold_mv, slope_mv;
+    float adapt_ns, adapt_tau_ms, spike_adapt_pa, reset_mv;
+    float afterdepol_spike_pa, afterdepol_tau_ms;
+    uint8_t type, active, modulator, inhibitory_transmitter;
+    float lp_scale[4];            /* LPConductance: calcium, calcium_k, h, leak */
+    float prinz_g[8];             /* PrinzConductance */
+    float prinz_ambient_leak_ns;
+    int8_t sensory_channel;
+    uint8_t pad0_[3];
+    float receptor_baseline_hz, receptor_gain_hz;
+    float receptor_event_ns, receptor_decay_ms;
+    float target_affinity;
+} ns_neuron_gene;
+
+/* src/physiology.cuh:50-58 NeuronState (120 B).  `cell` is the union of
+ * LPCell x[16] (lp_cell.cuh:16), PrinzCell {v, ca, m[7], h[4]}
+ * (prinz_cell.cuh:18) and LFSCell {voltage, threshold, activated[3], open[3]}
+ * (physiology.cuh:46-49). */
+typedef struct ns_neuron_state {
+    float voltage_mv, adaptation_pa, afterdepol_pa, inh_ns, recent_depol_mv;
+    float log_ampa_scale;
+    float dendrite_exc_ns[4];
+    float cell[16];
+    int32_t total_spikes;
+    uint8_t spike, pad0_[3];
+    float receptor_ns;
+    uint32_t receptor_rng;
+} ns_neuron_state;
+
+/* src/physiology.cuh:59-65 SynapseGene (40 B). pre/post are brain-local
+ * neuron indices on the batch surface and pool slots on the world surface. */
+typedef struct ns_synapse_gene {
+    float initial_weight_ns, utilization, recovery_ms, facilitation_ms;
+    int32_t pre, post;
+    uint8_t delay, active, plasticity, branch, release_sites, mod_receptors;
+    uint8_t pad0_[2];
+    float slow_recovery_ms, slow_depletion;
+} ns_synapse_gene;
+
+/* src/physiology.cuh:66-76 SynapseState (80 B). */
+typedef struct ns_synapse_state {
+    float weight_ns, running_utilization, pre_trace, post_trace;
+    float eligible_ltp, eligible_ltd, da_age_ms;
+    float spine_volume, structural_age_ms;
+    float pre_age_ms, post_age_ms;
+    uint32_t pad0_;
+    int64_t last_arrival_tick;
+    uint64_t ready_mask;
+    uint32_t rng_state;
+    uint8_t arrived, pad1_[3];
+    float slow_resource;
+    uint32_t pad2_;
+} ns_synapse_state;
+
+/* src/main.cu:164-171 CheckpointNeuron (344 B), HISTORY = 9. */
+typedef struct ns_checkpoint_neuron {
+    int32_t slot, owner, next, port;
+    ns_neuron_gene gene;
+    ns_neuron_state state;
+    float pending_exc[4], pending_inh;
+    uint64_t mod_event_tick[6];
+    uint8_t spike_history[9];
+    uint8_t pad0_[7];
+} ns_checkpoint_neuron;
+
+/* src/main.cu:173-177 CheckpointSynapse (136 B). */
+typedef struct ns_checkpoint_synapse {
+    int32_t slot, owner, next;
+    ns_synapse_gene gene;
+    uint32_t pad0_;
+    ns_synapse_state state;
+} ns_checkpoint_synapse;
+
+/* ------------------------------------------------------------------------ */
+/* Errors                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* gym_adapter/native.cpp:45 — message of the last failure on this thread. */
+const char *ns_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Batched-episode surface                                               */
+/* ------------------------------------------------------------------------ */
+
+/* gym_adapter/native.cpp:46-82 ns_create.  Host-side genome handle; no
+ * device work.  Neuron genes are count x 10 floats {profile, sensory
+ * channel, active, inhibitory, modulator, capacitance multiplier, leak
+ * multiplier, threshold mV, receptor gain Hz, receptor event nS}; synapse
+ * genes are edges x 11 floats {pre, post, weight nS, active, delay 1..8,
+ * branch, U, recovery ms, facilitation ms, plasticity 0..5, modulator
+ * receptor mask}.  Same validation and messages as the reference. */
+void *ns_create(int count, int edges, int channels, int outputs,
+                const float *neuron_genes, const float *synapse_genes,
+                const int *motors);
+
+/* gym_adapter/native.cpp:83-87 — original indices used to seed reset RNGs
+ * after subgraph compilation (seed + original*7919 / + original*104729). */
+void ns_original_indices(void *brain, const uint32_t *neurons,
+                         const uint32_t *synapses);
+
+/* gym_adapter/native.cpp:88 */
+void ns_destroy(void *brain);
+
+/* gym_adapter/gpu.cu:67-109 ns_gpu_create.  Borrows `count` handles made by
+ * ns_create (or ns_create_ex); the caller may destroy them afterwards.  All
+ * brains must share channels and outputs.  Uploads genes into the
+ * structure-of-arrays device pool and builds the incoming CSR in ascending
+ * edge order. */
+void *ns_gpu_create(void **brains, int count);
+
+/* gym_adapter/gpu.cu:110 */
+void ns_gpu_destroy(void *batch);
+
+/* gym_adapter/gpu.cu:111-113 — reset masked brains: tick 0, fresh state
+ * seeded seeds[i] + original_neuron*7919 / + original_synapse*104729. */
+int ns_gpu_reset(void *batch, const uint8_t *mask, const uint32_t *seeds);
+
+/* gym_adapter/gpu.cu:114-123 — advance masked brains `ticks` (1..10000)
+ * one-ms ticks with constant encoded input (count x channels, finite, in
+ * [0,1]) and input gain (>= 0); output = per-motor spike counts
+ * (count x outputs), status[i] = -1 if brain i reached a nonfinite voltage.
+ * Synchronous, like the reference. */
+int ns_gpu_step(void *batch, const float *input, const uint8_t *mask,
+                int ticks, float gain, float *output, int *status);
+
+/* ---- extensions of the batch surface (no reference counterpart) -------- */
+
+/* ns_create with the delay bound raised from 8 up to NS_MAX_DELAY ticks
+ * (a delay ring longer than the reference's 9 slots). */
+#define NS_MAX_DELAY 32
+void *ns_create_ex(int count, int edges, int channels, int outputs,
+                   const float *neuron_genes, const float *synapse_genes,
+                   const int *motors, int max_delay);
+
+/* Build a batch from many genomes in one call: brain i has neuron_counts[i]
+ * neurons and edge_counts[i] synapses; genes and motors are concatenated in
+ * brain order with the ns_create encodings (synapse pre/post brain-local).
+ * world_rules = 1 selects src/main.cu's neuron_step rules (230 pA per
+ * sensory level, receptor events only for sensory channels < 6, inactive
+ * neurons keep their history) instead of core.cuh's. */
+void *ns_gpu_create_bulk(int count, const int *neuron_counts,
+                         const int *edge_counts, int channels, int outputs,
+                         const float *neuron_genes, const float *synapse_genes,
+                         const int *motors, int max_delay, int world_rules);
+
+/* Execution form of ns_gpu_step:
+ *   NS_FORM_RESIDENT — one launch per call; each workgroup keeps a pack of
+ *                      brains in registers/LDS for all ticks (default);
+ *   NS_FORM_PER_TICK — plain form: delivery, neuron and plasticity kernels
+ *                      launched once per tick over the global pools.
+ * Both give bit-identical results. */
+#define NS_FORM_RESIDENT 0
+#define NS_FORM_PER_TICK 1
+int ns_gpu_set_form(void *batch, int form);
+
+/* Pack tuning: workgroup size (64..1024, multiple of 64), neurons and
+ * synapses per thread.  ns_gpu_autotune times candidate shapes on the
+ * current pools (state is saved and restored around the trials). */
+int ns_gpu_set_tiles(void *batch, int threads, int neurons_per_thread,
+                     int synapses_per_thread);
+int ns_gpu_autotune(void *batch, int ticks, int *threads,
+                    int *neurons_per_thread, int *synapses_per_thread);
+
+/* Device-resident step: input/mask/output/status are device pointers and
+ * `stream` is a hipStream_t (NULL = default stream).  Asynchronous.  No
+ * input validation (the caller owns the device buffers). */
+int ns_gpu_step_device(void *batch, const float *input, const uint8_t *mask,
+                       int ticks, float gain, float *output, int *status,
+                       void *stream);
+
+/* Copy state out / in, AoS in genome order.  history9 is count_neurons x 9
+ * bytes laid out like gym_adapter/core.cuh's history[i][9] ring
+ * (slot = tick % 9).  ticks: one int64 per brain. Any pointer may be NULL. */
+int ns_gpu_export(void *batch, ns_neuron_state *neurons,
+                  ns_synapse_state *synapses, uint8_t *history9,
+                  int64_t *ticks);
+int ns_gpu_import(void *batch, const ns_neuron_state *neurons,
+                  const ns_synapse_state *synapses, const uint8_t *history9,
+                  const int64_t *ticks);
+
+/* Raw 32-bit spike shift registers (bit k = spike k ticks before the last
+ * completed tick), one per neuron in genome order. */
+int ns_gpu_export_spike_bits(void *batch, uint32_t *bits);
+
+typedef struct ns_batch_info {
+    int64_t brains, neurons, synapses;
+    int64_t resident_packs, fallback_brains;
+    int64_t events;             /* delivered arrivals since create */
+    int threads, neurons_per_thread, synapses_per_thread, form;
+    int64_t device_bytes;
+} ns_batch_info;
+int ns_gpu_info(void *batch, ns_batch_info *info);
+
+/* ------------------------------------------------------------------------ */
+/* 2. World-pool surface (src/main.cu)                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Pool over checkpoint records.  Capacities bound the record slots
+ * (main.cu:33-35: 32768 animals, 4194304 neurons, 8388608 synapses). */
+void *ns_pool_create(int animal_capacity, int neuron_capacity,
+                     int synapse_capacity);
+void ns_pool_destroy(void *pool);
+
+/* Load occupied neuron/synapse records (ascending slot order, as written by
+ * checkpoint_impl.cuh:135-220) into per-animal device segments.  Validates
+ * like checkpoint_impl.cuh:272-311.  Replaces all previous contents. */
+int ns_pool_import_records(void *pool, const ns_checkpoint_neuron *neurons,
+                           int neuron_count,
+                           const ns_checkpoint_synapse *synapses,
+                           int synapse_count);
+
+/* Write the records back in ascending slot order (main.cu:197-264). */
+int ns_pool_export_records(void *pool, ns_checkpoint_neuron *neurons,
+                           int neuron_capacity,
+                           ns_checkpoint_synapse *synapses,
+                           int synapse_capacity, int *neuron_count,
+                           int *synapse_count);
+
+/* One world step of neural ticks (main.cu:2702-2728): ticks first_tick ..
+ * first_tick+15 for every imported animal.  senses: animal_capacity x 19
+ * levels (main.cu:729-737); motors: animal_capacity x 5 spike counts,
+ * incremented (main.cu:743); frozen: animal_capacity flags or NULL
+ * (main.cu:142-146).  Host pointers, synchronous. */
+int ns_world_neural_step(void *pool, int64_t first_tick, const float *senses,
+                         int *motors, const uint8_t *frozen);
+
+/* homeostasis_step (main.cu:1287-1300): age[a] in world steps. */
+int ns_world_homeostasis(void *pool, const int *age, const uint8_t *frozen);
+
+/* ------------------------------------------------------------------------ */
+/* 3. Serving: continuous batching of genome-evaluation episodes            */
+/* ------------------------------------------------------------------------ */
+
+/* Episode task: a seeded 4-d linear system x' = A x + B u + N(0, sigma^2)
+ * observed through an 8-sensor opponent rate code; two motor neurons drive
+ * u.  The brain and task state stay resident in the device pools until the
+ * episode ends.  Requests are admitted when their neurons and synapses fit
+ * the free pool space. */
+typedef struct ns_serve_config {
+    int neuron_capacity, synapse_capacity, episode_capacity;
+    int steps_per_episode;      /* env steps, 16 ticks each */
+    int ticks_per_step;
+    int steps_per_launch;       /* env steps per device launch */
+    float noise_sigma;
+    float input_gain;
+} ns_serve_config;
+
+void *ns_serve_create(const ns_serve_config *config);
+void ns_serve_destroy(void *server);
+
+/* Queue one episode (genome in ns_create encoding with channels = 8 and
+ * outputs = 2).  Returns a request id >= 0.  Thread-safe with respect to
+ * the serving thread. */
+int64_t ns_serve_submit(void *server, int count, int edges,
+                        const float *neuron_genes, const float *synapse_genes,
+                        const int *motors, uint32_t seed, uint32_t task_seed);
+
+typedef struct ns_serve_result {
+    int64_t id;
+    double submit_s, admit_s, finish_s;  /* seconds since server creation */
+    float episode_return;                /* sum of -|x|^2 over steps */
+    float final_state[4];
+    int32_t motor_spikes[2];
+    int32_t status;
+} ns_serve_result;
+
+/* Move up to `capacity` finished results into `out`; returns how many. */
+int ns_serve_poll(void *server, ns_serve_result *out, int capacity);
+
+/* Block until every submitted episode has finished. */
+int ns_serve_drain(void *server);
+
+typedef struct ns_serve_stats {
+    int64_t submitted, admitted, finished, launches;
+    int64_t resident_peak;
+    double busy_s;
+} ns_serve_stats;
+int ns_serve_stats_get(void *server, ns_serve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NS_AMD_H */

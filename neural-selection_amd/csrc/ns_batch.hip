@@ -1,0 +1,1002 @@
+// Batched-episode surface of libns_amd.so: genome handles (ns_create, a
+// drop-in for gym_adapter/native.cpp:46-94) and the device batch (ns_gpu_*,
+// a drop-in for gym_adapter/gpu.cu:66-123), plus the pool builder shared by
+// every surface.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "ns_host.h"
+
+namespace nsa {
+
+thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+
+// ---------------------------------------------------------------------------
+// Neuron profiles (src/physiology.cuh:82-176, prinz_cell.cuh:24-28).
+// ---------------------------------------------------------------------------
+static ns_neuron_gene blank_gene() {
+    ns_neuron_gene g;
+    std::memset(&g, 0, sizeof g);
+    g.sensory_channel = -1;
+    g.receptor_decay_ms = 3;
+    g.target_affinity = 1;
+    return g;
+}
+
+static void set_membrane(ns_neuron_gene &g, const float (&v)[11]) {
+    g.capacitance_pf = v[0]; g.leak_ns = v[1]; g.rest_mv = v[2]; g.threshold_mv = v[3];
+    g.slope_mv = v[4]; g.adapt_ns = v[5]; g.adapt_tau_ms = v[6]; g.spike_adapt_pa = v[7];
+    g.reset_mv = v[8]; g.afterdepol_spike_pa = v[9]; g.afterdepol_tau_ms = v[10];
+}
+
+ns_neuron_gene profile_gene(int type) {
+    static const float kPyramidal[11] = {104, 4.3f, -65, -52, 0.8f, -0.8f, 88, 65, -53, 0, 0};
+    static const float kInterneuron[11] = {59, 2.9f, -62, -42, 3.0f, 1.8f, 16, 61, -54, 0, 0};
+    ns_neuron_gene g = blank_gene();
+    g.active = 1;
+    const bool fast_spiking = type == 1 || type == 2 || type == 3;
+    set_membrane(g, fast_spiking ? kInterneuron : kPyramidal);
+    g.inhibitory_transmitter = fast_spiking ? 1 : 0;
+    g.type = uint8_t(type > 6 ? 6 : type);
+    switch (g.type) {
+    case 0:
+        g.reset_mv = -68.0f; g.afterdepol_spike_pa = 120.0f; g.afterdepol_tau_ms = 45.0f;
+        break;
+    case 2:
+        for (float &x : g.lp_scale) x = 1;
+        break;
+    case 3: {
+        const float pm[8] = {400 * 628.0f, 2.5f * 628, 6 * 628.0f, 50 * 628.0f,
+                             10 * 628.0f, 100 * 628.0f, .01f * 628, 0};
+        std::memcpy(g.prinz_g, pm, sizeof pm);
+        break;
+    }
+    case 4:
+        g.capacitance_pf = 8.7f; g.leak_ns = 8.7f / 12.0f; g.rest_mv = -60.0f;
+        g.threshold_mv = -40.0f; g.reset_mv = -60.0f; g.adapt_ns = 0; g.spike_adapt_pa = 0;
+        break;
+    case 5:
+        g.capacitance_pf = 10.0f; g.leak_ns = 0.5f; g.rest_mv = -60.0f; g.threshold_mv = -40.0f;
+        g.reset_mv = -60.0f; g.adapt_ns = 0; g.spike_adapt_pa = 0;
+        g.receptor_baseline_hz = 7.0f; g.receptor_gain_hz = 43.0f;
+        g.receptor_event_ns = 2.0f; g.receptor_decay_ms = 3.0f;
+        break;
+    case 6:
+        g.capacitance_pf = 1200.0f; g.leak_ns = 1000.0f / 65.6f; g.rest_mv = -45.1f;
+        g.threshold_mv = -51.8f;
+        break;
+    default:
+        break;
+    }
+    return g;
+}
+
+// gym_adapter/native.cpp:46-82, with the delay bound generalised.
+HostBrain *parse_gym_brain(int count, int edges, int channels, int outputs, const float *ng,
+                           const float *sg, const int *motors, int max_delay) {
+    if (count < 1 || edges < 0 || channels < 1 || channels > 65535 || outputs < 1)
+        throw std::runtime_error("Invalid brain dimensions");
+    if (max_delay < 1 || max_delay > NS_MAX_DELAY) throw std::runtime_error("Invalid delay bound");
+    auto *b = new HostBrain;
+    try {
+        b->channels = channels;
+        b->max_delay = max_delay;
+        for (int i = 0; i < count; ++i) {
+            const float *p = ng + size_t(i) * 10;
+            if (p[0] < 0 || p[0] > 6 || p[1] < -1 || p[1] >= channels || p[2] < 0 || p[2] > 1 ||
+                p[3] < 0 || p[3] > 1 || p[4] < 0 || p[4] > 3)
+                throw std::runtime_error("Invalid neuron gene");
+            ns_neuron_gene g = profile_gene(int(uint8_t(p[0])));
+            b->sensory.push_back(int(p[1]));
+            g.sensory_channel = p[1] < 0 ? -1 : 0;
+            g.active = uint8_t(p[2]);
+            g.inhibitory_transmitter = uint8_t(p[3]);
+            g.modulator = uint8_t(p[4]);
+            if (p[5] <= 0 || p[6] <= 0 || p[7] < -80 || p[7] > -20 || p[8] < 0 || p[9] < 0)
+                throw std::runtime_error("Invalid membrane parameters");
+            g.capacitance_pf *= p[5];
+            g.leak_ns *= p[6];
+            g.threshold_mv = p[7];
+            g.receptor_baseline_hz = 0;
+            g.receptor_gain_hz = p[8];
+            g.receptor_event_ns = p[9];
+            b->genes.push_back(g);
+            b->receptor.push_back(g.type == 5);
+        }
+        for (int i = 0; i < edges; ++i) {
+            const float *p = sg + size_t(i) * 11;
+            if (p[0] < 0 || p[0] >= count || p[1] < 0 || p[1] >= count || p[2] < 0 || p[3] < 0 ||
+                p[3] > 1 || p[4] < 1 || p[4] > max_delay || p[5] < 0 || p[5] >= 4 || p[6] <= 0 ||
+                p[6] > 1 || p[7] <= 0 || p[8] < 0 || p[9] < 0 || p[9] > 5 || p[10] < 0 ||
+                p[10] > 7)
+                throw std::runtime_error("Invalid synapse gene");
+            ns_synapse_gene g;
+            std::memset(&g, 0, sizeof g);
+            g.pre = int(p[0]); g.post = int(p[1]); g.initial_weight_ns = p[2];
+            g.active = uint8_t(p[3]); g.delay = uint8_t(p[4]); g.branch = uint8_t(p[5]);
+            g.utilization = p[6]; g.recovery_ms = p[7]; g.facilitation_ms = p[8];
+            g.plasticity = uint8_t(p[9]); g.mod_receptors = uint8_t(p[10]); g.release_sites = 14;
+            b->edges.push_back(g);
+        }
+        for (int i = 0; i < outputs; ++i) {
+            if (motors[i] < 0 || motors[i] >= count) throw std::runtime_error("Invalid motor index");
+            b->motors.push_back(motors[i]);
+        }
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+static inline bool excitatory_cell(unsigned t) { return t == 0 || t == 4 || t == 5; }
+static inline int site_count(const ns_synapse_gene &g) {   // physiology.cuh:376-378
+    return g.release_sites ? (g.release_sites > 64 ? 64 : g.release_sites) : 14;
+}
+
+// ---------------------------------------------------------------------------
+// Pool builder
+// ---------------------------------------------------------------------------
+namespace {
+struct Bin {
+    int nn = 0, ns = 0, nb = 0;
+    std::vector<int> members;
+};
+}  // namespace
+
+void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile &t,
+                       hipStream_t st) {
+    arena.release();
+    stream = st;
+    tile = t;
+    outputs = outs;
+    nb = int(specs.size());
+    const int nmax = t.nmax(), smax = t.smax();
+    const int brains_per_pack = std::min(kPackBrains, std::max(1, kPackMotors / std::max(1, outs)));
+
+    // 1. Which brains stay resident in a workgroup, which take the per-tick path.
+    std::vector<uint8_t> slow(nb, 0), eligible(nb, 0);
+    for (int i = 0; i < nb; ++i) {
+        const HostBrain &b = *specs[i].brain;
+        for (const auto &g : b.genes)
+            if (g.type == 2 || g.type == 3) slow[i] = 1;
+        const int nc = int(b.genes.size()), sc = int(b.edges.size());
+        eligible[i] = !slow[i] && nc <= nmax && sc <= smax && outs <= kPackMotors;
+    }
+    // 2. First-fit-decreasing packing of resident brains by cost.
+    std::vector<int> order;
+    for (int i = 0; i < nb; ++i)
+        if (eligible[i]) order.push_back(i);
+    auto cost = [&](int i) {
+        const HostBrain &b = *specs[i].brain;
+        return int64_t(b.genes.size()) * 24 + int64_t(b.edges.size());
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<Bin> bins;
+    const size_t window = 64;
+    size_t open_from = 0;
+    for (int i : order) {
+        const int nc = int(specs[i].brain->genes.size()), sc = int(specs[i].brain->edges.size());
+        bool placed = false;
+        for (size_t k = open_from; k < bins.size(); ++k) {
+            Bin &bin = bins[k];
+            if (bin.nn + nc <= nmax && bin.ns + sc <= smax && bin.nb < brains_per_pack) {
+                bin.nn += nc; bin.ns += sc; bin.nb += 1; bin.members.push_back(i);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) {
+            bins.emplace_back();
+            Bin &bin = bins.back();
+            bin.nn = nc; bin.ns = sc; bin.nb = 1; bin.members.push_back(i);
+            if (bins.size() - open_from > window) ++open_from;
+        }
+    }
+    // 3. Pool order: per-tick brains first, then bins.
+    std::vector<int> pool_order;
+    for (int i = 0; i < nb; ++i)
+        if (!eligible[i]) pool_order.push_back(i);
+    fallback_brains = int(pool_order.size());
+    for (const Bin &bin : bins)
+        for (int i : bin.members) pool_order.push_back(i);
+
+    // 4. Positions.
+    bn0.assign(nb, 0); bnc.assign(nb, 0); bs0.assign(nb, 0); bsc.assign(nb, 0);
+    batch_of.assign(nb, 0); pool_of.assign(nb, 0); fallback.assign(nb, 0);
+    nn = 0; ns = 0;
+    for (int pb = 0; pb < nb; ++pb) {
+        const int i = pool_order[pb];
+        const HostBrain &b = *specs[i].brain;
+        bn0[pb] = int32_t(nn); bnc[pb] = int32_t(b.genes.size());
+        bs0[pb] = int32_t(ns); bsc[pb] = int32_t(b.edges.size());
+        nn += bnc[pb]; ns += bsc[pb];
+        batch_of[pb] = specs[i].batch_index;
+        pool_of[specs[i].batch_index] = pb;
+        fallback[pb] = pb < fallback_brains;
+    }
+    if (nn > INT32_MAX || ns > INT32_MAX) throw std::runtime_error("Pool exceeds 2^31 elements");
+
+    // 5. Host structure-of-arrays images.
+    const int64_t N = std::max<int64_t>(nn, 1), S = std::max<int64_t>(ns, 1);
+    std::vector<float> ng(NG_FIELDS * N, 0.0f), nx(NX_FIELDS * N, 0.0f);
+    std::vector<uint32_t> nflags(N, 0), norig(N, 0);
+    std::vector<int32_t> ninput(N, -1), nin(N + 1, 0), nbrain(N, 0), slow_pos;
+    std::vector<float> sg(SG_FIELDS * S, 0.0f);
+    std::vector<uint32_t> sflags(S, 0), sorig(S, 0);
+    std::vector<int32_t> spre(S, 0), spost(S, 0), sbrain(S, 0), motor(int64_t(nb) * outs, 0);
+    std::vector<int32_t> bout0(nb, 0);
+    syn_perm.assign(S, 0);
+    for (int pb = 0; pb < nb; ++pb) {
+        const BrainSpec &spec = specs[pool_order[pb]];
+        const HostBrain &b = *spec.brain;
+        const int64_t n0 = bn0[pb], s0 = bs0[pb];
+        const int nc = bnc[pb], sc = bsc[pb];
+        bout0[pb] = spec.output_base;
+        for (int j = 0; j < outs; ++j) motor[int64_t(pb) * outs + j] = int32_t(n0 + b.motors[j]);
+        for (int i = 0; i < nc; ++i) {
+            const ns_neuron_gene &g = b.genes[i];
+            const int64_t n = n0 + i;
+            const float f[NG_FIELDS] = {g.capacitance_pf, g.leak_ns, g.rest_mv, g.threshold_mv,
+                                        g.slope_mv, g.adapt_ns, g.adapt_tau_ms, g.spike_adapt_pa,
+                                        g.reset_mv, g.afterdepol_spike_pa, g.afterdepol_tau_ms,
+                                        g.receptor_baseline_hz, g.receptor_gain_hz,
+                                        g.receptor_event_ns, g.receptor_decay_ms};
+            for (int k = 0; k < NG_FIELDS; ++k) ng[k * N + n] = f[k];
+            for (int k = 0; k < 4; ++k) nx[(NX_LP + k) * N + n] = g.lp_scale[k];
+            for (int k = 0; k < 8; ++k) nx[(NX_PRINZ + k) * N + n] = g.prinz_g[k];
+            nx[NX_AMBIENT * N + n] = g.prinz_ambient_leak_ns;
+            nflags[n] = uint32_t(g.type) | (g.active ? NF_ACTIVE : 0) |
+                        (b.receptor[i] ? NF_RECEPTOR : 0) | (spec.world ? NF_WORLD : 0);
+            ninput[n] = b.sensory[i] >= 0 ? spec.input_base + b.sensory[i] : -1;
+            nbrain[n] = pb;
+            norig[n] = b.orig_n.empty() ? uint32_t(i) : b.orig_n[i];
+            if (g.type == 2 || g.type == 3) slow_pos.push_back(int32_t(n));
+        }
+        // synapses sorted by (post, edge index): each neuron's incoming run is
+        // contiguous and in the reference's accumulation order
+        std::vector<int> perm(sc);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int a, int c) { return b.edges[a].post < b.edges[c].post; });
+        std::vector<int> incoming(nc + 1, 0);
+        for (int j = 0; j < sc; ++j) incoming[b.edges[j].post + 1]++;
+        for (int i = 0; i < nc; ++i) incoming[i + 1] += incoming[i];
+        for (int i = 0; i < nc; ++i) nin[n0 + i] = int32_t(s0 + incoming[i]);
+        for (int k = 0; k < sc; ++k) {
+            const int j = perm[k];
+            const ns_synapse_gene &g = b.edges[j];
+            const int64_t q = s0 + k;
+            const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
+            const float f[SG_FIELDS] = {g.initial_weight_ns, g.utilization, g.recovery_ms,
+                                        g.facilitation_ms, g.slow_recovery_ms, g.slow_depletion};
+            for (int m = 0; m < SG_FIELDS; ++m) sg[m * S + q] = f[m];
+            uint32_t route = uint32_t(g.branch & 3);
+            if (a.modulator) route |= uint32_t(a.modulator & 3) << R_MOD_SHIFT;
+            else if (a.inhibitory_transmitter) route |= R_INH;
+            if (g.active && a.active && z.active) route |= R_VALID;
+            if (!a.inhibitory_transmitter || a.modulator) route |= R_STP;
+            if (!a.inhibitory_transmitter && excitatory_cell(a.type) && !a.modulator &&
+                !z.inhibitory_transmitter && excitatory_cell(z.type))
+                route |= R_PAIR;
+            sflags[q] = uint32_t(g.delay) | (uint32_t(g.plasticity & 7) << SF_PLAST_SHIFT) |
+                        (uint32_t(g.mod_receptors & 7) << SF_MODREC_SHIFT) |
+                        (uint32_t(site_count(g)) << SF_SITES_SHIFT) | (route << SF_ROUTE_SHIFT);
+            spre[q] = int32_t(n0 + g.pre);
+            spost[q] = int32_t(n0 + g.post);
+            sbrain[q] = pb;
+            sorig[q] = b.orig_s.empty() ? uint32_t(j) : b.orig_s[j];
+            syn_perm[q] = j;
+        }
+    }
+    nin[nn] = int32_t(ns);
+    for (int64_t n = nn; n < N; ++n) nin[n + 1] = int32_t(ns);
+
+    // 6. Device arrays.
+    Pool p{};
+    p.ncap = N;
+    p.scap = S;
+    auto up = [&](auto *dst, const auto &src) {
+        NS_HIP(hipMemcpy(dst, src.data(), sizeof(src[0]) * src.size(), hipMemcpyHostToDevice));
+    };
+    float *d_ng = arena.alloc<float>(NG_FIELDS * N); up(d_ng, ng); p.ng = d_ng;
+    float *d_nx = arena.alloc<float>(NX_FIELDS * N); up(d_nx, nx); p.nx = d_nx;
+    uint32_t *d_nflags = arena.alloc<uint32_t>(N); up(d_nflags, nflags); p.nflags = d_nflags;
+    int32_t *d_ninput = arena.alloc<int32_t>(N); up(d_ninput, ninput); p.ninput = d_ninput;
+    int32_t *d_nin = arena.alloc<int32_t>(N + 1); up(d_nin, nin); p.nin = d_nin;
+    int32_t *d_nbrain = arena.alloc<int32_t>(N); up(d_nbrain, nbrain); p.nbrain = d_nbrain;
+    p.ns = arena.alloc<float>(NS_FIELDS * N);
+    p.cell = arena.alloc<float>(kCellFields * N);
+    p.total = arena.alloc<int32_t>(N);
+    p.rrng = arena.alloc<uint32_t>(N);
+    p.hist = arena.alloc<uint32_t>(N);
+    p.spike = arena.alloc<uint8_t>(N);
+    p.modbits = arena.alloc<uint8_t>(N);
+    p.stamps = arena.alloc<uint64_t>(6 * N);
+    float *d_sg = arena.alloc<float>(SG_FIELDS * S); up(d_sg, sg); p.sg = d_sg;
+    uint32_t *d_sflags = arena.alloc<uint32_t>(S); up(d_sflags, sflags); p.sflags = d_sflags;
+    int32_t *d_spre = arena.alloc<int32_t>(S); up(d_spre, spre); p.spre = d_spre;
+    int32_t *d_spost = arena.alloc<int32_t>(S); up(d_spost, spost); p.spost = d_spost;
+    int32_t *d_sbrain = arena.alloc<int32_t>(S); up(d_sbrain, sbrain); p.sbrain = d_sbrain;
+    p.ss = arena.alloc<float>(SS_FIELDS * S);
+    p.last = arena.alloc<int64_t>(S);
+    p.ready = arena.alloc<uint64_t>(S);
+    p.srng = arena.alloc<uint32_t>(S);
+    p.arrived = arena.alloc<uint8_t>(S);
+    p.released = arena.alloc<float>(S);
+    const int B = std::max(nb, 1);
+    int32_t *d_bn0 = arena.alloc<int32_t>(B); up(d_bn0, bn0); p.bn0 = d_bn0;
+    int32_t *d_bnc = arena.alloc<int32_t>(B); up(d_bnc, bnc); p.bnc = d_bnc;
+    int32_t *d_bs0 = arena.alloc<int32_t>(B); up(d_bs0, bs0); p.bs0 = d_bs0;
+    int32_t *d_bsc = arena.alloc<int32_t>(B); up(d_bsc, bsc); p.bsc = d_bsc;
+    p.bin0 = nullptr;
+    int32_t *d_bout0 = arena.alloc<int32_t>(B); up(d_bout0, bout0); p.bout0 = d_bout0;
+    p.btick = arena.alloc<int64_t>(B);
+    p.brun = arena.alloc<uint8_t>(B);
+    p.bfrozen = arena.alloc<uint8_t>(B);
+    p.bstatus = arena.alloc<int32_t>(B);
+    int32_t *d_motor = arena.alloc<int32_t>(int64_t(B) * std::max(outs, 1));
+    up(d_motor, motor);
+    p.motor = d_motor;
+    p.outputs = outs;
+    p.events = arena.alloc<unsigned long long>(1);
+    NS_HIP(hipMemset(p.btick, 0, sizeof(int64_t) * B));
+    NS_HIP(hipMemset(p.brun, 0, B));
+    NS_HIP(hipMemset(p.bfrozen, 0, B));
+    NS_HIP(hipMemset(p.bstatus, 0, sizeof(int32_t) * B));
+    NS_HIP(hipMemset(p.events, 0, sizeof(unsigned long long)));
+    NS_HIP(hipMemset(p.hist, 0, sizeof(uint32_t) * N));
+    NS_HIP(hipMemset(p.stamps, 0, sizeof(uint64_t) * 6 * N));
+    NS_HIP(hipMemset(p.released, 0, sizeof(float) * S));
+    NS_HIP(hipMemset(p.modbits, 0, N));
+    d_norig = arena.alloc<uint32_t>(N); up(d_norig, norig);
+    d_sorig = arena.alloc<uint32_t>(S); up(d_sorig, sorig);
+    d_bseed = arena.alloc<uint32_t>(B);
+    NS_HIP(hipMemset(d_bseed, 0, sizeof(uint32_t) * B));
+    slow_count = int(slow_pos.size());
+    d_slow = arena.alloc<int32_t>(std::max(slow_count, 1));
+    if (slow_count) up(d_slow, slow_pos);
+    view = p;
+
+    // 7. Packs and per-tick ranges.
+    packs.clear();
+    int pb = fallback_brains;
+    for (const Bin &bin : bins) {
+        Pack pk{};
+        pk.b0 = pb;
+        pk.nb = bin.nb;
+        pk.n0 = bn0[pb];
+        pk.s0 = bs0[pb];
+        pk.nn = bin.nn;
+        pk.ns = bin.ns;
+        pk.nm = bin.nb * outs;
+        packs.push_back(pk);
+        pb += bin.nb;
+    }
+    d_packs = arena.alloc<Pack>(std::max<size_t>(packs.size(), 1));
+    if (!packs.empty()) up(d_packs, packs);
+    const int64_t fb_n = fallback_brains ? int64_t(bn0[fallback_brains - 1]) + bnc[fallback_brains - 1] : 0;
+    const int64_t fb_s = fallback_brains ? int64_t(bs0[fallback_brains - 1]) + bsc[fallback_brains - 1] : 0;
+    fallback_range = PlainRange{0, fb_n, 0, fb_s, nb, fallback_brains * outs, d_slow, slow_count};
+    all_range = PlainRange{0, nn, 0, ns, nb, nb * outs, d_slow, slow_count};
+}
+
+// Neuron/synapse state in batch order, genome order within each brain.
+void DevicePool::download_state(std::vector<ns_neuron_state> &out_n,
+                                std::vector<ns_synapse_state> &out_s, std::vector<uint32_t> &hist,
+                                std::vector<int64_t> &ticks, std::vector<uint64_t> *stamps) {
+    const int64_t N = view.ncap, S = view.scap;
+    std::vector<float> fns(NS_FIELDS * N), cell(kCellFields * N), fss(SS_FIELDS * S);
+    std::vector<int32_t> total(N);
+    std::vector<uint32_t> rrng(N), h(N), srng(S);
+    std::vector<uint8_t> spike(N), arrived(S);
+    std::vector<int64_t> last(S), btick(std::max(nb, 1));
+    std::vector<uint64_t> ready(S), st(6 * N);
+    auto down = [&](auto &dst, const auto *src) {
+        NS_HIP(hipMemcpy(dst.data(), src, sizeof(dst[0]) * dst.size(), hipMemcpyDeviceToHost));
+    };
+    NS_HIP(hipStreamSynchronize(stream));
+    down(fns, view.ns); down(cell, view.cell); down(fss, view.ss); down(total, view.total);
+    down(rrng, view.rrng); down(h, view.hist); down(srng, view.srng); down(spike, view.spike);
+    down(arrived, view.arrived); down(last, view.last); down(btick, view.btick);
+    down(ready, view.ready);
+    if (stamps) down(st, view.stamps);
+    // batch-order offsets
+    std::vector<int64_t> noff(nb + 1, 0), soff(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) {
+        noff[i + 1] = noff[i] + bnc[pool_of[i]];
+        soff[i + 1] = soff[i] + bsc[pool_of[i]];
+    }
+    out_n.assign(noff[nb], ns_neuron_state{});
+    out_s.assign(soff[nb], ns_synapse_state{});
+    hist.assign(noff[nb], 0);
+    ticks.assign(nb, 0);
+    if (stamps) stamps->assign(noff[nb] * 6, 0);
+    for (int i = 0; i < nb; ++i) {
+        const int pb = pool_of[i];
+        ticks[i] = btick[pb];
+        for (int k = 0; k < bnc[pb]; ++k) {
+            const int64_t n = int64_t(bn0[pb]) + k;
+            ns_neuron_state &r = out_n[noff[i] + k];
+            std::memset(&r, 0, sizeof r);
+            r.voltage_mv = fns[NS_V * N + n];
+            r.adaptation_pa = fns[NS_W * N + n];
+            r.afterdepol_pa = fns[NS_ADP * N + n];
+            r.inh_ns = fns[NS_INH * N + n];
+            r.recent_depol_mv = fns[NS_DEPOL * N + n];
+            r.log_ampa_scale = fns[NS_LAS * N + n];
+            for (int d = 0; d < 4; ++d) r.dendrite_exc_ns[d] = fns[(NS_D0 + d) * N + n];
+            for (int c = 0; c < kCellFields; ++c) r.cell[c] = cell[c * N + n];
+            r.total_spikes = total[n];
+            r.spike = spike[n];
+            r.receptor_ns = fns[NS_RNS * N + n];
+            r.receptor_rng = rrng[n];
+            hist[noff[i] + k] = h[n];
+            if (stamps)
+                for (int e = 0; e < 6; ++e) (*stamps)[(noff[i] + k) * 6 + e] = st[e * N + n];
+        }
+        for (int k = 0; k < bsc[pb]; ++k) {
+            const int64_t q = int64_t(bs0[pb]) + k;
+            ns_synapse_state &r = out_s[soff[i] + syn_perm[q]];
+            std::memset(&r, 0, sizeof r);
+            r.weight_ns = fss[SS_WEIGHT * S + q];
+            r.running_utilization = fss[SS_UTIL * S + q];
+            r.pre_trace = fss[SS_PTRACE * S + q];
+            r.post_trace = fss[SS_QTRACE * S + q];
+            r.eligible_ltp = fss[SS_ELTP * S + q];
+            r.eligible_ltd = fss[SS_ELTD * S + q];
+            r.da_age_ms = fss[SS_DA_AGE * S + q];
+            r.spine_volume = fss[SS_SPINE * S + q];
+            r.structural_age_ms = fss[SS_SAGE * S + q];
+            r.pre_age_ms = fss[SS_PRE_AGE * S + q];
+            r.post_age_ms = fss[SS_POST_AGE * S + q];
+            r.last_arrival_tick = last[q];
+            r.ready_mask = ready[q];
+            r.rng_state = srng[q];
+            r.arrived = arrived[q];
+            r.slow_resource = fss[SS_SLOW * S + q];
+        }
+    }
+}
+
+void DevicePool::upload_state(const std::vector<ns_neuron_state> &in_n,
+                              const std::vector<ns_synapse_state> &in_s,
+                              const std::vector<uint32_t> &hist, const std::vector<int64_t> &ticks,
+                              const std::vector<uint64_t> *stamps) {
+    const int64_t N = view.ncap, S = view.scap;
+    std::vector<float> fns(NS_FIELDS * N, 0.0f), cell(kCellFields * N, 0.0f), fss(SS_FIELDS * S, 0.0f);
+    std::vector<int32_t> total(N, 0);
+    std::vector<uint32_t> rrng(N, 0), h(N, 0), srng(S, 0);
+    std::vector<uint8_t> spike(N, 0), arrived(S, 0);
+    std::vector<int64_t> last(S, 0), btick(std::max(nb, 1), 0);
+    std::vector<uint64_t> ready(S, 0), st(6 * N, 0);
+    std::vector<int64_t> noff(nb + 1, 0), soff(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) {
+        noff[i + 1] = noff[i] + bnc[pool_of[i]];
+        soff[i + 1] = soff[i] + bsc[pool_of[i]];
+    }
+    if (int64_t(in_n.size()) != noff[nb] || int64_t(in_s.size()) != soff[nb] ||
+        int64_t(hist.size()) != noff[nb] || int(ticks.size()) != nb)
+        throw std::runtime_error("State dimensions differ");
+    for (int i = 0; i < nb; ++i) {
+        const int pb = pool_of[i];
+        btick[pb] = ticks[i];
+        for (int k = 0; k < bnc[pb]; ++k) {
+            const int64_t n = int64_t(bn0[pb]) + k;
+            const ns_neuron_state &r = in_n[noff[i] + k];
+            fns[NS_V * N + n] = r.voltage_mv;
+            fns[NS_W * N + n] = r.adaptation_pa;
+            fns[NS_ADP * N + n] = r.afterdepol_pa;
+            fns[NS_INH * N + n] = r.inh_ns;
+            fns[NS_DEPOL * N + n] = r.recent_depol_mv;
+            fns[NS_LAS * N + n] = r.log_ampa_scale;
+            for (int d = 0; d < 4; ++d) fns[(NS_D0 + d) * N + n] = r.dendrite_exc_ns[d];
+            for (int c = 0; c < kCellFields; ++c) cell[c * N + n] = r.cell[c];
+            total[n] = r.total_spikes;
+            spike[n] = r.spike;
+            fns[NS_RNS * N + n] = r.receptor_ns;
+            rrng[n] = r.receptor_rng;
+            h[n] = hist[noff[i] + k];
+            if (stamps)
+                for (int e = 0; e < 6; ++e) st[e * N + n] = (*stamps)[(noff[i] + k) * 6 + e];
+        }
+        for (int k = 0; k < bsc[pb]; ++k) {
+            const int64_t q = int64_t(bs0[pb]) + k;
+            const ns_synapse_state &r = in_s[soff[i] + syn_perm[q]];
+            fss[SS_WEIGHT * S + q] = r.weight_ns;
+            fss[SS_UTIL * S + q] = r.running_utilization;
+            fss[SS_PTRACE * S + q] = r.pre_trace;
+            fss[SS_QTRACE * S + q] = r.post_trace;
+            fss[SS_ELTP * S + q] = r.eligible_ltp;
+            fss[SS_ELTD * S + q] = r.eligible_ltd;
+            fss[SS_DA_AGE * S + q] = r.da_age_ms;
+            fss[SS_SPINE * S + q] = r.spine_volume;
+            fss[SS_SAGE * S + q] = r.structural_age_ms;
+            fss[SS_PRE_AGE * S + q] = r.pre_age_ms;
+            fss[SS_POST_AGE * S + q] = r.post_age_ms;
+            last[q] = r.last_arrival_tick;
+            ready[q] = r.ready_mask;
+            srng[q] = r.rng_state;
+            arrived[q] = r.arrived;
+            fss[SS_SLOW * S + q] = r.slow_resource;
+        }
+    }
+    auto up = [&](auto *dst, const auto &src) {
+        NS_HIP(hipMemcpy(dst, src.data(), sizeof(src[0]) * src.size(), hipMemcpyHostToDevice));
+    };
+    NS_HIP(hipStreamSynchronize(stream));
+    up(view.ns, fns); up(view.cell, cell); up(view.ss, fss); up(view.total, total);
+    up(view.rrng, rrng); up(view.hist, h); up(view.srng, srng); up(view.spike, spike);
+    up(view.arrived, arrived); up(view.last, last); up(view.ready, ready);
+    NS_HIP(hipMemcpy(view.btick, btick.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
+    if (stamps) up(view.stamps, st);
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+static Tile default_tile(const std::vector<HostBrain> &brains) {
+    // Pick the smallest standard tile whose pack holds the typical (90th
+    // percentile) brain; ns_gpu_autotune refines this by measurement.
+    std::vector<int> n, s;
+    for (const auto &b : brains) { n.push_back(int(b.genes.size())); s.push_back(int(b.edges.size())); }
+    std::sort(n.begin(), n.end());
+    std::sort(s.begin(), s.end());
+    const int n90 = n[n.size() * 9 / 10], s90 = s[s.size() * 9 / 10];
+    for (int k = 0; k < kNumTiles; ++k) {
+        const TileShape &t = kTiles[k];
+        if (t.threads * t.neurons_per_thread >= 4 * n90 && t.threads * t.synapses_per_thread >= 4 * s90)
+            return Tile{t.threads, t.neurons_per_thread, t.synapses_per_thread};
+    }
+    const TileShape &t = kTiles[kNumTiles - 2];
+    return Tile{t.threads, t.neurons_per_thread, t.synapses_per_thread};
+}
+
+void Batch::create(std::vector<HostBrain> &&owned, int) {
+    brains = std::move(owned);
+    count = int(brains.size());
+    channels = brains[0].channels;
+    outputs = int(brains[0].motors.size());
+    for (const auto &b : brains)
+        if (b.channels != channels || int(b.motors.size()) != outputs)
+            throw std::runtime_error("Batch ports differ");
+    NS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    specs.clear();
+    for (int i = 0; i < count; ++i)
+        specs.push_back(BrainSpec{&brains[i], i, i * channels, i * outputs, world});
+    pool.build(specs, outputs, default_tile(brains), stream);
+    d_input = io.alloc<float>(int64_t(count) * channels);
+    d_output = io.alloc<float>(int64_t(count) * outputs);
+    h_brun.assign(count, 0);
+    h_status.assign(count, 0);
+}
+
+void Batch::relayout(const Tile &tile) {
+    std::vector<ns_neuron_state> n;
+    std::vector<ns_synapse_state> s;
+    std::vector<uint32_t> h;
+    std::vector<int64_t> ticks;
+    std::vector<uint64_t> stamps;
+    pool.download_state(n, s, h, ticks, &stamps);
+    pool.build(specs, outputs, tile, stream);
+    pool.upload_state(n, s, h, ticks, &stamps);
+    brun_valid = false;
+}
+
+// Uploads the per-brain run flags only when they change; the upload waits
+// for earlier launches that may still read them.
+void Batch::set_run(const uint8_t *mask) {
+    std::vector<uint8_t> next(count);
+    for (int pb = 0; pb < count; ++pb)
+        next[pb] = uint8_t((mask[pool.batch_of[pb]] ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
+    if (brun_valid && next == h_brun) return;
+    NS_HIP(hipDeviceSynchronize());
+    h_brun = next;
+    NS_HIP(hipMemcpy(pool.view.brun, h_brun.data(), count, hipMemcpyHostToDevice));
+    brun_valid = true;
+}
+
+void Batch::run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
+                hipStream_t st) {
+    launch_fill(d_out, int64_t(count) * outputs, 0.0f, st);
+    launch_fill_i32(pool.view.bstatus, count, 0, st);
+    if (form == NS_FORM_PER_TICK) {
+        launch_per_tick(pool.view, pool.all_range, ticks, 1, d_in, gain, d_out, homeostasis, st);
+        return;
+    }
+    if (!pool.packs.empty()) {
+        const TileShape *shape = nullptr;
+        for (int k = 0; k < kNumTiles; ++k)
+            if (kTiles[k].threads == pool.tile.threads &&
+                kTiles[k].neurons_per_thread == pool.tile.kn &&
+                kTiles[k].synapses_per_thread == pool.tile.ks)
+                shape = &kTiles[k];
+        if (!shape) throw std::runtime_error("Unknown tile shape");
+        NS_HIP(shape->launch(pool.view, pool.d_packs, int(pool.packs.size()), ticks, d_in, gain,
+                             d_out, homeostasis, st));
+    }
+    if (pool.fallback_brains)
+        launch_per_tick(pool.view, pool.fallback_range, ticks, 3, d_in, gain, d_out, homeostasis, st);
+}
+
+Batch::~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    pool.arena.release();
+    io.release();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace nsa
+
+using namespace nsa;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *ns_error(void) { return g_error.c_str(); }
+
+void *ns_create(int count, int edges, int channels, int outputs, const float *ng, const float *sg,
+                const int *motors) {
+    try {
+        return parse_gym_brain(count, edges, channels, outputs, ng, sg, motors, 8);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void *ns_create_ex(int count, int edges, int channels, int outputs, const float *ng,
+                   const float *sg, const int *motors, int max_delay) {
+    try {
+        return parse_gym_brain(count, edges, channels, outputs, ng, sg, motors, max_delay);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void ns_original_indices(void *p, const uint32_t *neurons, const uint32_t *synapses) {
+    auto &b = *static_cast<HostBrain *>(p);
+    b.orig_n.assign(neurons, neurons + b.genes.size());
+    if (!b.edges.empty()) b.orig_s.assign(synapses, synapses + b.edges.size());
+}
+
+void ns_destroy(void *p) { delete static_cast<HostBrain *>(p); }
+
+static void *make_batch(std::vector<HostBrain> &&brains, bool world) {
+    auto *batch = new Batch;
+    try {
+        batch->world = world;
+        batch->create(std::move(brains), 0);
+    } catch (...) {
+        delete batch;
+        throw;
+    }
+    return batch;
+}
+
+void *ns_gpu_create(void **handles, int count) {
+    try {
+        if (count < 1) throw std::runtime_error("Empty GPU batch");
+        std::vector<HostBrain> brains;
+        brains.reserve(count);
+        for (int i = 0; i < count; ++i) brains.push_back(*static_cast<HostBrain *>(handles[i]));
+        return make_batch(std::move(brains), false);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void *ns_gpu_create_bulk(int count, const int *neuron_counts, const int *edge_counts, int channels,
+                         int outputs, const float *ng, const float *sg, const int *motors,
+                         int max_delay, int world_rules) {
+    try {
+        if (count < 1) throw std::runtime_error("Empty GPU batch");
+        std::vector<HostBrain> brains;
+        brains.reserve(count);
+        int64_t n0 = 0, s0 = 0;
+        for (int i = 0; i < count; ++i) {
+            std::unique_ptr<HostBrain> b(parse_gym_brain(neuron_counts[i], edge_counts[i], channels,
+                                                         outputs, ng + n0 * 10, sg + s0 * 11,
+                                                         motors + int64_t(i) * outputs, max_delay));
+            if (world_rules) {
+                // src/main.cu:729-737: 230 pA per level, receptor events only on
+                // channels below 6 (gene sensory channel = input channel).
+                for (size_t k = 0; k < b->genes.size(); ++k) {
+                    b->genes[k].sensory_channel = int8_t(b->sensory[k]);
+                    b->receptor[k] = b->genes[k].type == 5 && b->sensory[k] >= 0 && b->sensory[k] < 6;
+                }
+            }
+            brains.push_back(std::move(*b));
+            n0 += neuron_counts[i];
+            s0 += edge_counts[i];
+        }
+        return make_batch(std::move(brains), world_rules != 0);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void ns_gpu_destroy(void *p) { delete static_cast<Batch *>(p); }
+
+int ns_gpu_reset(void *p, const uint8_t *mask, const uint32_t *seeds) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        std::vector<uint32_t> bseed(b.count);
+        for (int pb = 0; pb < b.count; ++pb) bseed[pb] = seeds[b.pool.batch_of[pb]];
+        b.set_run(mask);
+        NS_HIP(hipMemcpyAsync(b.pool.d_bseed, bseed.data(), sizeof(uint32_t) * b.count,
+                              hipMemcpyHostToDevice, b.stream));
+        launch_reset(b.pool.view, b.pool.nn, b.pool.ns, b.count, b.pool.d_bseed, b.pool.d_norig,
+                     b.pool.d_sorig, b.stream);
+        NS_HIP(hipGetLastError());
+        NS_HIP(hipStreamSynchronize(b.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_step(void *p, const float *input, const uint8_t *mask, int ticks, float gain,
+                float *output, int *status) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        if (ticks < 1 || ticks > 10000 || !std::isfinite(gain) || gain < 0)
+            throw std::runtime_error("Invalid neural integration interval");
+        for (int i = 0; i < b.count; ++i)
+            if (mask[i])
+                for (int j = 0; j < b.channels; ++j) {
+                    float v = input[size_t(i) * b.channels + j];
+                    if (!std::isfinite(v) || v < 0 || v > 1)
+                        throw std::runtime_error("Invalid encoded observation");
+                }
+        NS_HIP(hipMemcpyAsync(b.d_input, input, sizeof(float) * b.count * b.channels,
+                              hipMemcpyHostToDevice, b.stream));
+        b.set_run(mask);
+        b.run(ticks, b.d_input, gain, b.d_output, b.world ? 0 : 1, b.stream);
+        NS_HIP(hipGetLastError());
+        NS_HIP(hipMemcpyAsync(output, b.d_output, sizeof(float) * b.count * b.outputs,
+                              hipMemcpyDeviceToHost, b.stream));
+        NS_HIP(hipMemcpyAsync(b.h_status.data(), b.pool.view.bstatus, sizeof(int32_t) * b.count,
+                              hipMemcpyDeviceToHost, b.stream));
+        NS_HIP(hipStreamSynchronize(b.stream));
+        for (int pb = 0; pb < b.count; ++pb) status[b.pool.batch_of[pb]] = b.h_status[pb];
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_step_device(void *p, const float *input, const uint8_t *mask, int ticks, float gain,
+                       float *output, int *status, void *stream) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        if (ticks < 1 || ticks > 10000) throw std::runtime_error("Invalid neural integration interval");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b.stream;
+        if (mask) {
+            std::vector<uint8_t> h(b.count);
+            NS_HIP(hipMemcpy(h.data(), mask, b.count, hipMemcpyDeviceToHost));
+            b.set_run(h.data());
+        } else {
+            std::vector<uint8_t> ones(b.count, 1);
+            b.set_run(ones.data());
+        }
+        b.run(ticks, input, gain, output, b.world ? 0 : 1, st);
+        NS_HIP(hipGetLastError());
+        if (status) {
+            NS_HIP(hipStreamSynchronize(st));
+            NS_HIP(hipMemcpy(b.h_status.data(), b.pool.view.bstatus, sizeof(int32_t) * b.count,
+                             hipMemcpyDeviceToHost));
+            std::vector<int32_t> s(b.count);
+            for (int pb = 0; pb < b.count; ++pb) s[b.pool.batch_of[pb]] = b.h_status[pb];
+            NS_HIP(hipMemcpy(status, s.data(), sizeof(int32_t) * b.count, hipMemcpyHostToDevice));
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_set_form(void *p, int form) {
+    auto &b = *static_cast<Batch *>(p);
+    if (form != NS_FORM_RESIDENT && form != NS_FORM_PER_TICK) {
+        set_error("Invalid execution form");
+        return -1;
+    }
+    b.form = form;
+    return 0;
+}
+
+int ns_gpu_set_tiles(void *p, int threads, int kn, int ks) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        bool known = false;
+        for (int k = 0; k < kNumTiles; ++k)
+            known |= kTiles[k].threads == threads && kTiles[k].neurons_per_thread == kn &&
+                     kTiles[k].synapses_per_thread == ks;
+        if (!known) throw std::runtime_error("Tile shape not compiled");
+        b.relayout(Tile{threads, kn, ks});
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        std::vector<ns_neuron_state> n;
+        std::vector<ns_synapse_state> s;
+        std::vector<uint32_t> h;
+        std::vector<int64_t> tk;
+        std::vector<uint64_t> stamps;
+        b.pool.download_state(n, s, h, tk, &stamps);
+        unsigned long long events0 = 0;
+        NS_HIP(hipMemcpy(&events0, b.pool.view.events, sizeof events0, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> ones(b.count, 1);
+        std::vector<float> zeros(size_t(b.count) * b.channels, 0.0f);
+        b.brun_valid = false;
+        NS_HIP(hipMemcpy(b.d_input, zeros.data(), sizeof(float) * zeros.size(), hipMemcpyHostToDevice));
+        double best = 1e30;
+        Tile best_tile = b.pool.tile;
+        hipEvent_t e0, e1;
+        NS_HIP(hipEventCreate(&e0));
+        NS_HIP(hipEventCreate(&e1));
+        for (int k = 0; k < kNumTiles; ++k) {
+            Tile t{kTiles[k].threads, kTiles[k].neurons_per_thread, kTiles[k].synapses_per_thread};
+            b.pool.build(b.specs, b.outputs, t, b.stream);
+            b.pool.upload_state(n, s, h, tk, &stamps);
+            b.brun_valid = false;
+            b.set_run(ones.data());
+            b.run(ticks, b.d_input, 0.0f, b.d_output, 1, b.stream);   // warm-up
+            b.pool.upload_state(n, s, h, tk, &stamps);
+            NS_HIP(hipEventRecord(e0, b.stream));
+            b.run(ticks, b.d_input, 0.0f, b.d_output, 1, b.stream);
+            NS_HIP(hipEventRecord(e1, b.stream));
+            NS_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            NS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_tile = t; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b.pool.build(b.specs, b.outputs, best_tile, b.stream);
+        b.pool.upload_state(n, s, h, tk, &stamps);
+        b.brun_valid = false;
+        NS_HIP(hipMemcpy(b.pool.view.events, &events0, sizeof events0, hipMemcpyHostToDevice));
+        if (threads) *threads = best_tile.threads;
+        if (kn) *kn = best_tile.kn;
+        if (ks) *ks = best_tile.ks;
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_export(void *p, ns_neuron_state *neurons, ns_synapse_state *synapses, uint8_t *history9,
+                  int64_t *ticks) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        std::vector<ns_neuron_state> n;
+        std::vector<ns_synapse_state> s;
+        std::vector<uint32_t> h;
+        std::vector<int64_t> tk;
+        b.pool.download_state(n, s, h, tk);
+        if (neurons) std::memcpy(neurons, n.data(), n.size() * sizeof(n[0]));
+        if (synapses && !s.empty()) std::memcpy(synapses, s.data(), s.size() * sizeof(s[0]));
+        if (ticks) std::memcpy(ticks, tk.data(), tk.size() * sizeof(tk[0]));
+        if (history9) {
+            int64_t off = 0;
+            for (int i = 0; i < b.count; ++i) {
+                const int64_t T = tk[i];
+                const int nc = b.pool.bnc[b.pool.pool_of[i]];
+                for (int k = 0; k < nc; ++k) {
+                    const uint32_t bits = h[off + k];
+                    for (int slot = 0; slot < 9; ++slot) {
+                        const int64_t last = T - 1;
+                        const int64_t back = ((last - slot) % 9 + 9) % 9;
+                        const int64_t tau = last - back;
+                        history9[(off + k) * 9 + slot] = tau < 0 ? 0 : uint8_t((bits >> back) & 1u);
+                    }
+                }
+                off += nc;
+            }
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_import(void *p, const ns_neuron_state *neurons, const ns_synapse_state *synapses,
+                  const uint8_t *history9, const int64_t *ticks) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        std::vector<ns_neuron_state> n;
+        std::vector<ns_synapse_state> s;
+        std::vector<uint32_t> h;
+        std::vector<int64_t> tk;
+        b.pool.download_state(n, s, h, tk);
+        if (neurons) std::memcpy(n.data(), neurons, n.size() * sizeof(n[0]));
+        if (synapses && !s.empty()) std::memcpy(s.data(), synapses, s.size() * sizeof(s[0]));
+        if (ticks) std::memcpy(tk.data(), ticks, tk.size() * sizeof(tk[0]));
+        if (history9) {
+            int64_t off = 0;
+            for (int i = 0; i < b.count; ++i) {
+                const int64_t T = tk[i];
+                const int nc = b.pool.bnc[b.pool.pool_of[i]];
+                for (int k = 0; k < nc; ++k) {
+                    uint32_t bits = 0;
+                    for (int j = 0; j < 9; ++j) {
+                        const int64_t tau = T - 1 - j;
+                        if (tau >= 0 && history9[(off + k) * 9 + tau % 9]) bits |= 1u << j;
+                    }
+                    h[off + k] = bits;
+                }
+                off += nc;
+            }
+        }
+        b.pool.upload_state(n, s, h, tk);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_export_spike_bits(void *p, uint32_t *bits) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        std::vector<ns_neuron_state> n;
+        std::vector<ns_synapse_state> s;
+        std::vector<uint32_t> h;
+        std::vector<int64_t> tk;
+        b.pool.download_state(n, s, h, tk);
+        std::memcpy(bits, h.data(), h.size() * sizeof(uint32_t));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_info(void *p, ns_batch_info *info) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        unsigned long long events = 0;
+        NS_HIP(hipStreamSynchronize(b.stream));
+        NS_HIP(hipMemcpy(&events, b.pool.view.events, sizeof events, hipMemcpyDeviceToHost));
+        info->brains = b.count;
+        info->neurons = b.pool.nn;
+        info->synapses = b.pool.ns;
+        info->resident_packs = int64_t(b.pool.packs.size());
+        info->fallback_brains = b.pool.fallback_brains;
+        info->events = int64_t(events);
+        info->threads = b.pool.tile.threads;
+        info->neurons_per_thread = b.pool.tile.kn;
+        info->synapses_per_thread = b.pool.tile.ks;
+        info->form = b.form;
+        info->device_bytes = b.pool.arena.bytes + b.io.bytes;
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Host-side structures of libns_amd.so: genome handles, the device pool and
+// its builder.  Shared by the batch, world and serving surfaces.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ns_amd.h"
+#include "ns_kernels.h"
+
+namespace nsa {
+
+void set_error(const std::string &msg);
+
+#define NS_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Genome handle made by ns_create (gym_adapter/native.cpp:23-37, minus state).
+struct HostBrain {
+    std::vector<ns_neuron_gene> genes;
+    std::vector<ns_synapse_gene> edges;   // pre/post brain-local
+    std::vector<int32_t> sensory;         // input channel or -1
+    std::vector<uint8_t> receptor;        // receptor-event sensory path
+    std::vector<int32_t> motors;
+    std::vector<uint32_t> orig_n, orig_s;
+    int channels = 1;
+    int max_delay = 8;
+};
+
+ns_neuron_gene profile_gene(int type);
+HostBrain *parse_gym_brain(int count, int edges, int channels, int outputs, const float *ng,
+                           const float *sg, const int *motors, int max_delay);
+
+// Owning list of device allocations.
+struct DeviceArena {
+    std::vector<void *> blocks;
+    int64_t bytes = 0;
+    template <class T> T *alloc(int64_t count) {
+        if (count <= 0) count = 1;
+        void *p = nullptr;
+        NS_HIP(hipMalloc(&p, sizeof(T) * size_t(count)));
+        blocks.push_back(p);
+        bytes += int64_t(sizeof(T)) * count;
+        return static_cast<T *>(p);
+    }
+    void release() {
+        for (void *p : blocks) (void)hipFree(p);
+        blocks.clear();
+        bytes = 0;
+    }
+    ~DeviceArena() { release(); }
+};
+
+// One brain as the pool builder sees it.
+struct BrainSpec {
+    const HostBrain *brain;
+    int batch_index;     // input / output row
+    int input_base;      // absolute input index of channel 0
+    int output_base;     // absolute output index of motor 0
+    bool world;          // src/main.cu neuron_step rules
+};
+
+struct Tile {
+    int threads = 256, kn = 2, ks = 4;
+    int nmax() const { return threads * kn; }
+    int smax() const { return threads * ks; }
+};
+
+// Structure-of-arrays device pool over a set of brains, laid out in packs.
+class DevicePool {
+public:
+    ~DevicePool() { arena.release(); }
+    void build(const std::vector<BrainSpec> &specs, int outputs, const Tile &tile,
+               hipStream_t stream);
+
+    // state transfer in genome order for pool brain pb
+    void download_state(std::vector<ns_neuron_state> &ns, std::vector<ns_synapse_state> &ss,
+                        std::vector<uint32_t> &hist, std::vector<int64_t> &ticks,
+                        std::vector<uint64_t> *stamps = nullptr);
+    void upload_state(const std::vector<ns_neuron_state> &ns,
+                      const std::vector<ns_synapse_state> &ss, const std::vector<uint32_t> &hist,
+                      const std::vector<int64_t> &ticks, const std::vector<uint64_t> *stamps = nullptr);
+
+    Pool view{};
+    DeviceArena arena;
+    int64_t nn = 0, ns = 0;          // totals
+    int nb = 0, outputs = 0;
+    Tile tile;
+    // per pool brain
+    std::vector<int32_t> bn0, bnc, bs0, bsc, batch_of;
+    std::vector<uint8_t> fallback;   // brun bit 1
+    std::vector<int32_t> pool_of;    // batch index -> pool brain
+    // genome order -> pool position, per pool brain (offsets bs0)
+    std::vector<int32_t> syn_perm;   // for pool synapse position q: genome edge index
+    std::vector<Pack> packs;
+    Pack *d_packs = nullptr;
+    PlainRange fallback_range{}, all_range{};
+    int32_t *d_slow = nullptr;
+    int slow_count = 0, fallback_brains = 0;
+    uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+// Batched-episode surface (gym_adapter/gpu.cu GpuBatch).
+struct Batch {
+    DevicePool pool;
+    std::vector<BrainSpec> specs;
+    std::vector<HostBrain> brains;   // owned copies of the genome handles
+    int count = 0, channels = 0, outputs = 0;
+    int form = NS_FORM_RESIDENT;
+    bool world = false;
+    float *d_input = nullptr, *d_output = nullptr;
+    uint8_t *d_run_host_order = nullptr;
+    DeviceArena io;
+    hipStream_t stream = nullptr;
+    std::vector<uint8_t> h_brun;
+    bool brun_valid = false;
+    std::vector<int32_t> h_status;
+
+    void create(std::vector<HostBrain> &&owned, int max_delay_unused);
+    void relayout(const Tile &tile);
+    void set_run(const uint8_t *mask);                 // host mask, batch order
+    void run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
+             hipStream_t st);
+    ~Batch();
+};
+
+}  // namespace nsa

@@ -1,0 +1,47 @@
+// Launch interface of the tick kernels (host side of ns_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ns_device.h"
+
+namespace nsa {
+
+// Per-pack limits shared by the packer and the resident kernel's LDS arrays.
+constexpr int kPackBrains = 128;   // brains per pack
+constexpr int kPackMotors = 512;   // motor slots per pack
+
+// A pack: whole brains [b0, b0+nb) whose neurons [n0, n0+nn) and synapses
+// [s0, s0+ns) are contiguous in the pools; nm = nb * outputs motor slots.
+struct Pack {
+    int32_t n0, nn, s0, ns, b0, nb, nm, pad;
+};
+
+// Element ranges for the plain per-tick kernels.
+struct PlainRange {
+    int64_t n_lo, n_hi, s_lo, s_hi;
+    int brains;            // brain table size (need-mask decides who runs)
+    int slots;             // motor slots covered (brains * outputs)
+    const int32_t *slow;   // positions of type-2/3 neurons
+    int slow_count;
+};
+
+using ResidentLauncher = hipError_t (*)(const Pool &, const Pack *, int, int, const float *,
+                                        float, float *, int, hipStream_t);
+struct TileShape {
+    int threads, neurons_per_thread, synapses_per_thread;
+    ResidentLauncher launch;
+};
+extern const TileShape kTiles[];
+extern const int kNumTiles;
+
+void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t *bseed,
+                  const uint32_t *norig, const uint32_t *sorig, hipStream_t st);
+void launch_fill(float *x, int64_t n, float v, hipStream_t st);
+void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st);
+void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,
+                     const float *input, float gain, float *out, int homeostasis,
+                     hipStream_t st);
+void launch_world_homeostasis(const Pool &p, int64_t nn, const int32_t *age, hipStream_t st);
+
+}  // namespace nsa

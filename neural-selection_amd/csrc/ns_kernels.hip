@@ -1,0 +1,696 @@
+// Tick kernels for gfx950 (MI355X).
+//
+// Two execution forms over the same structure-of-arrays pools, bit-identical
+// to each other and to the reference's CPU tick order (gym_adapter/core.cuh:
+// 30-75, src/main.cu:659-801 and 2702-2728):
+//
+//  * plain per-tick form — one launch each of delivery, neuron and
+//    plasticity per tick over the global pools (the reference's global-slot
+//    kernels, with the float atomics replaced by a per-neuron sequential sum
+//    over its incoming CSR run);
+//  * resident form — one launch per call: each workgroup owns a pack of
+//    whole brains and runs every tick with neuron state and plasticity
+//    traces in registers, the spike rings, releases and routing in LDS, and
+//    two-phase delivery: arrivals are compacted into an LDS queue with
+//    wavefront ballots, then short-term plasticity runs densely over the
+//    queue.  Brains are independent, so no cross-workgroup synchronisation
+//    is needed inside a call.
+#include "ns_kernels.h"
+
+namespace nsa {
+
+__device__ const uint64_t g_exp_tab[32] = NS_EXP2_TABLE;
+__device__ const double g_log_tab[32] = NS_LOGF_TABLE;
+
+__device__ inline bool need_ok(const Pool &p, int32_t b, uint8_t need) {
+    return (p.brun[b] & need) == need;
+}
+
+__device__ inline uint32_t mix_seed(uint32_t seed) {   // physiology.cuh:183-186
+    uint32_t x = seed ? seed : 1;
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Reset: initial_neuron_state / initial_synapse_state (physiology.cuh:179-202,
+// 393-404) seeded as gym_adapter/gpu.cu:18-23.  need = 1: brains with brun&1.
+// ---------------------------------------------------------------------------
+__global__ void k_reset_neurons(Pool p, int64_t n, const uint32_t *bseed, const uint32_t *orig) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t b = p.nbrain[i];
+    if (!(p.brun[b] & 1)) return;
+    const int64_t N = p.ncap;
+    uint32_t type = p.nflags[i] & NF_TYPE;
+    float rest = p.ng[NG_REST * N + i];
+    for (int k = 0; k < NS_FIELDS; ++k) p.ns[k * N + i] = 0.0f;
+    float cell[kCellFields];
+    for (int k = 0; k < kCellFields; ++k) cell[k] = 0.0f;
+    float v = rest;
+    if (type == 2) {
+        const float lp[16] = {-56, 0, .2f, 0, 1, 0, 0, 1, 0, 0, 1, 1, 0, .04f, .01f, -56};
+        for (int k = 0; k < 16; ++k) cell[k] = lp[k];
+        v = cell[15] - 9.342518882f;
+    } else if (type == 3) {
+        cell[0] = -50.0f; cell[1] = .05f;
+        for (int k = 0; k < 4; ++k) cell[9 + k] = 1.0f;
+        v = cell[0];
+    } else if (type == 6) {
+        cell[0] = rest;
+        cell[1] = p.ng[NG_THR * N + i];
+    }
+    for (int k = 0; k < kCellFields; ++k) p.cell[k * N + i] = cell[k];
+    p.ns[NS_V * N + i] = v;
+    uint32_t x = mix_seed(bseed[b] + orig[i] * 7919u);
+    p.rrng[i] = x ? x : 1;
+    p.total[i] = 0;
+    p.spike[i] = 0;
+    p.hist[i] = 0;
+    p.modbits[i] = 0;
+    for (int k = 0; k < 6; ++k) p.stamps[k * N + i] = 0;
+}
+
+__global__ void k_reset_synapses(Pool p, int64_t n, const uint32_t *bseed, const uint32_t *orig) {
+    int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int32_t b = p.sbrain[q];
+    if (!(p.brun[b] & 1)) return;
+    const int64_t S = p.scap;
+    uint32_t x = mix_seed(bseed[b] + orig[q] * 104729u);
+    if (!x) x = 1;
+    float init[SS_FIELDS] = {};
+    init[SS_WEIGHT] = p.sg[SG_W0 * S + q];
+    init[SS_DA_AGE] = -1; init[SS_SPINE] = 1; init[SS_SAGE] = -1;
+    init[SS_PRE_AGE] = 1000000; init[SS_POST_AGE] = 1000000; init[SS_SLOW] = 1;
+    for (int k = 0; k < SS_FIELDS; ++k) p.ss[k * S + q] = init[k];
+    int sites = (p.sflags[q] >> SF_SITES_SHIFT) & 127;
+    p.ready[q] = sites == 64 ? ~uint64_t(0) : (uint64_t(1) << sites) - 1;
+    p.last[q] = -1000000;
+    p.srng[q] = x;
+    p.arrived[q] = 0;
+    p.released[q] = 0;
+}
+
+__global__ void k_set_ticks(Pool p, int nb, int64_t add, int64_t set, uint8_t need) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb || !need_ok(p, b, need)) return;
+    p.btick[b] = set >= 0 ? set : p.btick[b] + add;
+}
+
+__global__ void k_fill_f32(float *x, int64_t n, float v) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ void k_fill_i32(int32_t *x, int64_t n, int32_t v) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-tick form
+// ---------------------------------------------------------------------------
+
+// Delivery (src/main.cu:659-708; gym_adapter/core.cuh:37-47): arrival test on
+// the spike shift register, short-term release on arrival.
+__global__ void k_deliver(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t need) {
+    int64_t q = s_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned arr = 0;
+    if (q < s_hi) {
+        int32_t b = p.sbrain[q];
+        if (need_ok(p, b, need)) {
+            uint32_t flags = p.sflags[q];
+            uint32_t route = flags >> SF_ROUTE_SHIFT;
+            float rel = 0;
+            if (route & R_VALID) {
+                int64_t tick = p.btick[b] + t;
+                int d = flags & SF_DELAY;
+                if (tick >= d) arr = (p.hist[p.spre[q]] >> (d - 1)) & 1u;
+                if (arr) {
+                    float w = p.ss[SS_WEIGHT * p.scap + q];
+                    rel = (route & R_STP) ? release_at(p, q, flags, w, tick, g_exp_tab) : w;
+                }
+            }
+            p.arrived[q] = uint8_t(arr);
+            p.released[q] = rel;
+        }
+    }
+    uint64_t m = __ballot(arr);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(p.events, (unsigned long long)__popcll(m));
+}
+
+// Sequential per-neuron reduction of this tick's releases in ascending edge
+// order (gym_adapter/gpu.cu:40-47), with the world's modulation stamps
+// (src/main.cu:669-673).  Returns the modulator bits.
+__device__ inline unsigned gather_inputs(const Pool &p, int64_t i, int64_t tick, float &e0,
+                                         float &e1, float &e2, float &e3, float &inh,
+                                         const uint64_t *etab) {
+    unsigned mod = 0;
+    bool have_scale = false;
+    float scale = 0;
+    const int32_t q1 = p.nin[i + 1];
+    for (int32_t q = p.nin[i]; q < q1; ++q) {
+        float r = p.released[q];
+        if (r == 0.0f) continue;
+        uint32_t route = p.sflags[q] >> SF_ROUTE_SHIFT;
+        if (route & R_MOD) {
+            if (r > 0) {
+                unsigned m = (route >> R_MOD_SHIFT) & 3u;
+                mod |= 1u << (m - 1);
+                uint64_t &st = p.stamps[((m - 1) * 2 + (tick & 1)) * p.ncap + i];
+                if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
+            }
+        } else if (route & R_INH) {
+            inh += r;
+        } else {
+            if (!have_scale) {
+                scale = ns_expf(p.ns[NS_LAS * p.ncap + i], etab);
+                have_scale = true;
+            }
+            float v = r * scale;
+            switch (route & R_BRANCH) {
+            case 0: e0 += v; break;
+            case 1: e1 += v; break;
+            case 2: e2 += v; break;
+            default: e3 += v; break;
+            }
+        }
+    }
+    return mod;
+}
+
+__device__ inline float sensory_current(const Pool &p, int64_t i, NeuronRegs &s,
+                                        const NeuronConst &g, const float *input, float gain) {
+    int32_t idx = p.ninput[i];
+    if (idx < 0) return 0.0f;
+    float level = input[idx];
+    return (g.flags & NF_RECEPTOR) ? receptor_drive(s, g, level) : gain * level;
+}
+
+// neuron_step (src/main.cu:710-746) / the neuron loop of core.cuh:49-58.
+__global__ void k_neuron(Pool p, int64_t n_lo, int64_t n_hi, int t, uint8_t need,
+                         const float *input, float gain) {
+    int64_t i = n_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_hi) return;
+    int32_t b = p.nbrain[i];
+    if (!need_ok(p, b, need)) return;
+    uint32_t flags = p.nflags[i];
+    uint32_t type = flags & NF_TYPE;
+    if (type == 2 || type == 3) return;
+    if (!(flags & NF_ACTIVE)) {
+        uint32_t h = p.hist[i];
+        if (flags & NF_WORLD) {
+            p.hist[i] = (h << 1) | ((h >> (kWorldHistory - 1)) & 1u);
+        } else {
+            p.hist[i] = h << 1;
+            p.spike[i] = 0;
+        }
+        return;
+    }
+    int64_t tick = p.btick[b] + t;
+    float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
+    unsigned mod = gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab);
+    NeuronRegs s;
+    NeuronConst g;
+    load_neuron(p, i, s, g, g_exp_tab);
+    float current = sensory_current(p, i, s, g, input, gain);
+    int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, g_exp_tab);
+    store_neuron(p, i, s);
+    p.spike[i] = uint8_t(spike);
+    p.hist[i] = (p.hist[i] << 1) | uint32_t(spike);
+    p.modbits[i] = uint8_t(mod);
+    if (!isfinite(s.v)) p.bstatus[b] = -1;
+}
+
+// Types 2 (LP, lp_cell.cuh) and 3 (Prinz, prinz_cell.cuh): physiology.cuh:313-347.
+__global__ void k_neuron_slow(Pool p, const int32_t *list, int count, int t, uint8_t need,
+                              const float *input, float gain) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    int64_t i = list[k];
+    int32_t b = p.nbrain[i];
+    if (!need_ok(p, b, need)) return;
+    uint32_t flags = p.nflags[i];
+    const int64_t N = p.ncap;
+    if (!(flags & NF_ACTIVE)) {
+        uint32_t h = p.hist[i];
+        if (flags & NF_WORLD) {
+            p.hist[i] = (h << 1) | ((h >> (kWorldHistory - 1)) & 1u);
+        } else {
+            p.hist[i] = h << 1;
+            p.spike[i] = 0;
+        }
+        return;
+    }
+    int64_t tick = p.btick[b] + t;
+    float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh_in = 0;
+    unsigned mod = gather_inputs(p, i, tick, e0, e1, e2, e3, inh_in, g_exp_tab);
+    NeuronRegs s;
+    NeuronConst g;
+    load_neuron(p, i, s, g, g_exp_tab);
+    float current = sensory_current(p, i, s, g, input, gain);
+    s.d0 += e0; s.d1 += e1; s.d2 += e2; s.d3 += e3;
+    float sub = fmaxf(0.0f, fminf(s.v, -50.0f) - g.rest);
+    s.depol += 0.1f * (sub - s.depol);
+    s.inh += inh_in;
+    float c[kCellFields];
+    for (int j = 0; j < kCellFields; ++j) c[j] = p.cell[j * N + i];
+    float e = 0;
+    e += s.d0; e += s.d1; e += s.d2; e += s.d3;
+    int spike = 0;
+    if ((flags & NF_TYPE) == 2) {
+        float before = c[0];
+        float ext = current + e * (0.0f - s.v) + s.inh * (-75.0f - s.v) - 0.0f * (s.v - -68.0f);
+        float sc[4];
+        for (int j = 0; j < 4; ++j) sc[j] = p.nx[(NX_LP + j) * N + i];
+        lp_advance(c, ext / 1000.0f, sc, g_exp_tab);
+        s.v = c[15] - 9.342518882f;
+        spike = before < 0 && c[0] >= 0;
+    } else {
+        float gg[8];
+        for (int j = 0; j < 8; ++j) gg[j] = p.nx[(NX_PRINZ + j) * N + i];
+        float amb = p.nx[NX_AMBIENT * N + i];
+        float ext = current + e * (0.0f - s.v) + s.inh * (-75.0f - s.v);
+        bool fired = false;
+        for (int j = 0; j < 40; ++j) {
+            const float before = c[0];
+            pz_substep(c, gg, ext, amb + 0.0f, -68.0f, g_exp_tab, g_log_tab);
+            fired |= before < 0 && c[0] >= 0;
+        }
+        spike = fired;
+        s.v = c[0];
+    }
+    s.total += spike;
+    s.d0 *= kDecayExcMs; s.d1 *= kDecayExcMs; s.d2 *= kDecayExcMs; s.d3 *= kDecayExcMs;
+    s.inh *= kDecayInhMs;
+    for (int j = 0; j < kCellFields; ++j) p.cell[j * N + i] = c[j];
+    store_neuron(p, i, s);
+    p.spike[i] = uint8_t(spike);
+    p.hist[i] = (p.hist[i] << 1) | uint32_t(spike);
+    p.modbits[i] = uint8_t(mod);
+    if (!isfinite(s.v)) p.bstatus[b] = -1;
+}
+
+// Plasticity (src/main.cu:748-782; core.cuh:60-66) plus motor counting
+// (core.cuh:68, main.cu:742-743) for threads below `slots`.
+__global__ void k_plasticity(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t need,
+                             int slots, float *out) {
+    int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid < slots) {
+        int32_t b = int32_t(gid / p.outputs);
+        if (need_ok(p, b, need))
+            out[p.bout0[b] + gid % p.outputs] += float(p.spike[p.motor[gid]]);
+    }
+    int64_t q = s_lo + gid;
+    if (q >= s_hi) return;
+    int32_t b = p.sbrain[q];
+    if (!need_ok(p, b, need) || p.bfrozen[b]) return;
+    uint32_t flags = p.sflags[q];
+    uint32_t route = flags >> SF_ROUTE_SHIFT;
+    int profile = (flags >> SF_PLAST_SHIFT) & 7;
+    if (!(route & R_VALID) || !(route & R_PAIR) || !profile) return;
+    int32_t post = p.spost[q];
+    PlastRegs r;
+    load_plast(p, q, profile, r);
+    plasticity_step(r, profile, (flags >> SF_MODREC_SHIFT) & 7, p.sg[SG_W0 * p.scap + q],
+                    p.arrived[q] != 0, p.spike[post] != 0, p.ns[NS_DEPOL * p.ncap + post],
+                    p.modbits[post], g_exp_tab);
+    store_plast(p, q, profile, r);
+}
+
+// Homeostasis at the end of a batched call (core.cuh:70-72): elapsed = ticks,
+// rate = total_spikes * 1000 / max(ticks, tick after the call).
+__global__ void k_homeostasis_gym(Pool p, int64_t n_lo, int64_t n_hi, int ticks, uint8_t need) {
+    int64_t i = n_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_hi) return;
+    int32_t b = p.nbrain[i];
+    if (!need_ok(p, b, need)) return;
+    uint32_t flags = p.nflags[i];
+    if (!(flags & NF_ACTIVE) || (flags & NF_TYPE) != 0) return;
+    int64_t tick = p.btick[b] + ticks;
+    float hz = p.total[i] * 1000.0f / float(ticks > tick ? int64_t(ticks) : tick);
+    float &las = p.ns[NS_LAS * p.ncap + i];
+    las = homeostasis_las(las, float(ticks), hz);
+}
+
+// homeostasis_step (src/main.cu:1287-1300): once per world step, elapsed 16
+// ms, rate = total_spikes * 1000 / max(16, age * 16).
+__global__ void k_homeostasis_world(Pool p, int64_t n, const int32_t *age) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t b = p.nbrain[i];
+    if (!(p.brun[b] & 1) || p.bfrozen[b]) return;
+    uint32_t flags = p.nflags[i];
+    if (!(flags & NF_ACTIVE) || (flags & NF_TYPE) != 0) return;
+    float elapsed = fmaxf(16.0f, age[b] * 16.0f);
+    float hz = p.total[i] * 1000.0f / elapsed;
+    float &las = p.ns[NS_LAS * p.ncap + i];
+    las = homeostasis_las(las, 16.0f, hz);
+}
+
+// ---------------------------------------------------------------------------
+// Resident form
+// ---------------------------------------------------------------------------
+template <int WG, int KN, int KS>
+__global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int ticks,
+                                                 const float *input, float gain, float *out,
+                                                 int homeostasis) {
+    constexpr int NMAX = WG * KN, SMAX = WG * KS;
+    __shared__ uint64_t etab[32];
+    __shared__ uint32_t hist_l[NMAX];
+    __shared__ float depol_l[NMAX];
+    __shared__ uint8_t mod_l[NMAX];
+    __shared__ uint16_t in_l[NMAX + 1];
+    __shared__ float rel_l[SMAX];
+    __shared__ uint8_t route_l[SMAX];
+    __shared__ uint8_t sbr_l[SMAX];
+    __shared__ uint16_t queue_l[SMAX];
+    __shared__ int64_t btick_l[kPackBrains];
+    __shared__ uint8_t brun_l[kPackBrains], bfrozen_l[kPackBrains];
+    __shared__ uint16_t mot_l[kPackMotors];
+    __shared__ uint16_t cnt_l[kPackMotors];
+    __shared__ int qlen;
+    __shared__ unsigned events_l;
+
+    const Pack pk = packs[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int64_t N = p.ncap, S = p.scap;
+    const int64_t n0 = pk.n0, s0 = pk.s0;
+
+    if (tid < 32) etab[tid] = g_exp_tab[tid];
+    for (int b = tid; b < pk.nb; b += WG) {
+        btick_l[b] = p.btick[pk.b0 + b];
+        brun_l[b] = p.brun[pk.b0 + b] & 1;
+        bfrozen_l[b] = p.bfrozen[pk.b0 + b];
+    }
+    for (int n = tid; n < pk.nn; n += WG) {
+        hist_l[n] = p.hist[n0 + n];
+        in_l[n] = uint16_t(p.nin[n0 + n] - s0);
+        mod_l[n] = 0;
+        depol_l[n] = 0;
+    }
+    if (tid == 0) {
+        in_l[pk.nn] = uint16_t(pk.ns);
+        qlen = 0;
+        events_l = 0;
+    }
+    for (int q = tid; q < pk.ns; q += WG) {
+        route_l[q] = uint8_t(p.sflags[s0 + q] >> SF_ROUTE_SHIFT);
+        sbr_l[q] = uint8_t(p.sbrain[s0 + q] - pk.b0);
+    }
+    for (int m = tid; m < pk.nm; m += WG) {
+        mot_l[m] = uint16_t(p.motor[int64_t(pk.b0) * p.outputs + m] - n0);
+        cnt_l[m] = 0;
+    }
+    __syncthreads();
+
+    // Neuron registers.
+    NeuronRegs nr[KN];
+    NeuronConst nc[KN];
+    float nscale[KN], ncur[KN];
+    int nbr[KN];
+    bool nlive[KN];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+        const int n = k * WG + tid;
+        nlive[k] = false;
+        nbr[k] = 0;
+        nscale[k] = 0;
+        ncur[k] = 0;
+        if (n < pk.nn) {
+            const int64_t i = n0 + n;
+            nbr[k] = p.nbrain[i] - pk.b0;
+            load_neuron(p, i, nr[k], nc[k], etab);
+            nlive[k] = brun_l[nbr[k]] != 0;
+            nscale[k] = ns_expf(p.ns[NS_LAS * N + i], etab);
+            int32_t idx = p.ninput[i];
+            ncur[k] = idx >= 0 ? input[idx] : -1.0f;   // level, or -1 for no input
+            if (idx < 0) nc[k].flags |= 1u << 30;
+        }
+    }
+    // Synapse registers.
+    uint32_t sfl[KS], spp[KS];
+    float sw0[KS];
+    PlastRegs sp[KS];
+    unsigned arrmask = 0;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const int q = k * WG + tid;
+        sfl[k] = 0;
+        spp[k] = 0;
+        sw0[k] = 0;
+        sp[k] = PlastRegs{0, 0, 0, 0, 0};
+        if (q < pk.ns) {
+            const int64_t gq = s0 + q;
+            uint32_t f = p.sflags[gq];
+            if (!brun_l[sbr_l[q]]) f &= ~(R_VALID << SF_ROUTE_SHIFT);
+            sfl[k] = f;
+            spp[k] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << 16);
+            sw0[k] = p.sg[SG_W0 * S + gq];
+            int profile = (f >> SF_PLAST_SHIFT) & 7;
+            load_plast(p, gq, profile, sp[k]);
+        }
+    }
+
+    for (int t = 0; t < ticks; ++t) {
+        // Phase A: plasticity of tick t-1, then arrivals of tick t.
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const int q = k * WG + tid;
+            const uint32_t f = sfl[k];
+            const uint32_t route = f >> SF_ROUTE_SHIFT;
+            bool arrive_stp = false;
+            if (route & R_VALID) {
+                const int profile = (f >> SF_PLAST_SHIFT) & 7;
+                const int post = spp[k] >> 16, pre = spp[k] & 0xffff;
+                if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[sbr_l[q]])
+                    plasticity_step(sp[k], profile, (f >> SF_MODREC_SHIFT) & 7, sw0[k],
+                                    (arrmask >> k) & 1, hist_l[post] & 1, depol_l[post],
+                                    mod_l[post], etab);
+                const int64_t tick = btick_l[sbr_l[q]] + t;
+                const int d = f & SF_DELAY;
+                const bool arr = tick >= d && ((hist_l[pre] >> (d - 1)) & 1u);
+                arrmask = (arrmask & ~(1u << k)) | (unsigned(arr) << k);
+                rel_l[q] = arr ? sp[k].weight : 0.0f;
+                arrive_stp = arr && (route & R_STP);
+            } else if (q < pk.ns) {
+                rel_l[q] = 0.0f;
+                arrmask &= ~(1u << k);
+            }
+            // wavefront ballot compaction of arrivals that need short-term release
+            const uint64_t m = __ballot(arrive_stp);
+            if (m) {
+                const int lane = tid & 63;
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&qlen, __popcll(m));
+                base = __shfl(base, 0);
+                if (arrive_stp) queue_l[base + __popcll(m & ((uint64_t(1) << lane) - 1))] = uint16_t(q);
+            }
+        }
+        unsigned my_arr = __popc(arrmask);
+        for (int off = 32; off > 0; off >>= 1) my_arr += __shfl_xor(my_arr, off);
+        if ((tid & 63) == 0 && my_arr) atomicAdd(&events_l, my_arr);
+        __syncthreads();
+        // Phase A2: short-term release over the compacted arrivals.
+        {
+            const int n_q = qlen;
+            for (int j = tid; j < n_q; j += WG) {
+                const int q = queue_l[j];
+                const int64_t gq = s0 + q;
+                const int64_t tick = btick_l[sbr_l[q]] + t;
+                rel_l[q] = release_at(p, gq, p.sflags[gq], rel_l[q], tick, etab);
+            }
+        }
+        __syncthreads();
+        // Phase B: neurons.
+        if (tid == 0) qlen = 0;
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+            const int n = k * WG + tid;
+            if (!nlive[k]) continue;
+            const uint32_t flags = nc[k].flags;
+            if (!(flags & NF_ACTIVE)) {
+                const uint32_t h = hist_l[n];
+                hist_l[n] = (flags & NF_WORLD) ? (h << 1) | ((h >> (kWorldHistory - 1)) & 1u) : h << 1;
+                continue;
+            }
+            const int64_t tick = btick_l[nbr[k]] + t;
+            float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
+            unsigned mod = 0;
+            const int q1 = in_l[n + 1];
+            for (int q = in_l[n]; q < q1; ++q) {
+                const float r = rel_l[q];
+                if (r == 0.0f) continue;
+                const uint32_t route = route_l[q];
+                if (route & R_MOD) {
+                    if (r > 0) {
+                        const unsigned m = (route >> R_MOD_SHIFT) & 3u;
+                        mod |= 1u << (m - 1);
+                        uint64_t &st = p.stamps[((m - 1) * 2 + (tick & 1)) * N + n0 + n];
+                        if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
+                    }
+                } else if (route & R_INH) {
+                    inh += r;
+                } else {
+                    const float v = r * nscale[k];
+                    switch (route & R_BRANCH) {
+                    case 0: e0 += v; break;
+                    case 1: e1 += v; break;
+                    case 2: e2 += v; break;
+                    default: e3 += v; break;
+                    }
+                }
+            }
+            float current = 0.0f;
+            if (!(flags & (1u << 30)))
+                current = (flags & NF_RECEPTOR) ? receptor_drive(nr[k], nc[k], ncur[k])
+                                                : gain * ncur[k];
+            const int spike = neuron_tick(nr[k], nc[k], current, e0, e1, e2, e3, inh, p.cell, N,
+                                          n0 + n, etab);
+            hist_l[n] = (hist_l[n] << 1) | uint32_t(spike);
+            depol_l[n] = nr[k].depol;
+            mod_l[n] = uint8_t(mod);
+            if (!isfinite(nr[k].v)) p.bstatus[pk.b0 + nbr[k]] = -1;
+        }
+        __syncthreads();
+        // Motor spike counts of tick t (core.cuh:68).
+        for (int m = tid; m < pk.nm; m += WG)
+            if (brun_l[m / p.outputs]) cnt_l[m] += hist_l[mot_l[m]] & 1u;
+    }
+    // Plasticity of the last tick.
+    if (ticks > 0) {
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const int q = k * WG + tid;
+            const uint32_t f = sfl[k];
+            const uint32_t route = f >> SF_ROUTE_SHIFT;
+            const int profile = (f >> SF_PLAST_SHIFT) & 7;
+            if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[sbr_l[q]]) {
+                const int post = spp[k] >> 16;
+                plasticity_step(sp[k], profile, (f >> SF_MODREC_SHIFT) & 7, sw0[k],
+                                (arrmask >> k) & 1, hist_l[post] & 1, depol_l[post], mod_l[post],
+                                etab);
+            }
+        }
+    }
+    // Write back.
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+        const int n = k * WG + tid;
+        if (n >= pk.nn || !brun_l[nbr[k]]) continue;
+        const int64_t i = n0 + n;
+        if (homeostasis && (nc[k].flags & NF_ACTIVE) && (nc[k].flags & NF_TYPE) == 0) {
+            const int64_t tick = btick_l[nbr[k]] + ticks;
+            const float hz = nr[k].total * 1000.0f / float(ticks > tick ? int64_t(ticks) : tick);
+            p.ns[NS_LAS * N + i] = homeostasis_las(p.ns[NS_LAS * N + i], float(ticks), hz);
+        }
+        if (nc[k].flags & NF_ACTIVE) {
+            store_neuron(p, i, nr[k]);
+            p.spike[i] = uint8_t(hist_l[n] & 1u);
+            p.modbits[i] = mod_l[n];
+        } else if (!(nc[k].flags & NF_WORLD)) {
+            p.spike[i] = 0;
+        }
+        p.hist[i] = hist_l[n];
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const int q = k * WG + tid;
+        if (q >= pk.ns || !brun_l[sbr_l[q]]) continue;
+        const int64_t gq = s0 + q;
+        const int profile = (sfl[k] >> SF_PLAST_SHIFT) & 7;
+        store_plast(p, gq, profile, sp[k]);
+        p.arrived[gq] = uint8_t((arrmask >> k) & 1);
+    }
+    for (int m = tid; m < pk.nm; m += WG) {
+        const int b = m / p.outputs;
+        out[p.bout0[pk.b0 + b] + m % p.outputs] = brun_l[b] ? float(cnt_l[m]) : 0.0f;
+    }
+    for (int b = tid; b < pk.nb; b += WG)
+        if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
+    __syncthreads();
+    if (tid == 0 && events_l) atomicAdd(p.events, (unsigned long long)events_l);
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n, int threads) {
+    return unsigned((n + threads - 1) / threads);
+}
+
+void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t *bseed,
+                  const uint32_t *norig, const uint32_t *sorig, hipStream_t st) {
+    if (nn) hipLaunchKernelGGL(k_reset_neurons, dim3(blocks_for(nn, 256)), dim3(256), 0, st, p, nn, bseed, norig);
+    if (ns) hipLaunchKernelGGL(k_reset_synapses, dim3(blocks_for(ns, 256)), dim3(256), 0, st, p, ns, bseed, sorig);
+    hipLaunchKernelGGL(k_set_ticks, dim3(blocks_for(nb, 256)), dim3(256), 0, st, p, nb, int64_t(0), int64_t(0), uint8_t(1));
+}
+
+void launch_fill(float *x, int64_t n, float v, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, v);
+}
+void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, v);
+}
+
+void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,
+                     const float *input, float gain, float *out, int homeostasis,
+                     hipStream_t st) {
+    const int T = 256;
+    const int64_t ns = r.s_hi - r.s_lo, nn = r.n_hi - r.n_lo;
+    const int64_t plast_threads = ns > r.slots ? ns : r.slots;
+    for (int t = 0; t < ticks; ++t) {
+        if (ns) hipLaunchKernelGGL(k_deliver, dim3(blocks_for(ns, T)), dim3(T), 0, st, p, r.s_lo, r.s_hi, t, need);
+        if (nn) hipLaunchKernelGGL(k_neuron, dim3(blocks_for(nn, T)), dim3(T), 0, st, p, r.n_lo, r.n_hi, t, need, input, gain);
+        if (r.slow_count)
+            hipLaunchKernelGGL(k_neuron_slow, dim3(blocks_for(r.slow_count, 64)), dim3(64), 0, st, p, r.slow, r.slow_count, t, need, input, gain);
+        if (plast_threads)
+            hipLaunchKernelGGL(k_plasticity, dim3(blocks_for(plast_threads, T)), dim3(T), 0, st, p, r.s_lo, r.s_hi, t, need, r.slots, out);
+    }
+    if (homeostasis && nn)
+        hipLaunchKernelGGL(k_homeostasis_gym, dim3(blocks_for(nn, T)), dim3(T), 0, st, p, r.n_lo, r.n_hi, ticks, need);
+    hipLaunchKernelGGL(k_set_ticks, dim3(blocks_for(r.brains, T)), dim3(T), 0, st, p, r.brains, int64_t(ticks), int64_t(-1), need);
+}
+
+void launch_world_homeostasis(const Pool &p, int64_t nn, const int32_t *age, hipStream_t st) {
+    if (nn) hipLaunchKernelGGL(k_homeostasis_world, dim3(blocks_for(nn, 256)), dim3(256), 0, st, p, nn, age);
+}
+
+template <int WG, int KN, int KS>
+static hipError_t launch_resident_t(const Pool &p, const Pack *packs, int npacks, int ticks,
+                                    const float *input, float gain, float *out, int homeostasis,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL((k_resident<WG, KN, KS>), dim3(npacks), dim3(WG), 0, st, p, packs, ticks,
+                       input, gain, out, homeostasis);
+    return hipGetLastError();
+}
+
+// Explicit instantiations so the device pass emits every tile's kernel.
+#define NS_INSTANTIATE(WG, KN, KS)                                                             \
+    template __global__ void k_resident<WG, KN, KS>(Pool, const Pack *, int, const float *, float, \
+                                                    float *, int);
+NS_INSTANTIATE(256, 1, 4)
+NS_INSTANTIATE(256, 2, 4)
+NS_INSTANTIATE(256, 2, 8)
+NS_INSTANTIATE(512, 1, 4)
+NS_INSTANTIATE(512, 2, 4)
+NS_INSTANTIATE(512, 1, 8)
+NS_INSTANTIATE(1024, 1, 4)
+NS_INSTANTIATE(1024, 1, 8)
+NS_INSTANTIATE(128, 2, 8)
+
+// Host-only table: clang would otherwise promote this const array of host
+// function pointers into device constant memory.
+#if !defined(__HIP_DEVICE_COMPILE__)
+#define NS_TILE(WG, KN, KS) {WG, KN, KS, &launch_resident_t<WG, KN, KS>}
+const TileShape kTiles[] = {
+    NS_TILE(256, 1, 4), NS_TILE(256, 2, 4), NS_TILE(256, 2, 8),
+    NS_TILE(512, 1, 4), NS_TILE(512, 2, 4), NS_TILE(512, 1, 8),
+    NS_TILE(1024, 1, 4), NS_TILE(1024, 1, 8), NS_TILE(128, 2, 8),
+};
+const int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
+#endif
+
+}  // namespace nsa

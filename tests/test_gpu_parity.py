@@ -1,0 +1,58 @@
+"""Device tick path vs the CPU oracle, bit-exact, through the C ABI."""
+import numpy as np
+import pytest
+
+import ns_amd
+import ns_workloads as W
+from parity_util import assert_state_equal, oracle_brains, oracle_export_all, oracle_step_all
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_compare(w, form, calls, check_every=10, tiles=None, mask_fn=None):
+    print(f'[parity] workload={w.name} brains={w.count} neurons={w.total_neurons} '
+          f'synapses={w.total_synapses} form={form} calls={calls} tiles={tiles}', flush=True)
+    batch = ns_amd.GpuBatch(workload=w)
+    if tiles:
+        batch.set_tiles(*tiles)
+    batch.set_form(form)
+    batch.reset(w.seeds())
+    brains = oracle_brains(w)
+    for call in range(calls):
+        x = w.inputs(call)
+        mask = mask_fn(call) if mask_fn else None
+        out, status = batch.step(x, w.ticks, w.gain, mask)
+        oo, orc = oracle_step_all(brains, x, w.ticks, w.gain, mask)
+        assert np.array_equal(out, oo), f'call {call}: motor outputs differ'
+        assert np.array_equal(status != 0, orc != 0), f'call {call}: status differs'
+        if call % check_every == check_every - 1 or call == calls - 1:
+            assert_state_equal(batch.export(), oracle_export_all(brains), f'call {call}')
+    info = batch.info()
+    batch.close()
+    return info
+
+
+@pytest.mark.parametrize('form', [ns_amd.FORM_RESIDENT, ns_amd.FORM_PER_TICK])
+def test_c1_plain_tick_bit_exact(form):
+    w = W.c1(calls=200)
+    info = _run_compare(w, form, calls=200, check_every=50)
+    assert info['events'] > 0
+
+
+@pytest.mark.parametrize('form', [ns_amd.FORM_RESIDENT, ns_amd.FORM_PER_TICK])
+@pytest.mark.parametrize('seed,types', [(11, (0, 1, 4, 5, 6)), (12, (0, 1, 2, 3, 4, 5, 6))])
+def test_mixed_plastic_brains_bit_exact(form, seed, types):
+    w = W.small_mix(seed=seed, brains=12, calls=40, types=types)
+    _run_compare(w, form, calls=40, check_every=5)
+
+
+def test_masks_and_resets_match():
+    w = W.small_mix(seed=21, brains=6, calls=30, types=(0, 1, 4, 5))
+    _run_compare(w, ns_amd.FORM_RESIDENT, calls=30, check_every=3,
+                 mask_fn=lambda c: np.array([1, c % 2, 1, c % 3 == 0, 0, 1], np.uint8))
+
+
+@pytest.mark.parametrize('tiles', [(256, 1, 4), (512, 2, 4), (1024, 1, 8), (128, 2, 8)])
+def test_tile_shapes_agree(tiles):
+    w = W.small_mix(seed=31, brains=10, calls=12, types=(0, 1, 4, 5, 6), lo=20, hi=120)
+    _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=tiles)
