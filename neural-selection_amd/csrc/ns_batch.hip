@@ -238,7 +238,8 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
         const int64_t n0 = bn0[pb], s0 = bs0[pb];
         const int nc = bnc[pb], sc = bsc[pb];
         bout0[pb] = spec.output_base;
-        for (int j = 0; j < outs; ++j) motor[int64_t(pb) * outs + j] = int32_t(n0 + b.motors[j]);
+        for (int j = 0; j < outs; ++j)
+            motor[int64_t(pb) * outs + j] = b.motors[j] >= 0 ? int32_t(n0 + b.motors[j]) : -1;
         for (int i = 0; i < nc; ++i) {
             const ns_neuron_gene &g = b.genes[i];
             const int64_t n = n0 + i;
@@ -603,8 +604,13 @@ void Batch::set_run(const uint8_t *mask) {
 
 void Batch::run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
                 hipStream_t st) {
-    launch_fill(d_out, int64_t(count) * outputs, 0.0f, st);
-    launch_fill_i32(pool.view.bstatus, count, 0, st);
+    run_pool(pool, form, ticks, d_in, gain, d_out, int64_t(count) * outputs, homeostasis, st);
+}
+
+void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
+              int64_t out_count, int homeostasis, hipStream_t st) {
+    launch_fill(d_out, out_count, 0.0f, st);
+    launch_fill_i32(pool.view.bstatus, pool.nb, 0, st);
     if (form == NS_FORM_PER_TICK) {
         launch_per_tick(pool.view, pool.all_range, ticks, 1, d_in, gain, d_out, homeostasis, st);
         return;

@@ -108,6 +108,12 @@ public:
     hipStream_t stream = nullptr;
 };
 
+// One call of `ticks` ticks over every brain whose run flag is set: the
+// resident kernel over packs plus the per-tick kernels over the rest (or the
+// per-tick kernels over everything when form == NS_FORM_PER_TICK).
+void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
+              int64_t out_count, int homeostasis, hipStream_t st);
+
 // Batched-episode surface (gym_adapter/gpu.cu GpuBatch).
 struct Batch {
     DevicePool pool;

@@ -10,6 +10,7 @@ namespace nsa {
 // Per-pack limits shared by the packer and the resident kernel's LDS arrays.
 constexpr int kPackBrains = 128;   // brains per pack
 constexpr int kPackMotors = 512;   // motor slots per pack
+constexpr uint16_t kNoMotor = 0xffff;   // motor slot without a neuron (world ports)
 
 // A pack: whole brains [b0, b0+nb) whose neurons [n0, n0+nn) and synapses
 // [s0, s0+ns) are contiguous in the pools; nm = nb * outputs motor slots.

@@ -300,8 +300,8 @@ __global__ void k_plasticity(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t 
     int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gid < slots) {
         int32_t b = int32_t(gid / p.outputs);
-        if (need_ok(p, b, need))
-            out[p.bout0[b] + gid % p.outputs] += float(p.spike[p.motor[gid]]);
+        const int32_t mn = p.motor[gid];
+        if (need_ok(p, b, need) && mn >= 0) out[p.bout0[b] + gid % p.outputs] += float(p.spike[mn]);
     }
     int64_t q = s_lo + gid;
     if (q >= s_hi) return;
@@ -401,7 +401,8 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
         sbr_l[q] = uint8_t(p.sbrain[s0 + q] - pk.b0);
     }
     for (int m = tid; m < pk.nm; m += WG) {
-        mot_l[m] = uint16_t(p.motor[int64_t(pk.b0) * p.outputs + m] - n0);
+        const int32_t mn = p.motor[int64_t(pk.b0) * p.outputs + m];
+        mot_l[m] = mn >= 0 ? uint16_t(mn - n0) : kNoMotor;
         cnt_l[m] = 0;
     }
     __syncthreads();
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
         __syncthreads();
         // Motor spike counts of tick t (core.cuh:68).
         for (int m = tid; m < pk.nm; m += WG)
-            if (brun_l[m / p.outputs]) cnt_l[m] += hist_l[mot_l[m]] & 1u;
+            if (brun_l[m / p.outputs] && mot_l[m] != kNoMotor) cnt_l[m] += hist_l[mot_l[m]] & 1u;
     }
     // Plasticity of the last tick.
     if (ticks > 0) {

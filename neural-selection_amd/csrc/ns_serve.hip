@@ -1,0 +1,392 @@
+// Serving surface of libns_amd.so: continuous batching of genome-evaluation
+// episodes (no reference counterpart; gym_adapter/batch_evaluate.py runs
+// lock-step batches with masks instead).
+//
+// A request is one episode: a brain genome (ns_create encoding, 8 sensory
+// channels, 2 motors) driving a seeded 4-d linear system
+//     x' = A x + B u + sigma * n,   u_j = motor_j spikes / ticks per step,
+// observed through an opponent rate code on 8 channels (2 per dimension).
+// Return = -sum over steps of |x|^2.
+//
+// A serving thread admits queued requests at every launch boundary while
+// their neurons and synapses fit the free pool capacity.  Requests admitted
+// together form a cohort: one device pool (resident packs plus per-tick
+// fallback), its own stream, and its task state in device memory.  Cohorts
+// admitted at different times run concurrently on different streams, so new
+// work overlaps work already on the device.  Each episode's arithmetic is
+// independent of its cohort's composition (per-brain sequential sums, seeds
+// per episode), so results are bit-identical alone and in any mix.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "ns_host.h"
+
+namespace nsa {
+
+namespace {
+constexpr int kChannels = 8, kOutputs = 2, kDims = 4;
+// task record per episode (floats): x[4], A[16], B[8], return, motor totals[2], steps
+constexpr int kTaskFloats = 32;
+enum { T_X = 0, T_A = 4, T_B = 20, T_RET = 28, T_M0 = 29, T_M1 = 30, T_STEPS = 31 };
+
+__global__ void k_encode(const float *task, int count, float *input) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const float *x = task + size_t(e) * kTaskFloats + T_X;
+    for (int d = 0; d < kDims; ++d) {
+        input[e * kChannels + 2 * d] = fminf(1.0f, fmaxf(0.0f, x[d]));
+        input[e * kChannels + 2 * d + 1] = fminf(1.0f, fmaxf(0.0f, -x[d]));
+    }
+}
+
+__device__ inline float uniform01(uint32_t &s) {
+    s ^= s << 13; s ^= s >> 17; s ^= s << 5;
+    return (s >> 8) * (1.0f / 16777216.0f);
+}
+
+// x' = A x + B u + sigma * (sum of 12 uniforms - 6); fixed evaluation order.
+__global__ void k_env(float *task, uint32_t *rng, const float *out, int count, float ticks,
+                      float sigma) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    float *t = task + size_t(e) * kTaskFloats;
+    const float u0 = out[e * kOutputs] / ticks, u1 = out[e * kOutputs + 1] / ticks;
+    uint32_t s = rng[e];
+    float nx[kDims];
+    for (int i = 0; i < kDims; ++i) {
+        float acc = 0.0f;
+        for (int j = 0; j < kDims; ++j) acc += t[T_A + i * kDims + j] * t[T_X + j];
+        acc += t[T_B + i * 2] * u0;
+        acc += t[T_B + i * 2 + 1] * u1;
+        float n = -6.0f;
+        for (int k = 0; k < 12; ++k) n += uniform01(s);
+        nx[i] = acc + sigma * n;
+    }
+    float cost = 0.0f;
+    for (int i = 0; i < kDims; ++i) {
+        t[T_X + i] = nx[i];
+        cost += nx[i] * nx[i];
+    }
+    t[T_RET] -= cost;
+    t[T_M0] += out[e * kOutputs];
+    t[T_M1] += out[e * kOutputs + 1];
+    t[T_STEPS] += 1.0f;
+    rng[e] = s;
+}
+
+struct Request {
+    int64_t id;
+    HostBrain brain;
+    uint32_t seed, task_seed;
+    double submit_s, admit_s = 0;
+};
+
+struct Cohort {
+    std::vector<Request> reqs;
+    std::vector<BrainSpec> specs;
+    DevicePool pool;
+    DeviceArena io;
+    float *d_input = nullptr, *d_out = nullptr, *d_task = nullptr;
+    uint32_t *d_rng = nullptr;
+    hipStream_t stream = nullptr;
+    int steps_done = 0;
+    int64_t neurons = 0, synapses = 0;
+    ~Cohort() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        pool.arena.release();
+        io.release();
+    }
+};
+
+// Seeded stable task: rows of A scaled so |A|_inf = 0.9 (spectral radius
+// < 1); B entries in [-0.5, 0.5]; x0 in [-1, 1].
+void make_task(uint32_t seed, float *t) {
+    uint64_t z = 0x9e3779b97f4a7c15ull * (uint64_t(seed) + 1);
+    auto next = [&]() {   // SplitMix64
+        z += 0x9e3779b97f4a7c15ull;
+        uint64_t r = z;
+        r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+        r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+        r ^= r >> 31;
+        return float(double(r >> 11) * (1.0 / 9007199254740992.0));
+    };
+    std::memset(t, 0, sizeof(float) * kTaskFloats);
+    for (int i = 0; i < kDims; ++i) t[T_X + i] = 2.0f * next() - 1.0f;
+    for (int i = 0; i < kDims; ++i) {
+        float row[kDims], norm = 0;
+        for (int j = 0; j < kDims; ++j) { row[j] = 2.0f * next() - 1.0f; norm += std::fabs(row[j]); }
+        for (int j = 0; j < kDims; ++j) t[T_A + i * kDims + j] = 0.9f * row[j] / norm;
+    }
+    for (int k = 0; k < 8; ++k) t[T_B + k] = next() - 0.5f;
+}
+
+struct Server {
+    ns_serve_config cfg;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request> queue;
+    std::vector<std::unique_ptr<Cohort>> live;
+    std::vector<ns_serve_result> done;
+    std::vector<hipStream_t> streams;
+    std::thread worker;
+    bool stop = false;
+    std::string failure;
+    int64_t next_id = 0, used_n = 0, used_s = 0, used_e = 0;
+    ns_serve_stats stats{};
+    size_t stream_rr = 0;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    std::unique_ptr<Cohort> admit_locked() {
+        std::unique_ptr<Cohort> c;
+        while (!queue.empty()) {
+            Request &r = queue.front();
+            const int64_t n = int64_t(r.brain.genes.size()), s = int64_t(r.brain.edges.size());
+            if (used_n + n > cfg.neuron_capacity || used_s + s > cfg.synapse_capacity ||
+                used_e + 1 > cfg.episode_capacity)
+                break;
+            if (!c) c.reset(new Cohort);
+            used_n += n; used_s += s; used_e += 1;
+            c->neurons += n; c->synapses += s;
+            r.admit_s = now();
+            c->reqs.push_back(std::move(r));
+            queue.pop_front();
+        }
+        return c;
+    }
+
+    void start_cohort(Cohort &c) {
+        const int count = int(c.reqs.size());
+        for (int i = 0; i < count; ++i)
+            c.specs.push_back(BrainSpec{&c.reqs[i].brain, i, i * kChannels, i * kOutputs, false});
+        c.stream = streams[stream_rr++ % streams.size()];
+        c.pool.build(c.specs, kOutputs, Tile{256, 2, 4}, c.stream);
+        c.d_input = c.io.alloc<float>(int64_t(count) * kChannels);
+        c.d_out = c.io.alloc<float>(int64_t(count) * kOutputs);
+        c.d_task = c.io.alloc<float>(int64_t(count) * kTaskFloats);
+        c.d_rng = c.io.alloc<uint32_t>(count);
+        std::vector<float> task(size_t(count) * kTaskFloats);
+        std::vector<uint32_t> rng(count), bseed(count);
+        std::vector<uint8_t> run(count);
+        for (int i = 0; i < count; ++i) {
+            make_task(c.reqs[i].task_seed, &task[size_t(i) * kTaskFloats]);
+            rng[i] = c.reqs[i].task_seed * 2654435761u + 12345u;
+            if (!rng[i]) rng[i] = 1;
+        }
+        for (int pb = 0; pb < count; ++pb) {
+            bseed[pb] = c.reqs[c.pool.batch_of[pb]].seed;
+            run[pb] = uint8_t(1 | (c.pool.fallback[pb] ? 2 : 0));
+        }
+        NS_HIP(hipMemcpy(c.d_task, task.data(), sizeof(float) * task.size(), hipMemcpyHostToDevice));
+        NS_HIP(hipMemcpy(c.d_rng, rng.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
+        NS_HIP(hipMemcpy(c.pool.d_bseed, bseed.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
+        NS_HIP(hipMemcpy(c.pool.view.brun, run.data(), count, hipMemcpyHostToDevice));
+        launch_reset(c.pool.view, c.pool.nn, c.pool.ns, count, c.pool.d_bseed, c.pool.d_norig,
+                     c.pool.d_sorig, c.stream);
+        NS_HIP(hipGetLastError());
+    }
+
+    void enqueue_steps(Cohort &c, int steps) {
+        const int count = int(c.reqs.size());
+        const unsigned blocks = unsigned((count + 127) / 128);
+        for (int k = 0; k < steps; ++k) {
+            hipLaunchKernelGGL(k_encode, dim3(blocks), dim3(128), 0, c.stream, c.d_task, count,
+                               c.d_input);
+            run_pool(c.pool, NS_FORM_RESIDENT, cfg.ticks_per_step, c.d_input, cfg.input_gain,
+                     c.d_out, int64_t(count) * kOutputs, 1, c.stream);
+            hipLaunchKernelGGL(k_env, dim3(blocks), dim3(128), 0, c.stream, c.d_task, c.d_rng,
+                               c.d_out, count, float(cfg.ticks_per_step), cfg.noise_sigma);
+        }
+        NS_HIP(hipGetLastError());
+    }
+
+    void finish_cohort(Cohort &c) {
+        const int count = int(c.reqs.size());
+        std::vector<float> task(size_t(count) * kTaskFloats);
+        std::vector<int32_t> status(count);
+        NS_HIP(hipStreamSynchronize(c.stream));
+        NS_HIP(hipMemcpy(task.data(), c.d_task, sizeof(float) * task.size(), hipMemcpyDeviceToHost));
+        const double t = now();
+        std::lock_guard<std::mutex> lock(mu);
+        for (int i = 0; i < count; ++i) {
+            const float *tk = &task[size_t(i) * kTaskFloats];
+            ns_serve_result r{};
+            r.id = c.reqs[i].id;
+            r.submit_s = c.reqs[i].submit_s;
+            r.admit_s = c.reqs[i].admit_s;
+            r.finish_s = t;
+            r.episode_return = tk[T_RET];
+            for (int d = 0; d < kDims; ++d) r.final_state[d] = tk[T_X + d];
+            r.motor_spikes[0] = int32_t(tk[T_M0]);
+            r.motor_spikes[1] = int32_t(tk[T_M1]);
+            r.status = 0;
+            done.push_back(r);
+        }
+        used_n -= c.neurons; used_s -= c.synapses; used_e -= count;
+        stats.finished += count;
+        cv.notify_all();
+    }
+
+    void loop() {
+        try {
+            while (true) {
+                std::unique_ptr<Cohort> fresh;
+                {
+                    std::unique_lock<std::mutex> lock(mu);
+                    if (live.empty())
+                        cv.wait(lock, [&] { return stop || !queue.empty(); });
+                    if (stop && queue.empty() && live.empty()) break;
+                    fresh = admit_locked();
+                    if (fresh) stats.admitted += int64_t(fresh->reqs.size());
+                }
+                const auto busy0 = std::chrono::steady_clock::now();
+                if (fresh) {
+                    start_cohort(*fresh);
+                    live.push_back(std::move(fresh));
+                }
+                if (int64_t(live.size()) > stats.resident_peak) stats.resident_peak = int64_t(live.size());
+                for (auto &c : live) {
+                    const int steps = std::min(cfg.steps_per_launch, cfg.steps_per_episode - c->steps_done);
+                    enqueue_steps(*c, steps);
+                    c->steps_done += steps;
+                }
+                stats.launches += 1;
+                for (auto &c : live) NS_HIP(hipStreamSynchronize(c->stream));
+                stats.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy0).count();
+                for (size_t k = 0; k < live.size();) {
+                    if (live[k]->steps_done >= cfg.steps_per_episode) {
+                        finish_cohort(*live[k]);
+                        live.erase(live.begin() + k);
+                    } else {
+                        ++k;
+                    }
+                }
+            }
+        } catch (const std::exception &e) {
+            std::lock_guard<std::mutex> lock(mu);
+            failure = e.what();
+            stop = true;
+            cv.notify_all();
+        }
+    }
+};
+}  // namespace
+
+}  // namespace nsa
+
+using namespace nsa;
+
+extern "C" {
+
+void *ns_serve_create(const ns_serve_config *config) {
+    try {
+        if (!config || config->neuron_capacity < 1 || config->synapse_capacity < 0 ||
+            config->episode_capacity < 1 || config->steps_per_episode < 1 ||
+            config->ticks_per_step < 1 || config->ticks_per_step > 10000 ||
+            config->steps_per_launch < 1 || !(config->noise_sigma >= 0) || !(config->input_gain >= 0))
+            throw std::runtime_error("Invalid serving configuration");
+        auto *s = new Server;
+        s->cfg = *config;
+        s->t0 = std::chrono::steady_clock::now();
+        try {
+            for (int k = 0; k < 4; ++k) {
+                hipStream_t st;
+                NS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+                s->streams.push_back(st);
+            }
+            s->worker = std::thread([s] { s->loop(); });
+        } catch (...) {
+            for (auto st : s->streams) (void)hipStreamDestroy(st);
+            delete s;
+            throw;
+        }
+        return s;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void ns_serve_destroy(void *p) {
+    auto *s = static_cast<Server *>(p);
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+        s->queue.clear();
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    s->live.clear();
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    delete s;
+}
+
+int64_t ns_serve_submit(void *p, int count, int edges, const float *ng, const float *sg,
+                        const int *motors, uint32_t seed, uint32_t task_seed) {
+    try {
+        auto &s = *static_cast<Server *>(p);
+        std::unique_ptr<HostBrain> b(parse_gym_brain(count, edges, kChannels, kOutputs, ng, sg,
+                                                     motors, 8));
+        if (count > s.cfg.neuron_capacity || edges > s.cfg.synapse_capacity)
+            throw std::runtime_error("Episode exceeds the pool capacity");
+        std::lock_guard<std::mutex> lock(s.mu);
+        if (!s.failure.empty()) throw std::runtime_error("Server failed: " + s.failure);
+        Request r;
+        r.id = s.next_id++;
+        r.brain = std::move(*b);
+        r.seed = seed;
+        r.task_seed = task_seed;
+        r.submit_s = s.now();
+        s.queue.push_back(std::move(r));
+        s.stats.submitted += 1;
+        s.cv.notify_all();
+        return s.next_id - 1;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_serve_poll(void *p, ns_serve_result *out, int capacity) {
+    auto &s = *static_cast<Server *>(p);
+    std::lock_guard<std::mutex> lock(s.mu);
+    if (!s.failure.empty()) {
+        set_error("Server failed: " + s.failure);
+        return -1;
+    }
+    const int n = std::min<int>(capacity, int(s.done.size()));
+    std::copy(s.done.begin(), s.done.begin() + n, out);
+    s.done.erase(s.done.begin(), s.done.begin() + n);
+    return n;
+}
+
+int ns_serve_drain(void *p) {
+    auto &s = *static_cast<Server *>(p);
+    std::unique_lock<std::mutex> lock(s.mu);
+    s.cv.wait(lock, [&] { return !s.failure.empty() || s.stats.finished == s.stats.submitted; });
+    if (!s.failure.empty()) {
+        set_error("Server failed: " + s.failure);
+        return -1;
+    }
+    return 0;
+}
+
+int ns_serve_stats_get(void *p, ns_serve_stats *stats) {
+    auto &s = *static_cast<Server *>(p);
+    std::lock_guard<std::mutex> lock(s.mu);
+    *stats = s.stats;
+    return 0;
+}
+
+}  // extern "C"

@@ -47,6 +47,7 @@ def _pad_mask(dtype):
     return m
 
 
+NEURON_GENE_BYTES = _pad_mask(NEURON_GENE)
 NEURON_STATE_BYTES = _pad_mask(NEURON_STATE)
 SYNAPSE_STATE_BYTES = _pad_mask(SYNAPSE_STATE)
 
@@ -231,8 +232,20 @@ class RefBrain:
         self.close()
 
 
+def cat_records(parts, dtype):
+    """Concatenate record arrays keeping the padded C layout (numpy's own
+    concatenate may drop the padding of structured dtypes)."""
+    out = np.zeros(sum(len(p) for p in parts), dtype)
+    raw = out.view(np.uint8).reshape(len(out), dtype.itemsize)
+    i = 0
+    for p in parts:
+        raw[i:i + len(p)] = np.ascontiguousarray(p).view(np.uint8).reshape(len(p), dtype.itemsize)
+        i += len(p)
+    return out
+
+
 def state_bytes(arr, mask):
     """Record bytes with padding bytes zeroed, for bit-exact comparison."""
-    b = arr.view(np.uint8).reshape(len(arr), -1).copy()
+    b = np.ascontiguousarray(arr).view(np.uint8).reshape(len(arr), len(mask)).copy()
     b[:, ~mask] = 0
     return b

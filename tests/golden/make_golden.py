@@ -1,0 +1,274 @@
+"""Write the golden vectors under tests/golden/ from the reference's own CPU
+path (gym_adapter/native.cpp + src/physiology.cuh + gym_adapter/core.cuh,
+host-compiled by oracle/Makefile into oracle/_ref/libns_ref.so).
+
+Run in a container that has the reference checkout (make -C oracle first):
+    python tests/golden/make_golden.py
+
+Fixtures (numpy .npz, data only):
+  c1_reference.npz    C1 workload (64 brains x 32 neurons, 200 calls x 16
+                      ticks): motor outputs per call, SHA-256 of the full
+                      neuron state / synapse state / 9-slot history per call,
+                      and the final state of brains 0-3.
+  c1_ticks.npz        8 C1 brains stepped one tick per call for 320 ticks:
+                      the spike raster (history slot of each tick).
+  mix_reference.npz   ragged brains with every cell type (0-6), plasticity
+                      profiles 1-5, modulators and inactive cells: per-call
+                      outputs and state hashes, final state.
+  functions.npz       per-function input/output records for neuron_advance
+                      (all 7 types), receptor_current, synapse_release,
+                      synapse_plasticity_step (profiles 0-5), homeostatic
+                      advance, initial states and profiles.
+"""
+import hashlib
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path[:0] = [str(ROOT / 'oracle'), str(ROOT / 'neural-selection_amd')]
+import oracle as O  # noqa: E402
+import ns_workloads as W  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def digest(arr, mask=None):
+    b = O.state_bytes(arr, mask) if mask is not None else np.ascontiguousarray(arr).view(np.uint8)
+    return np.frombuffer(hashlib.sha256(b.tobytes()).digest(), np.uint8)
+
+
+def run_workload(w, calls, keep_final=4):
+    seeds = w.seeds()
+    brains = []
+    for i in range(w.count):
+        n, s, m = w.brain(i)
+        b = O.RefBrain(n, s, m, w.channels)
+        b.reset(seeds[i])
+        brains.append(b)
+    outputs = np.zeros((calls, w.count, w.outputs), np.float32)
+    hashes = np.zeros((calls, 3, 32), np.uint8)
+    for c in range(calls):
+        x = w.inputs(c)
+        for i, b in enumerate(brains):
+            outputs[c, i], rc = b.step(x[i], w.ticks, w.gain)
+            assert rc == 0
+        parts = [b.export() for b in brains]
+        ns = O.cat_records([p[0] for p in parts], O.NEURON_STATE)
+        ss = O.cat_records([p[1] for p in parts], O.SYNAPSE_STATE)
+        h9 = np.concatenate([p[2] for p in parts])
+        hashes[c, 0] = digest(ns, O.NEURON_STATE_BYTES)
+        hashes[c, 1] = digest(ss, O.SYNAPSE_STATE_BYTES)
+        hashes[c, 2] = digest(h9)
+    final = [b.export() for b in brains[:keep_final]]
+    return outputs, hashes, final
+
+
+def c1_fixture():
+    w = W.c1(calls=200)
+    outputs, hashes, final = run_workload(w, 200)
+    np.savez_compressed(OUT / 'c1_reference.npz', outputs=outputs.astype(np.uint8), hashes=hashes,
+                        final_neurons=O.cat_records([f[0] for f in final], O.NEURON_STATE).view(np.uint8),
+                        final_synapses=O.cat_records([f[1] for f in final], O.SYNAPSE_STATE).view(np.uint8),
+                        final_history=np.concatenate([f[2] for f in final]),
+                        final_ticks=np.array([f[3] for f in final], np.int64))
+    print('c1_reference.npz', outputs.sum())
+
+
+def c1_ticks_fixture():
+    w = W.c1(calls=320, brains=8)
+    seeds = w.seeds()
+    raster = np.zeros((320, w.total_neurons), np.uint8)
+    brains = []
+    for i in range(w.count):
+        n, s, m = w.brain(i)
+        b = O.RefBrain(n, s, m, w.channels)
+        b.reset(seeds[i])
+        brains.append(b)
+    for t in range(320):
+        x = w.inputs(t)
+        row = []
+        for i, b in enumerate(brains):
+            b.step(x[i], 1, w.gain)
+            h9 = b.export()[2]
+            row.append(h9[:, t % 9])
+        raster[t] = np.concatenate(row)
+    np.savez_compressed(OUT / 'c1_ticks.npz', raster=np.packbits(raster, axis=1))
+    print('c1_ticks.npz', raster.sum())
+
+
+def mix_fixture():
+    w = W.small_mix(seed=12, brains=8, calls=40, types=(0, 1, 2, 3, 4, 5, 6))
+    outputs, hashes, final = run_workload(w, 40, keep_final=8)
+    np.savez_compressed(OUT / 'mix_reference.npz', outputs=outputs.astype(np.uint8), hashes=hashes,
+                        final_neurons=O.cat_records([f[0] for f in final], O.NEURON_STATE).view(np.uint8),
+                        final_synapses=O.cat_records([f[1] for f in final], O.SYNAPSE_STATE).view(np.uint8),
+                        final_history=np.concatenate([f[2] for f in final]),
+                        final_ticks=np.array([f[3] for f in final], np.int64))
+    print('mix_reference.npz', outputs.sum())
+
+
+def functions_fixture():
+    lib = O.ref_lib()
+    rng = np.random.default_rng(99)
+    rec = {}
+    # profiles and initial neuron states for every type
+    genes = np.zeros(7, O.NEURON_GENE)
+    for t in range(7):
+        lib.ref_fn_profile(t, genes[t:t + 1].ctypes.data)
+    rec['profiles'] = genes.view(np.uint8)
+    seeds = rng.integers(0, 2**32, 7 * 8, dtype=np.uint64).astype(np.uint32)
+    init = np.zeros(7 * 8, O.NEURON_STATE)
+    for k in range(7 * 8):
+        g = genes[k // 8:k // 8 + 1].copy()
+        g['sensory_channel'] = 0
+        lib.ref_fn_initial_neuron(g.ctypes.data, int(seeds[k]), init[k:k + 1].ctypes.data)
+    rec['init_neuron_seeds'] = seeds
+    rec['init_neurons'] = init.view(np.uint8)
+    # neuron_advance: random states around rest, random inputs, all types
+    cases = []
+    for t in range(7):
+        for k in range(48 if t in (2, 3) else 256):
+            g = genes[t:t + 1].copy()
+            g['sensory_channel'] = 0
+            s = np.zeros(1, O.NEURON_STATE)
+            lib.ref_fn_initial_neuron(g.ctypes.data, int(rng.integers(1, 2**31)), s.ctypes.data)
+            if t not in (2, 3):
+                s['f'][0, 0] = g['f'][0, 2] + rng.normal(0, 8)
+                s['f'][0, 1] = rng.normal(0, 30)
+                s['f'][0, 2] = abs(rng.normal(0, 40))
+                s['f'][0, 3] = abs(rng.normal(0, 3))
+                s['f'][0, 4] = abs(rng.normal(0, 2))
+                s['dend'][0] = np.abs(rng.normal(0, 3, 4))
+            exc = np.abs(rng.normal(0, 2, 4)).astype(np.float32) * (rng.random(4) < 0.5)
+            inh = np.float32(abs(rng.normal(0, 2)) * (rng.random() < 0.5))
+            cur = np.float32(rng.normal(0, 150))
+            before = s.copy()
+            lib.ref_fn_neuron_advance(g.ctypes.data, s.ctypes.data, float(cur),
+                                      exc.ctypes.data_as(O._f), float(inh))
+            cases.append((t, before, exc, inh, cur, s))
+    rec['adv_type'] = np.array([c[0] for c in cases], np.uint8)
+    rec['adv_before'] = O.cat_records([c[1] for c in cases], O.NEURON_STATE).view(np.uint8)
+    rec['adv_exc'] = np.stack([c[2] for c in cases])
+    rec['adv_inh'] = np.array([c[3] for c in cases], np.float32)
+    rec['adv_current'] = np.array([c[4] for c in cases], np.float32)
+    rec['adv_after'] = O.cat_records([c[5] for c in cases], O.NEURON_STATE).view(np.uint8)
+    # receptor_current (type 5)
+    g5 = genes[5:6].copy()
+    g5['sensory_channel'] = 0
+    rc_in, rc_lvl, rc_out, rc_val = [], [], [], []
+    for k in range(512):
+        s = np.zeros(1, O.NEURON_STATE)
+        lib.ref_fn_initial_neuron(g5.ctypes.data, int(rng.integers(1, 2**31)), s.ctypes.data)
+        s['receptor_ns'] = abs(rng.normal(0, 2))
+        s['f'][0, 0] = rng.normal(-55, 10)
+        lvl = np.float32(rng.random())
+        rc_in.append(s.copy())
+        v = lib.ref_fn_receptor_current(g5.ctypes.data, s.ctypes.data, float(lvl))
+        rc_lvl.append(lvl)
+        rc_out.append(s)
+        rc_val.append(v)
+    rec['rc_gene'] = g5.view(np.uint8)
+    rec['rc_before'] = O.cat_records(rc_in, O.NEURON_STATE).view(np.uint8)
+    rec['rc_level'] = np.array(rc_lvl, np.float32)
+    rec['rc_after'] = O.cat_records(rc_out, O.NEURON_STATE).view(np.uint8)
+    rec['rc_value'] = np.array(rc_val, np.float32)
+    # synapse_release
+    sg = np.zeros(1024, O.SYNAPSE_GENE)
+    sg['initial_weight_ns'] = rng.lognormal(-1, 0.5, 1024)
+    sg['utilization'] = rng.uniform(0.05, 1.0, 1024)
+    sg['recovery_ms'] = rng.uniform(20, 800, 1024)
+    sg['facilitation_ms'] = np.where(rng.random(1024) < 0.4, rng.uniform(10, 2000, 1024), 0)
+    sg['release_sites'] = rng.choice([0, 1, 5, 14, 50, 63, 64, 80], 1024)
+    sg['slow_recovery_ms'] = np.where(rng.random(1024) < 0.3, 7500.0, 0.0)
+    sg['slow_depletion'] = 0.023
+    sg['active'] = 1
+    ss_before = np.zeros(1024, O.SYNAPSE_STATE)
+    ss_after = np.zeros(1024, O.SYNAPSE_STATE)
+    ticks = np.zeros(1024, np.int64)
+    vals = np.zeros(1024, np.float32)
+    for k in range(1024):
+        s = np.zeros(1, O.SYNAPSE_STATE)
+        lib.ref_fn_initial_synapse(sg[k:k + 1].ctypes.data, int(rng.integers(1, 2**31)), s.ctypes.data)
+        t0 = int(rng.integers(0, 5000))
+        if rng.random() < 0.7:   # a history of arrivals
+            for j in range(int(rng.integers(1, 6))):
+                t0 += int(rng.integers(1, 200))
+                lib.ref_fn_synapse_release(sg[k:k + 1].ctypes.data, s.ctypes.data, t0)
+        ss_before[k] = s[0]
+        ticks[k] = t0 + int(rng.integers(1, 400))
+        vals[k] = lib.ref_fn_synapse_release(sg[k:k + 1].ctypes.data, s.ctypes.data, int(ticks[k]))
+        ss_after[k] = s[0]
+    rec['rel_genes'] = sg.view(np.uint8)
+    rec['rel_before'] = ss_before.view(np.uint8)
+    rec['rel_tick'] = ticks
+    rec['rel_after'] = ss_after.view(np.uint8)
+    rec['rel_value'] = vals
+    # synapse_plasticity_step: every profile, random traces and events
+    n = 6 * 400
+    pg = np.zeros(n, O.SYNAPSE_GENE)
+    pg['plasticity'] = np.repeat(np.arange(6), 400)
+    pg['initial_weight_ns'] = rng.lognormal(-1, 0.5, n)
+    pg['mod_receptors'] = rng.integers(0, 8, n)
+    pb = np.zeros(n, O.SYNAPSE_STATE)
+    pa = np.zeros(n, O.SYNAPSE_STATE)
+    flags = np.zeros((n, 4), np.int32)
+    depol = rng.uniform(0, 5, n).astype(np.float32)
+    for k in range(n):
+        s = np.zeros(1, O.SYNAPSE_STATE)
+        lib.ref_fn_initial_synapse(pg[k:k + 1].ctypes.data, 7, s.ctypes.data)
+        f = s['f'][0]
+        f[0] = pg['initial_weight_ns'][k] * rng.uniform(0, 2)
+        f[2] = rng.uniform(0, 3); f[3] = rng.uniform(0, 3)
+        f[4] = rng.uniform(0, 5); f[5] = rng.uniform(0, 5)
+        f[6] = rng.choice([-1.0, rng.uniform(0, 25000)])
+        f[7] = rng.uniform(1, 2)
+        f[8] = rng.choice([-1.0, rng.uniform(0, 6000)])
+        f[9] = float(rng.integers(0, 60)); f[10] = float(rng.integers(0, 60))
+        s['f'][0] = f
+        pb[k] = s[0]
+        fl = [int(rng.random() < 0.4), int(rng.random() < 0.4), int(rng.random() < 0.9),
+              int(rng.integers(0, 8))]
+        flags[k] = fl
+        lib.ref_fn_plasticity(pg[k:k + 1].ctypes.data, s.ctypes.data, fl[0], fl[1], fl[2],
+                              float(depol[k]), fl[3])
+        pa[k] = s[0]
+    rec['pl_genes'] = pg.view(np.uint8)
+    rec['pl_before'] = pb.view(np.uint8)
+    rec['pl_flags'] = flags
+    rec['pl_depol'] = depol
+    rec['pl_after'] = pa.view(np.uint8)
+    # homeostatic_advance (type 0 responds; others unchanged)
+    hs_b = np.zeros(256, O.NEURON_STATE)
+    hs_b['f'][:, 5] = rng.uniform(-3, 3, 256)
+    hz = rng.uniform(0, 3, 256).astype(np.float32)
+    el = rng.choice([1.0, 16.0, 160.0, 1e5], 256).astype(np.float32)
+    hs_a = hs_b.copy()
+    for k in range(256):
+        lib.ref_fn_homeostasis(genes[k % 7:k % 7 + 1].ctypes.data, hs_a[k:k + 1].ctypes.data,
+                               float(el[k]), float(hz[k]))
+    rec['ho_type'] = (np.arange(256) % 7).astype(np.uint8)
+    rec['ho_before'] = hs_b.view(np.uint8)
+    rec['ho_elapsed'] = el
+    rec['ho_hz'] = hz
+    rec['ho_after'] = hs_a.view(np.uint8)
+    # initial synapse states
+    syn_seeds = rng.integers(0, 2**32, 256, dtype=np.uint64).astype(np.uint32)
+    si = np.zeros(256, O.SYNAPSE_STATE)
+    for k in range(256):
+        lib.ref_fn_initial_synapse(sg[k:k + 1].ctypes.data, int(syn_seeds[k]), si[k:k + 1].ctypes.data)
+    rec['init_syn_seeds'] = syn_seeds
+    rec['init_synapses'] = si.view(np.uint8)
+    np.savez_compressed(OUT / 'functions.npz', **rec)
+    print('functions.npz', len(cases), 'advance cases')
+
+
+if __name__ == '__main__':
+    if not O.ref_available():
+        sys.exit('oracle/_ref/libns_ref.so missing: build it with make -C oracle (needs the '
+                 'reference checkout)')
+    c1_fixture()
+    c1_ticks_fixture()
+    mix_fixture()
+    functions_fixture()

@@ -31,8 +31,8 @@ def oracle_step_all(brains, x, ticks, gain, mask=None):
 
 def oracle_export_all(brains):
     parts = [b.export() for b in brains]
-    ns = np.concatenate([p[0] for p in parts])
-    ss = np.concatenate([p[1] for p in parts]) if sum(len(p[1]) for p in parts) else np.zeros(0, O.SYNAPSE_STATE)
+    ns = O.cat_records([p[0] for p in parts], O.NEURON_STATE)
+    ss = O.cat_records([p[1] for p in parts], O.SYNAPSE_STATE) if sum(len(p[1]) for p in parts) else np.zeros(0, O.SYNAPSE_STATE)
     h9 = np.concatenate([p[2] for p in parts])
     ticks = np.array([p[3] for p in parts], np.int64)
     return ns, ss, h9, ticks
