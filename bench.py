@@ -1,0 +1,253 @@
+"""Benchmark of the per-tick brain update on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
+
+A step is one world step: 16 one-ms ticks of every brain in the workload
+(one resident tick-kernel launch, plus the per-tick kernels for brains too
+large for a workgroup), with the encoded sensory input already resident in
+HBM.  Prints one JSON line: neuron-updates/s (value), synaptic events/s,
+synapse-updates/s, p50/p99 world-step latency, the plain per-tick form's
+throughput on the same pools, the roofline of the resident tick kernel
+(HIP-event durations measured here, on the stream it runs on) and the CPU
+oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / 'neural-selection_amd'), str(ROOT / 'oracle')]
+
+import ns_amd  # noqa: E402
+import ns_workloads as W  # noqa: E402
+
+HBM_PEAK = 8.0e12     # MI355X HBM3E, bytes/s
+# Algorithmic bytes per unit of work (SoA-minimal streaming form, one tick):
+NEURON_UPDATE_BYTES = 178
+SYNAPSE_UPDATE_BYTES_PLASTIC = 66
+SYNAPSE_UPDATE_BYTES_STATIC = 12
+EVENT_BYTES = 78
+
+
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    if name == 'c1':
+        return W.c1(), False
+    if name == 'c2':
+        return W.c2(), False
+    if name == 'c3':
+        return W.c3(), True
+    raise SystemExit(f'unknown workload {name}')
+
+
+def describe(name, w):
+    base = {'workload': name, 'brains': w.count, 'neurons': w.total_neurons,
+            'synapses': w.total_synapses, 'ticks_per_step': w.ticks, 'max_delay': w.max_delay}
+    if name == 'c3':
+        base.update(desc='capacity scale with plasticity: 32768 brains, Pareto(1.5, 32) sizes '
+                         'truncated at 4096 and rescaled to 4194304 neurons, 2 synapses/neuron, '
+                         'STDP profiles 1/2 with eligibility (3), D1 dopamine (4) and spine (5) '
+                         'profiles, modulator neurons; births/deaths churn not included',
+                    plasticity=True)
+    elif name == 'c2':
+        base.update(desc='2048 brains, lognormal(ln 96, 0.75) sizes clipped 16..1024, 20 '
+                         'synapses/neuron, delays 1..16, STP on, plasticity off; the world '
+                         '(grid, food) is not included', plasticity=False)
+    else:
+        base.update(desc='64 brains x 32 neurons, fan-out 8, delays 1..8, plasticity off',
+                    plasticity=False)
+    return base
+
+
+def cpu_baseline(w, seconds=12.0, max_neurons=65536):
+    """The CPU oracle (bit-identical to the reference's CPU path) on the host
+    cores, over the first brains of the same workload."""
+    import ctypes as ct
+    import oracle as O
+    lib = O.oracle_lib()
+    n_b, n_n = 0, 0
+    while n_b < w.count and n_n + int(w.neuron_counts[n_b]) <= max_neurons:
+        n_n += int(w.neuron_counts[n_b])
+        n_b += 1
+    n_b = max(n_b, 1)
+    seeds = w.seeds()
+    brains = []
+    for i in range(n_b):
+        n, s, m = w.brain(i)
+        b = O.OracleBrain(n, s, m, w.channels, max_delay=max(8, w.max_delay))
+        b.reset(seeds[i])
+        brains.append(b)
+    threads = min(os.cpu_count() or 1, 16)
+    arr = (ct.c_void_p * n_b)(*[b.ptr for b in brains])
+    out = np.zeros((n_b, w.outputs), np.float32)
+    status = np.zeros(n_b, np.int32)
+    neurons = int(w.neuron_counts[:n_b].sum())
+    calls, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and calls < 400:
+        x = np.ascontiguousarray(w.inputs(calls)[:n_b])
+        lib.ora_batch_step(arr, n_b, x.ctypes.data_as(O._f), w.channels, None, w.ticks, w.gain,
+                           out.ctypes.data_as(O._f), w.outputs, status.ctypes.data_as(O._i32),
+                           threads)
+        calls += 1
+    dt = time.perf_counter() - t0
+    value = neurons * w.ticks * calls / dt
+    return {'value': value, 'unit': 'neuron-updates/s', 'cores': threads, 'kind': 'port',
+            'sample': f'first {n_b} brains ({neurons} neurons) of the same workload, {calls} calls '
+                      f'x {w.ticks} ticks, {dt:.1f} s wall, oracle/liboracle.so threaded over brains'}
+
+
+def load_traffic(name):
+    f = ROOT / 'profiles' / f'k_resident_{name}_hbm.json'
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get('bytes_per_launch')
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--workload', default='c3')
+    ap.add_argument('--plain-steps', type=int, default=5)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-autotune', action='store_true')
+    ap.add_argument('--tiles', default='')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU only (asked for {args.gpus}); brains are independent, '
+              'so the update shards by brain with no exchange (see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.init()
+
+    log(f'building workload {args.workload}')
+    t0 = time.perf_counter()
+    w, plastic = make_workload(args.workload)
+    log(f'{w.count} brains, {w.total_neurons} neurons, {w.total_synapses} synapses '
+        f'({time.perf_counter() - t0:.1f} s)')
+    batch = ns_amd.GpuBatch(workload=w)
+    batch.reset(w.seeds())
+    if args.tiles:
+        batch.set_tiles(*[int(x) for x in args.tiles.split(',')])
+    elif not args.no_autotune:
+        log('autotuning tile shape')
+        tiles = batch.autotune(w.ticks)
+        log(f'tile {tiles}')
+    info = batch.info()
+    log(f'packs {info["resident_packs"]} fallback brains {info["fallback_brains"]} '
+        f'({info["fallback_neurons"]} neurons) device {info["device_bytes"] / 2**30:.2f} GiB')
+
+    patterns = 8
+    inputs = torch.from_numpy(np.stack([w.inputs(c) for c in range(patterns)])).cuda()
+    out = torch.zeros((w.count, w.outputs), dtype=torch.float32, device='cuda')
+    stream = torch.cuda.current_stream()
+
+    def step(k):
+        batch.step_device(inputs[k % patterns].data_ptr(), w.ticks, w.gain, out.data_ptr(),
+                          stream=stream.cuda_stream)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    batch.set_timing(True)
+    i0 = batch.info()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        evs[k].record(stream)
+        step(args.warmup + k)
+    evs[-1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    i1 = batch.info()
+    step_ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)])
+    log(f'{args.steps} steps in {wall:.3f} s; step p50 {np.percentile(step_ms, 50):.3f} ms')
+
+    nu = w.total_neurons * w.ticks * args.steps
+    su = w.total_synapses * w.ticks * args.steps
+    ev = i1['events'] - i0['events']
+    value = nu / wall
+    res_launches = i1['resident_launches'] - i0['resident_launches']
+    res_ms = (i1['resident_ms'] - i0['resident_ms']) / max(res_launches, 1)
+    res_neurons = w.total_neurons - i1['fallback_neurons']
+    res_synapses = w.total_synapses - i1['fallback_synapses']
+    res_events = (i1['resident_events'] - i0['resident_events']) / max(res_launches, 1)
+    syn_bytes = SYNAPSE_UPDATE_BYTES_PLASTIC if plastic else SYNAPSE_UPDATE_BYTES_STATIC
+    alg_bytes = (NEURON_UPDATE_BYTES * res_neurons * w.ticks + syn_bytes * res_synapses * w.ticks
+                 + EVENT_BYTES * res_events)
+    achieved = alg_bytes / (res_ms * 1e-3) if res_ms > 0 else 0.0
+    per_tick_share = (i1['per_tick_ms'] - i0['per_tick_ms']) / max(args.steps, 1)
+
+    # plain per-tick form on the same pools (the 5x denominator)
+    batch.set_timing(False)
+    batch.set_form(ns_amd.FORM_PER_TICK)
+    for k in range(2):
+        step(k)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for k in range(args.plain_steps):
+        step(k)
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t1
+    plain_value = w.total_neurons * w.ticks * args.plain_steps / plain_wall
+    batch.set_form(ns_amd.FORM_RESIDENT)
+    log(f'resident {value:.3e} vs plain per-tick {plain_value:.3e} neuron-updates/s')
+
+    cpu = None
+    if not args.no_cpu:
+        log('timing the CPU oracle on a bounded sample')
+        cpu = cpu_baseline(w)
+
+    tile = (i1['threads'], i1['neurons_per_thread'], i1['synapses_per_thread'])
+    line = {
+        'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
+        'value': value,
+        'unit': 'neuron-updates/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': wall * 1e3 / args.steps,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded genomes and Poisson sensory input from ns_workloads.py)',
+        'config': describe(args.workload, w),
+        'synaptic_events_per_s': ev / wall,
+        'synapse_updates_per_s': su / wall,
+        'step_latency_ms': {'p50': float(np.percentile(step_ms, 50)),
+                            'p99': float(np.percentile(step_ms, 99)),
+                            'max': float(step_ms.max())},
+        'plain_per_tick': {'value': plain_value, 'speedup': value / plain_value,
+                           'steps': args.plain_steps},
+        'tile': {'threads': tile[0], 'neurons_per_thread': tile[1], 'synapses_per_thread': tile[2],
+                 'packs': i1['resident_packs'], 'fallback_brains': i1['fallback_brains'],
+                 'fallback_neurons': i1['fallback_neurons'],
+                 'per_tick_ms_per_step': per_tick_share},
+        'roofline': {'bound': 'hbm', 'kernel': 'k_resident', 'achieved': achieved,
+                     'peak': HBM_PEAK, 'unit': 'B/s', 'frac': achieved / HBM_PEAK,
+                     'traffic': load_traffic(args.workload),
+                     'launch_ms': res_ms, 'algorithmic_bytes_per_launch': alg_bytes},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+    batch.close()
+
+
+if __name__ == '__main__':
+    main()

@@ -227,8 +227,25 @@ typedef struct ns_batch_info {
     int64_t events;             /* delivered arrivals since create */
     int threads, neurons_per_thread, synapses_per_thread, form;
     int64_t device_bytes;
+    /* with timing on: HIP-event time of the resident tick kernel launches
+     * and of the per-tick kernel sequences, on the stream they ran on */
+    double resident_ms, per_tick_ms;
+    int64_t resident_launches, per_tick_calls;
+    int64_t resident_events;    /* arrivals delivered inside resident launches */
+    int64_t fallback_neurons, fallback_synapses;   /* on the per-tick path */
 } ns_batch_info;
 int ns_gpu_info(void *batch, ns_batch_info *info);
+
+/* Record HIP events around every tick-kernel launch of this batch (adds one
+ * event pair per launch; totals read back by ns_gpu_info). */
+int ns_gpu_set_timing(void *batch, int on);
+
+/* Diagnostic: evaluate the device expf (fn = 0) or logf (fn = 1) on every
+ * float bit pattern u in [lo, hi) and write one digest per `chunk` inputs:
+ * sum of bits(f(u)) * (u | 1) mod 2^64, NaN results as 0x7fc00000.  Lets a
+ * host check the device math against its libm without moving 2^32 floats. */
+int ns_device_math_digest(int fn, uint64_t lo, uint64_t hi, int64_t chunk,
+                          uint64_t *digests);
 
 /* ------------------------------------------------------------------------ */
 /* 2. World-pool surface (src/main.cu)                                       */

@@ -345,12 +345,12 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     up(d_motor, motor);
     p.motor = d_motor;
     p.outputs = outs;
-    p.events = arena.alloc<unsigned long long>(1);
+    p.events = arena.alloc<unsigned long long>(2);
     NS_HIP(hipMemset(p.btick, 0, sizeof(int64_t) * B));
     NS_HIP(hipMemset(p.brun, 0, B));
     NS_HIP(hipMemset(p.bfrozen, 0, B));
     NS_HIP(hipMemset(p.bstatus, 0, sizeof(int32_t) * B));
-    NS_HIP(hipMemset(p.events, 0, sizeof(unsigned long long)));
+    NS_HIP(hipMemset(p.events, 0, 2 * sizeof(unsigned long long)));
     NS_HIP(hipMemset(p.hist, 0, sizeof(uint32_t) * N));
     NS_HIP(hipMemset(p.stamps, 0, sizeof(uint64_t) * 6 * N));
     NS_HIP(hipMemset(p.released, 0, sizeof(float) * S));
@@ -604,15 +604,19 @@ void Batch::set_run(const uint8_t *mask) {
 
 void Batch::run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
                 hipStream_t st) {
-    run_pool(pool, form, ticks, d_in, gain, d_out, int64_t(count) * outputs, homeostasis, st);
+    run_pool(pool, form, ticks, d_in, gain, d_out, int64_t(count) * outputs, homeostasis, st,
+             timing ? &timer : nullptr);
 }
 
 void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
-              int64_t out_count, int homeostasis, hipStream_t st) {
+              int64_t out_count, int homeostasis, hipStream_t st, KernelTimer *timer) {
+    hipEvent_t ev;
     launch_fill(d_out, out_count, 0.0f, st);
     launch_fill_i32(pool.view.bstatus, pool.nb, 0, st);
     if (form == NS_FORM_PER_TICK) {
+        if (timer) timer->begin(st, 1, ev);
         launch_per_tick(pool.view, pool.all_range, ticks, 1, d_in, gain, d_out, homeostasis, st);
+        if (timer) timer->end(st);
         return;
     }
     if (!pool.packs.empty()) {
@@ -623,11 +627,16 @@ void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float ga
                 kTiles[k].synapses_per_thread == pool.tile.ks)
                 shape = &kTiles[k];
         if (!shape) throw std::runtime_error("Unknown tile shape");
+        if (timer) timer->begin(st, 0, ev);
         NS_HIP(shape->launch(pool.view, pool.d_packs, int(pool.packs.size()), ticks, d_in, gain,
                              d_out, homeostasis, st));
+        if (timer) timer->end(st);
     }
-    if (pool.fallback_brains)
+    if (pool.fallback_brains) {
+        if (timer) timer->begin(st, 1, ev);
         launch_per_tick(pool.view, pool.fallback_range, ticks, 3, d_in, gain, d_out, homeostasis, st);
+        if (timer) timer->end(st);
+    }
 }
 
 Batch::~Batch() {
@@ -815,6 +824,20 @@ int ns_gpu_step_device(void *p, const float *input, const uint8_t *mask, int tic
     }
 }
 
+int ns_gpu_set_timing(void *p, int on) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        b.timer.resolve();
+        b.timer.ms[0] = b.timer.ms[1] = 0;
+        b.timer.count[0] = b.timer.count[1] = 0;
+        b.timing = on != 0;
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 int ns_gpu_set_form(void *p, int form) {
     auto &b = *static_cast<Batch *>(p);
     if (form != NS_FORM_RESIDENT && form != NS_FORM_PER_TICK) {
@@ -850,8 +873,8 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks) {
         std::vector<int64_t> tk;
         std::vector<uint64_t> stamps;
         b.pool.download_state(n, s, h, tk, &stamps);
-        unsigned long long events0 = 0;
-        NS_HIP(hipMemcpy(&events0, b.pool.view.events, sizeof events0, hipMemcpyDeviceToHost));
+        unsigned long long events0[2] = {0, 0};
+        NS_HIP(hipMemcpy(events0, b.pool.view.events, sizeof events0, hipMemcpyDeviceToHost));
         std::vector<uint8_t> ones(b.count, 1);
         std::vector<float> zeros(size_t(b.count) * b.channels, 0.0f);
         b.brun_valid = false;
@@ -882,7 +905,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks) {
         b.pool.build(b.specs, b.outputs, best_tile, b.stream);
         b.pool.upload_state(n, s, h, tk, &stamps);
         b.brun_valid = false;
-        NS_HIP(hipMemcpy(b.pool.view.events, &events0, sizeof events0, hipMemcpyHostToDevice));
+        NS_HIP(hipMemcpy(b.pool.view.events, events0, sizeof events0, hipMemcpyHostToDevice));
         if (threads) *threads = best_tile.threads;
         if (kn) *kn = best_tile.kn;
         if (ks) *ks = best_tile.ks;
@@ -984,20 +1007,49 @@ int ns_gpu_export_spike_bits(void *p, uint32_t *bits) {
 int ns_gpu_info(void *p, ns_batch_info *info) {
     try {
         auto &b = *static_cast<Batch *>(p);
-        unsigned long long events = 0;
+        unsigned long long events[2] = {0, 0};
         NS_HIP(hipStreamSynchronize(b.stream));
-        NS_HIP(hipMemcpy(&events, b.pool.view.events, sizeof events, hipMemcpyDeviceToHost));
+        NS_HIP(hipMemcpy(events, b.pool.view.events, sizeof events, hipMemcpyDeviceToHost));
         info->brains = b.count;
         info->neurons = b.pool.nn;
         info->synapses = b.pool.ns;
         info->resident_packs = int64_t(b.pool.packs.size());
         info->fallback_brains = b.pool.fallback_brains;
-        info->events = int64_t(events);
+        info->events = int64_t(events[0] + events[1]);
+        info->resident_events = int64_t(events[0]);
+        info->fallback_neurons = b.pool.fallback_range.n_hi - b.pool.fallback_range.n_lo;
+        info->fallback_synapses = b.pool.fallback_range.s_hi - b.pool.fallback_range.s_lo;
         info->threads = b.pool.tile.threads;
         info->neurons_per_thread = b.pool.tile.kn;
         info->synapses_per_thread = b.pool.tile.ks;
         info->form = b.form;
         info->device_bytes = b.pool.arena.bytes + b.io.bytes;
+        b.timer.resolve();
+        info->resident_ms = b.timer.ms[0];
+        info->per_tick_ms = b.timer.ms[1];
+        info->resident_launches = b.timer.count[0];
+        info->per_tick_calls = b.timer.count[1];
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_device_math_digest(int fn, uint64_t lo, uint64_t hi, int64_t chunk, uint64_t *digests) {
+    try {
+        if ((fn != 0 && fn != 1) || hi <= lo || hi > (uint64_t(1) << 32) || chunk < 1)
+            throw std::runtime_error("Invalid digest range");
+        const int64_t n = int64_t((hi - lo + uint64_t(chunk) - 1) / uint64_t(chunk));
+        unsigned long long *d = nullptr;
+        NS_HIP(hipMalloc(&d, sizeof(unsigned long long) * n));
+        NS_HIP(hipMemset(d, 0, sizeof(unsigned long long) * n));
+        launch_math_digest(fn, lo, hi, chunk, d, nullptr);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(digests, d, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        NS_HIP(e);
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());

@@ -95,7 +95,7 @@ struct Pool {
     int32_t *bstatus;
     const int32_t *motor;     // absolute neuron position per output slot
     int outputs;              // outputs per brain
-    unsigned long long *events;
+    unsigned long long *events;   // [0] resident launches, [1] per-tick launches
 };
 
 // ---------------------------------------------------------------------------

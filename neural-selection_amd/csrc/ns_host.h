@@ -111,8 +111,44 @@ public:
 // One call of `ticks` ticks over every brain whose run flag is set: the
 // resident kernel over packs plus the per-tick kernels over the rest (or the
 // per-tick kernels over everything when form == NS_FORM_PER_TICK).
+struct KernelTimer;
 void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
-              int64_t out_count, int homeostasis, hipStream_t st);
+              int64_t out_count, int homeostasis, hipStream_t st, KernelTimer *timer = nullptr);
+
+// HIP-event timing of tick-kernel launches (pairs resolved lazily).
+struct KernelTimer {
+    struct Pair { hipEvent_t a, b; int kind; };
+    std::vector<Pair> pending;
+    double ms[2] = {0, 0};
+    int64_t count[2] = {0, 0};
+    void begin(hipStream_t st, int kind, hipEvent_t &a) {
+        NS_HIP(hipEventCreate(&a));
+        NS_HIP(hipEventRecord(a, st));
+        pending.push_back({a, nullptr, kind});
+    }
+    void end(hipStream_t st) {
+        NS_HIP(hipEventCreate(&pending.back().b));
+        NS_HIP(hipEventRecord(pending.back().b, st));
+    }
+    void resolve() {
+        for (auto &p : pending) {
+            NS_HIP(hipEventSynchronize(p.b));
+            float t = 0;
+            NS_HIP(hipEventElapsedTime(&t, p.a, p.b));
+            ms[p.kind] += t;
+            count[p.kind] += 1;
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        pending.clear();
+    }
+    ~KernelTimer() {
+        for (auto &p : pending) {
+            (void)hipEventDestroy(p.a);
+            if (p.b) (void)hipEventDestroy(p.b);
+        }
+    }
+};
 
 // Batched-episode surface (gym_adapter/gpu.cu GpuBatch).
 struct Batch {
@@ -129,6 +165,8 @@ struct Batch {
     std::vector<uint8_t> h_brun;
     bool brun_valid = false;
     std::vector<int32_t> h_status;
+    bool timing = false;
+    KernelTimer timer;
 
     void create(std::vector<HostBrain> &&owned, int max_delay_unused);
     void relayout(const Tile &tile);

@@ -43,6 +43,8 @@ void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st);
 void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,
                      const float *input, float gain, float *out, int homeostasis,
                      hipStream_t st);
+void launch_math_digest(int fn, uint64_t lo, uint64_t hi, int64_t chunk,
+                        unsigned long long *digests, hipStream_t st);
 void launch_world_homeostasis(const Pool &p, int64_t nn, const int32_t *age, hipStream_t st);
 
 }  // namespace nsa

@@ -138,7 +138,7 @@ __global__ void k_deliver(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t nee
         }
     }
     uint64_t m = __ballot(arr);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(p.events, (unsigned long long)__popcll(m));
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&p.events[1], (unsigned long long)__popcll(m));
 }
 
 // Sequential per-neuron reduction of this tick's releases in ascending edge
@@ -612,7 +612,27 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
     for (int b = tid; b < pk.nb; b += WG)
         if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
     __syncthreads();
-    if (tid == 0 && events_l) atomicAdd(p.events, (unsigned long long)events_l);
+    if (tid == 0 && events_l) atomicAdd(&p.events[0], (unsigned long long)events_l);
+}
+
+// ---------------------------------------------------------------------------
+// Device math digest (diagnostic)
+// ---------------------------------------------------------------------------
+__global__ void k_math_digest(int fn, uint64_t lo, uint64_t hi, int64_t chunk,
+                              unsigned long long *digests) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t u = lo + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < hi; u += stride) {
+        const float x = u2f(uint32_t(u));
+        const float y = fn == 0 ? ns_expf(x, g_exp_tab) : ns_logf(x, g_log_tab);
+        const uint32_t bits = y != y ? 0x7fc00000u : f2u(y);
+        const unsigned long long term = (unsigned long long)bits * (unsigned long long)(u | 1u);
+        atomicAdd(&digests[(u - lo) / uint64_t(chunk)], term);
+    }
+}
+
+void launch_math_digest(int fn, uint64_t lo, uint64_t hi, int64_t chunk,
+                        unsigned long long *digests, hipStream_t st) {
+    hipLaunchKernelGGL(k_math_digest, dim3(256 * 64), dim3(256), 0, st, fn, lo, hi, chunk, digests);
 }
 
 // ---------------------------------------------------------------------------

@@ -32,7 +32,10 @@ class BatchInfo(ct.Structure):
                 ('resident_packs', ct.c_int64), ('fallback_brains', ct.c_int64),
                 ('events', ct.c_int64), ('threads', ct.c_int), ('neurons_per_thread', ct.c_int),
                 ('synapses_per_thread', ct.c_int), ('form', ct.c_int),
-                ('device_bytes', ct.c_int64)]
+                ('device_bytes', ct.c_int64), ('resident_ms', ct.c_double),
+                ('per_tick_ms', ct.c_double), ('resident_launches', ct.c_int64),
+                ('per_tick_calls', ct.c_int64), ('resident_events', ct.c_int64),
+                ('fallback_neurons', ct.c_int64), ('fallback_synapses', ct.c_int64)]
 
 
 class ServeConfig(ct.Structure):
@@ -75,6 +78,9 @@ SIGNATURES = {
     'ns_gpu_import': ([_vp, _vp, _vp, _u8, _i64], ct.c_int),
     'ns_gpu_export_spike_bits': ([_vp, _u32], ct.c_int),
     'ns_gpu_info': ([_vp, ct.POINTER(BatchInfo)], ct.c_int),
+    'ns_gpu_set_timing': ([_vp, ct.c_int], ct.c_int),
+    'ns_device_math_digest': ([ct.c_int, ct.c_uint64, ct.c_uint64, ct.c_int64,
+                               ct.POINTER(ct.c_uint64)], ct.c_int),
     'ns_pool_create': ([ct.c_int, ct.c_int, ct.c_int], _vp),
     'ns_pool_destroy': ([_vp], None),
     'ns_pool_import_records': ([_vp, _vp, ct.c_int, _vp, ct.c_int], ct.c_int),
@@ -117,6 +123,15 @@ def _ptr(a, t):
 
 def error():
     return lib().ns_error().decode()
+
+
+def device_math_digest(fn, lo, hi, chunk):
+    n = (hi - lo + chunk - 1) // chunk
+    out = np.zeros(n, np.uint64)
+    rc = lib().ns_device_math_digest(fn, lo, hi, chunk, out.ctypes.data_as(ct.POINTER(ct.c_uint64)))
+    if rc:
+        raise RuntimeError(error())
+    return out
 
 
 class GpuBatch:
@@ -217,6 +232,9 @@ class GpuBatch:
         self._check(self.lib.ns_gpu_autotune(self.ptr, ticks, ct.byref(t), ct.byref(kn),
                                              ct.byref(ks)))
         return t.value, kn.value, ks.value
+
+    def set_timing(self, on=True):
+        self._check(self.lib.ns_gpu_set_timing(self.ptr, int(on)))
 
     def info(self):
         i = BatchInfo()

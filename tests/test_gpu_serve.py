@@ -1,0 +1,160 @@
+"""Serving loop: episodes admitted by free pool capacity run concurrently;
+each episode's result is bit-identical alone, in the mix, and to a CPU
+replay through the oracle (brain) plus the same task arithmetic in float32."""
+import ctypes as ct
+import time
+
+import numpy as np
+import pytest
+
+import ns_amd
+import ns_workloads as W
+from oracle import OracleBrain
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 40
+
+
+def config(**kw):
+    c = ns_amd.ServeConfig(neuron_capacity=kw.get('neurons', 200000),
+                           synapse_capacity=kw.get('synapses', 1000000),
+                           episode_capacity=kw.get('episodes', 4096),
+                           steps_per_episode=STEPS, ticks_per_step=16,
+                           steps_per_launch=kw.get('per_launch', 5), noise_sigma=0.01,
+                           input_gain=230.0)
+    return c
+
+
+def genomes(n, seed=4):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        size = int(np.exp(rng.uniform(np.log(16), np.log(300))))
+        nr, sr, mo = W.episode_genome(rng, size)
+        out.append((nr, sr, mo, 1000 + i, 77 + i))
+    return out
+
+
+def run(eps, cfg, gap=0.0):
+    L = ns_amd.lib()
+    srv = L.ns_serve_create(ct.byref(cfg))
+    assert srv, ns_amd.error()
+    f = ct.POINTER(ct.c_float)
+    ids = {}
+    try:
+        for k, (nr, sr, mo, seed, tseed) in enumerate(eps):
+            i = L.ns_serve_submit(srv, len(nr), len(sr), nr.ctypes.data_as(f), sr.ctypes.data_as(f),
+                                  mo.ctypes.data_as(ct.POINTER(ct.c_int)), seed, tseed)
+            assert i >= 0, ns_amd.error()
+            ids[i] = k
+            if gap:
+                time.sleep(gap)
+        assert L.ns_serve_drain(srv) == 0, ns_amd.error()
+        res = (ns_amd.ServeResult * len(eps))()
+        n = L.ns_serve_poll(srv, res, len(eps))
+        assert n == len(eps)
+        stats = ns_amd.ServeStats()
+        L.ns_serve_stats_get(srv, ct.byref(stats))
+        out = {}
+        for r in res[:n]:
+            out[ids[r.id]] = (np.float32(r.episode_return), np.array(r.final_state[:], np.float32),
+                              tuple(r.motor_spikes))
+        return out, stats
+    finally:
+        L.ns_serve_destroy(srv)
+
+
+def task(seed):
+    """make_task of the serving loop, restated (SplitMix64 -> float32)."""
+    mask = (1 << 64) - 1
+    z = (0x9e3779b97f4a7c15 * (seed + 1)) & mask
+
+    def nxt():
+        nonlocal z
+        z = (z + 0x9e3779b97f4a7c15) & mask
+        r = z
+        r = ((r ^ (r >> 30)) * 0xbf58476d1ce4e5b9) & mask
+        r = ((r ^ (r >> 27)) * 0x94d049bb133111eb) & mask
+        r ^= r >> 31
+        return np.float32((r >> 11) * (1.0 / 9007199254740992.0))
+    f = np.float32
+    x = np.array([f(2) * nxt() - f(1) for _ in range(4)], np.float32)
+    A = np.zeros((4, 4), np.float32)
+    for i in range(4):
+        row = [f(2) * nxt() - f(1) for _ in range(4)]
+        norm = f(0)
+        for v in row:
+            norm = f(norm + abs(v))
+        for j in range(4):
+            A[i, j] = f(f(0.9) * row[j]) / norm
+    B = np.array([nxt() - f(0.5) for _ in range(8)], np.float32).reshape(4, 2)
+    return x, A, B
+
+
+def cpu_episode(nr, sr, mo, seed, tseed):
+    f = np.float32
+    b = OracleBrain(nr, sr, mo, 8)
+    b.reset(seed)
+    x, A, B = task(tseed)
+    s = (tseed * 2654435761 + 12345) & 0xffffffff
+    s = s or 1
+    ret = f(0)
+    m = [f(0), f(0)]
+
+    def uni():
+        nonlocal s
+        s ^= (s << 13) & 0xffffffff
+        s ^= s >> 17
+        s ^= (s << 5) & 0xffffffff
+        return f((s >> 8) * (1.0 / 16777216.0))
+    for _ in range(STEPS):
+        inp = np.zeros(8, np.float32)
+        for d in range(4):
+            inp[2 * d] = min(f(1), max(f(0), x[d]))
+            inp[2 * d + 1] = min(f(1), max(f(0), -x[d]))
+        out, rc = b.step(inp, 16, 230.0)
+        u0, u1 = f(out[0]) / f(16), f(out[1]) / f(16)
+        nx = np.zeros(4, np.float32)
+        for i in range(4):
+            acc = f(0)
+            for j in range(4):
+                acc = f(acc + f(A[i, j] * x[j]))
+            acc = f(acc + f(B[i, 0] * u0))
+            acc = f(acc + f(B[i, 1] * u1))
+            n = f(-6)
+            for _ in range(12):
+                n = f(n + uni())
+            nx[i] = f(acc + f(f(0.01) * n))
+        cost = f(0)
+        for i in range(4):
+            cost = f(cost + f(nx[i] * nx[i]))
+        x = nx
+        ret = f(ret - cost)
+        m = [f(m[0] + out[0]), f(m[1] + out[1])]
+    return ret, x, (int(m[0]), int(m[1]))
+
+
+def test_serving_mix_equals_alone_equals_cpu():
+    eps = genomes(24)
+    print(f'[serve] {len(eps)} episodes x {STEPS} steps', flush=True)
+    mixed, stats = run(eps, config(), gap=0.002)
+    assert stats.finished == len(eps) and stats.launches > 0
+    for k in (0, 5, 17):
+        alone, _ = run([eps[k]], config())
+        assert alone[0][0] == mixed[k][0] and np.array_equal(alone[0][1], mixed[k][1])
+        assert alone[0][2] == mixed[k][2]
+    for k in (0, 9):
+        ret, x, m = cpu_episode(*eps[k])
+        assert ret == mixed[k][0], (ret, mixed[k][0])
+        assert np.array_equal(x, mixed[k][1]) and m == mixed[k][2]
+
+
+def test_admission_waits_for_free_capacity():
+    eps = genomes(10, seed=8)
+    need = max(len(e[0]) for e in eps)
+    cfg = config(neurons=need + 5, per_launch=10)
+    res, stats = run(eps, cfg)
+    assert len(res) == len(eps) and stats.admitted == len(eps) and stats.finished == len(eps)
+    alone, _ = run([eps[3]], config())
+    assert alone[0][0] == res[3][0] and np.array_equal(alone[0][1], res[3][1])

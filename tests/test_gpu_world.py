@@ -1,0 +1,190 @@
+"""World-pool surface (checkpoint records in, 16-tick world steps,
+homeostasis, records out) against the oracle's world rules, bit-exact."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import ns_amd
+import ns_workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NEURON_RECORD = np.dtype({
+    'names': ['slot', 'owner', 'next', 'port', 'gene', 'state', 'pending_exc', 'pending_inh',
+              'mod_event_tick', 'spike_history'],
+    'formats': ['<i4', '<i4', '<i4', '<i4', O.NEURON_GENE, O.NEURON_STATE, ('<f4', 4), '<f4',
+                ('<u8', 6), ('u1', 9)],
+    'offsets': [0, 4, 8, 12, 16, 140, 260, 276, 280, 328], 'itemsize': 344})
+SYNAPSE_RECORD = np.dtype({
+    'names': ['slot', 'owner', 'next', 'gene', 'state'],
+    'formats': ['<i4', '<i4', '<i4', O.SYNAPSE_GENE, O.SYNAPSE_STATE],
+    'offsets': [0, 4, 8, 12, 56], 'itemsize': 136})
+
+
+def build_world(seed=7, animals=10):
+    """Records for `animals` animals with interleaved slots; genes from the
+    small mixed workload, ports 0..18 sensory, 19..23 motors."""
+    lib = O.oracle_lib()
+    w = W.small_mix(seed=seed, brains=animals, types=(0, 1, 4, 5, 6), lo=30, hi=60,
+                    modulator_p=0.15)
+    rng = np.random.default_rng(seed)
+    neurons, synapses = [], []
+    slot_n, slot_s = 0, 0
+    brains = []
+    for a in range(animals):
+        nrow, srow, mot = w.brain(a)
+        n = len(nrow)
+        slots = slot_n + 2 * np.arange(n) + (a % 2)       # interleave two animals
+        slot_n += 2 * n
+        genes = np.zeros(n, O.NEURON_GENE)
+        for i in range(n):
+            lib.ora_fn_profile(int(nrow[i, 0]), genes[i:i + 1].ctypes.data)
+        genes['active'] = nrow[:, 2].astype(np.uint8)
+        genes['inhibitory'] = nrow[:, 3].astype(np.uint8)
+        genes['modulator'] = nrow[:, 4].astype(np.uint8)
+        ch = np.full(n, -1, np.int8)
+        ch[:8] = rng.integers(0, 19, 8)
+        ch[:3] = [0, 2, 7]                                # receptor path below 6, not above
+        genes['sensory_channel'] = ch
+        port = np.full(n, -1, np.int32)
+        port[n - 5:] = 19 + np.arange(5)
+        for i in range(n):
+            st = np.zeros(1, O.NEURON_STATE)
+            lib.ora_fn_initial_neuron(genes[i:i + 1].ctypes.data, int(seed * 1000 + a * 97 + i), st.ctypes.data)
+            r = np.zeros(1, NEURON_RECORD)
+            r['slot'] = slots[i]; r['owner'] = a; r['next'] = -1; r['port'] = port[i]
+            r['gene'] = genes[i]; r['state'] = st[0]
+            neurons.append(r)
+        sg = np.zeros(len(srow), O.SYNAPSE_GENE)
+        sg['pre'] = slots[srow[:, 0].astype(int)]
+        sg['post'] = slots[srow[:, 1].astype(int)]
+        sg['initial_weight_ns'] = srow[:, 2] * 3
+        sg['active'] = srow[:, 3].astype(np.uint8)
+        sg['delay'] = srow[:, 4].astype(np.uint8)
+        sg['branch'] = srow[:, 5].astype(np.uint8)
+        sg['utilization'] = srow[:, 6]
+        sg['recovery_ms'] = srow[:, 7]
+        sg['plasticity'] = srow[:, 9].astype(np.uint8)
+        sg['mod_receptors'] = srow[:, 10].astype(np.uint8)
+        sg['release_sites'] = 14
+        for k in range(len(srow)):
+            st = np.zeros(1, O.SYNAPSE_STATE)
+            lib.ora_fn_initial_synapse(sg[k:k + 1].ctypes.data, int(seed * 7 + a * 131 + k), st.ctypes.data)
+            r = np.zeros(1, SYNAPSE_RECORD)
+            r['slot'] = slot_s + k; r['owner'] = a; r['next'] = -1
+            r['gene'] = sg[k]; r['state'] = st[0]
+            synapses.append(r)
+        slot_s += len(srow)
+        brains.append((genes, sg, slots, ch, port))
+    nr = O.cat_records(neurons, NEURON_RECORD)
+    nr = O.cat_records([nr[np.argsort(nr['slot'])]], NEURON_RECORD)
+    return nr, O.cat_records(synapses, SYNAPSE_RECORD), brains
+
+
+def oracle_world(nr, sr, brains):
+    lib = O.oracle_lib()
+    handles = []
+    for a, (genes, sg, slots, ch, port) in enumerate(brains):
+        mine = nr[nr['owner'] == a]
+        mine = mine[np.argsort(mine['slot'])]
+        local = {int(s): i for i, s in enumerate(mine['slot'])}
+        g = np.ascontiguousarray(mine['gene'])
+        syn = sr[sr['owner'] == a]
+        sgl = np.ascontiguousarray(syn['gene']).copy()
+        sgl['pre'] = [local[int(x)] for x in syn['gene']['pre']]
+        sgl['post'] = [local[int(x)] for x in syn['gene']['post']]
+        chs = mine['gene']['sensory_channel'].astype(np.int32)
+        sensory = np.where((chs >= 0) & (chs < 19), a * 19 + chs, -1).astype(np.int32)
+        receptor = ((mine['gene']['type'] == 5) & (chs >= 0) & (chs < 6)).astype(np.uint8)
+        motors = np.zeros(5, np.int32)
+        for i, p in enumerate(mine['port']):
+            if p >= 19:
+                motors[p - 19] = i
+        h = lib.ora_create_records(len(g), len(sgl), 19 * len(brains), g.ctypes.data, sgl.ctypes.data,
+                                   sensory.ctypes.data_as(O._i32), receptor.ctypes.data_as(O._u8), 5,
+                                   motors.ctypes.data_as(O._i32), 8, 1)
+        ns = np.ascontiguousarray(mine['state'])
+        ss = np.ascontiguousarray(syn['state'])
+        lib.ora_import(h, ns.ctypes.data, ss.ctypes.data, None, 0)
+        handles.append((h, mine['slot'], syn['slot']))
+    return handles
+
+
+def test_world_steps_match_oracle_and_round_trip_records():
+    L = ns_amd.lib()
+    lib = O.oracle_lib()
+    nr, sr, brains = build_world()
+    P = len(brains)
+    print(f'[world] animals={P} neurons={len(nr)} synapses={len(sr)}', flush=True)
+    pool = L.ns_pool_create(P, int(nr['slot'].max()) + 1, int(sr['slot'].max()) + 1)
+    assert pool, ns_amd.error()
+    try:
+        assert L.ns_pool_import_records(pool, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == 0, ns_amd.error()
+        ora = oracle_world(nr, sr, brains)
+        rng = np.random.default_rng(3)
+        motors = np.zeros(P * 5, np.int32)
+        ora_motors = np.zeros(P * 5, np.float32)
+        frozen = np.zeros(P, np.uint8)
+        frozen[3] = 1
+        first = 5000
+        for step in range(12):
+            senses = (rng.random(P * 19) < 0.5).astype(np.float32)
+            tick = first + 16 * step
+            assert L.ns_world_neural_step(pool, tick, senses.ctypes.data_as(O._f),
+                                          motors.ctypes.data_as(O._i32), frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
+            age = (np.arange(P) * 3 + step).astype(np.int32)
+            assert L.ns_world_homeostasis(pool, age.ctypes.data_as(O._i32), frozen.ctypes.data_as(O._u8)) == 0
+            for a, (h, _, _) in enumerate(ora):
+                lib.ora_set_tick(h, tick)
+                lib.ora_set_frozen(h, int(frozen[a]))
+                out = np.zeros(5, np.float32)
+                lib.ora_step(h, senses.ctypes.data_as(O._f), 16, 230.0, out.ctypes.data_as(O._f), None)
+                ora_motors[a * 5:(a + 1) * 5] += out
+                lib.ora_world_homeostasis(h, int(age[a]))
+        assert np.array_equal(motors, ora_motors.astype(np.int32))
+        out_n = np.zeros(len(nr), NEURON_RECORD)
+        out_s = np.zeros(len(sr), SYNAPSE_RECORD)
+        cn, cs = ct.c_int(), ct.c_int()
+        assert L.ns_pool_export_records(pool, out_n.ctypes.data, len(nr), out_s.ctypes.data, len(sr),
+                                        ct.byref(cn), ct.byref(cs)) == 0
+        assert cn.value == len(nr) and cs.value == len(sr)
+        assert np.array_equal(out_n['slot'], nr['slot']) and np.array_equal(out_s['slot'], sr['slot'])
+        for a, (h, nslots, sslots) in enumerate(ora):
+            n = len(nslots)
+            ns = np.zeros(n, O.NEURON_STATE)
+            ss = np.zeros(len(sslots), O.SYNAPSE_STATE)
+            h9 = np.zeros((n, 9), np.uint8)
+            stamps = np.zeros(n * 6, np.uint64)
+            lib.ora_export(h, ns.ctypes.data, ss.ctypes.data, h9.ctypes.data_as(O._u8), None, None,
+                           stamps.ctypes.data_as(O._u64))
+            dn = out_n[np.isin(out_n['slot'], nslots)]
+            ds = out_s[np.isin(out_s['slot'], sslots)]
+            assert np.array_equal(O.state_bytes(np.ascontiguousarray(dn['state']), O.NEURON_STATE_BYTES),
+                                  O.state_bytes(ns, O.NEURON_STATE_BYTES)), f'animal {a} neurons'
+            assert np.array_equal(O.state_bytes(np.ascontiguousarray(ds['state']), O.SYNAPSE_STATE_BYTES),
+                                  O.state_bytes(ss, O.SYNAPSE_STATE_BYTES)), f'animal {a} synapses'
+            assert np.array_equal(dn['spike_history'], h9), f'animal {a} rings'
+            assert np.array_equal(dn['mod_event_tick'].reshape(-1), stamps), f'animal {a} stamps'
+        # re-import the exported records into a fresh pool and continue: same as continuing
+        pool2 = L.ns_pool_create(P, int(nr['slot'].max()) + 1, int(sr['slot'].max()) + 1)
+        assert L.ns_pool_import_records(pool2, out_n.ctypes.data, len(nr), out_s.ctypes.data, len(sr)) == 0
+        senses = (rng.random(P * 19) < 0.5).astype(np.float32)
+        tick = first + 16 * 12
+        m1 = np.zeros(P * 5, np.int32)
+        m2 = np.zeros(P * 5, np.int32)
+        for pl, m in ((pool, m1), (pool2, m2)):
+            assert L.ns_world_neural_step(pl, tick, senses.ctypes.data_as(O._f), m.ctypes.data_as(O._i32),
+                                          frozen.ctypes.data_as(O._u8)) == 0
+        assert np.array_equal(m1, m2)
+        e1 = np.zeros(len(nr), NEURON_RECORD)
+        e2 = np.zeros(len(nr), NEURON_RECORD)
+        f1 = np.zeros(len(sr), SYNAPSE_RECORD)
+        f2 = np.zeros(len(sr), SYNAPSE_RECORD)
+        L.ns_pool_export_records(pool, e1.ctypes.data, len(nr), f1.ctypes.data, len(sr), ct.byref(cn), ct.byref(cs))
+        L.ns_pool_export_records(pool2, e2.ctypes.data, len(nr), f2.ctypes.data, len(sr), ct.byref(cn), ct.byref(cs))
+        assert e1.tobytes() == e2.tobytes() and f1.tobytes() == f2.tobytes()
+        L.ns_pool_destroy(pool2)
+    finally:
+        L.ns_pool_destroy(pool)
