@@ -126,7 +126,19 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-autotune', action='store_true')
     ap.add_argument('--tiles', default='')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only copy MIB MiB device to device (HBM counter calibration)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * 2**20 // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_each_way': n * 4, 'copies': 3}))
+        return
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (asked for {args.gpus}); brains are independent, '
               'so the update shards by brain with no exchange (see DESIGN.md)', file=sys.stderr)
@@ -154,7 +166,9 @@ def main():
     patterns = 8
     inputs = torch.from_numpy(np.stack([w.inputs(c) for c in range(patterns)])).cuda()
     out = torch.zeros((w.count, w.outputs), dtype=torch.float32, device='cuda')
-    stream = torch.cuda.current_stream()
+    # a stream of our own (default priority): torch's default stream is the
+    # null stream, on which events would not bracket the library's launches
+    stream = torch.cuda.Stream()
 
     def step(k):
         batch.step_device(inputs[k % patterns].data_ptr(), w.ticks, w.gain, out.data_ptr(),
@@ -184,7 +198,7 @@ def main():
     value = nu / wall
     res_launches = i1['resident_launches'] - i0['resident_launches']
     res_ms = (i1['resident_ms'] - i0['resident_ms']) / max(res_launches, 1)
-    res_neurons = w.total_neurons - i1['fallback_neurons']
+    res_neurons = w.total_neurons - i1['fallback_neurons']      # packs + big brains
     res_synapses = w.total_synapses - i1['fallback_synapses']
     res_events = (i1['resident_events'] - i0['resident_events']) / max(res_launches, 1)
     syn_bytes = SYNAPSE_UPDATE_BYTES_PLASTIC if plastic else SYNAPSE_UPDATE_BYTES_STATIC
@@ -213,7 +227,8 @@ def main():
         log('timing the CPU oracle on a bounded sample')
         cpu = cpu_baseline(w)
 
-    tile = (i1['threads'], i1['neurons_per_thread'], i1['synapses_per_thread'])
+    tile = (i1['threads'], i1['neurons_per_thread'], i1['synapses_per_thread'], i1['min_waves'],
+            i1['lds_synapses'])
     line = {
         'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
         'value': value,
@@ -236,7 +251,10 @@ def main():
         'plain_per_tick': {'value': plain_value, 'speedup': value / plain_value,
                            'steps': args.plain_steps},
         'tile': {'threads': tile[0], 'neurons_per_thread': tile[1], 'synapses_per_thread': tile[2],
-                 'packs': i1['resident_packs'], 'fallback_brains': i1['fallback_brains'],
+                 'min_waves_per_simd': tile[3], 'lds_synapses': tile[4],
+                 'packs': i1['resident_packs'],
+                 'big_brains': i1['big_brains'], 'big_neurons': i1['big_neurons'],
+                 'fallback_brains': i1['fallback_brains'],
                  'fallback_neurons': i1['fallback_neurons'],
                  'per_tick_ms_per_step': per_tick_share},
         'roofline': {'bound': 'hbm', 'kernel': 'k_resident', 'achieved': achieved,

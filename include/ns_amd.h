@@ -192,13 +192,18 @@ void *ns_gpu_create_bulk(int count, const int *neuron_counts,
 #define NS_FORM_PER_TICK 1
 int ns_gpu_set_form(void *batch, int form);
 
-/* Pack tuning: workgroup size (64..1024, multiple of 64), neurons and
- * synapses per thread.  ns_gpu_autotune times candidate shapes on the
- * current pools (state is saved and restored around the trials). */
+/* Pack tuning: workgroup size, neurons per thread, synapses per thread
+ * (register-slot kernel) or 0 with `lds_synapses` = the LDS kernel's
+ * per-pack synapse capacity (plastic pools), and the launch bound's minimum
+ * waves per SIMD (caps registers; 0 = any compiled variant).  Only compiled
+ * shapes are accepted.  ns_gpu_autotune times every compiled shape on the
+ * current pools (state is saved and restored around the trials) and keeps
+ * the fastest. */
 int ns_gpu_set_tiles(void *batch, int threads, int neurons_per_thread,
-                     int synapses_per_thread);
+                     int synapses_per_thread, int min_waves, int lds_synapses);
 int ns_gpu_autotune(void *batch, int ticks, int *threads,
-                    int *neurons_per_thread, int *synapses_per_thread);
+                    int *neurons_per_thread, int *synapses_per_thread,
+                    int *min_waves, int *lds_synapses);
 
 /* Device-resident step: input/mask/output/status are device pointers and
  * `stream` is a hipStream_t (NULL = default stream).  Asynchronous.  No
@@ -233,6 +238,9 @@ typedef struct ns_batch_info {
     int64_t resident_launches, per_tick_calls;
     int64_t resident_events;    /* arrivals delivered inside resident launches */
     int64_t fallback_neurons, fallback_synapses;   /* on the per-tick path */
+    int64_t big_brains, big_neurons;   /* one workgroup each in the resident launch */
+    int min_waves;
+    int lds_synapses;           /* LDS synapse capacity per pack, 0 = register slots */
 } ns_batch_info;
 int ns_gpu_info(void *batch, ns_batch_info *info);
 

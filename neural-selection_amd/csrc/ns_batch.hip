@@ -149,6 +149,16 @@ struct Bin {
 };
 }  // namespace
 
+const TileShape *find_tile(const Tile &t) {
+    for (int k = 0; k < kNumTiles; ++k) {
+        const TileShape &s = kTiles[k];
+        if (s.threads == t.threads && s.neurons_per_thread == t.kn && s.synapses_per_thread == t.ks &&
+            s.min_waves == t.minw && (t.ks > 0 || s.lds_plastic == t.lds))
+            return &s;
+    }
+    return nullptr;
+}
+
 void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile &t,
                        hipStream_t st) {
     arena.release();
@@ -156,17 +166,35 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     tile = t;
     outputs = outs;
     nb = int(specs.size());
-    const int nmax = t.nmax(), smax = t.smax();
+    const TileShape *shape = find_tile(t);
+    if (!shape) throw std::runtime_error("Tile shape not compiled");
+    plastic = false;
+    for (const auto &spec : specs) {
+        const HostBrain &b = *spec.brain;
+        for (const auto &g : b.edges) {
+            const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
+            if (g.plasticity && g.active && a.active && z.active && !a.inhibitory_transmitter &&
+                excitatory_cell(a.type) && !a.modulator && !z.inhibitory_transmitter &&
+                excitatory_cell(z.type))
+                plastic = true;
+        }
+    }
+    const int nmax = t.nmax();
+    const int smax = t.ks > 0 ? t.threads * t.ks : (plastic ? shape->lds_plastic : shape->lds_static);
+    smax_lds = t.ks > 0 ? 0 : smax;
     const int brains_per_pack = std::min(kPackBrains, std::max(1, kPackMotors / std::max(1, outs)));
 
-    // 1. Which brains stay resident in a workgroup, which take the per-tick path.
-    std::vector<uint8_t> slow(nb, 0), eligible(nb, 0);
+    // 1. Classes: brains with LP/Prinz cells take the per-tick kernels; brains
+    // too large for a pack get a workgroup of their own in the resident
+    // launch; the rest are packed several per workgroup.
+    std::vector<uint8_t> slow(nb, 0), eligible(nb, 0), large(nb, 0);
     for (int i = 0; i < nb; ++i) {
         const HostBrain &b = *specs[i].brain;
         for (const auto &g : b.genes)
             if (g.type == 2 || g.type == 3) slow[i] = 1;
         const int nc = int(b.genes.size()), sc = int(b.edges.size());
         eligible[i] = !slow[i] && nc <= nmax && sc <= smax && outs <= kPackMotors;
+        large[i] = !slow[i] && !eligible[i];
     }
     // 2. First-fit-decreasing packing of resident brains by cost.
     std::vector<int> order;
@@ -198,11 +226,17 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
             if (bins.size() - open_from > window) ++open_from;
         }
     }
-    // 3. Pool order: per-tick brains first, then bins.
-    std::vector<int> pool_order;
+    // 3. Pool order: per-tick brains, then large brains (largest first, so
+    // their long-running workgroups start early), then bins.
+    std::vector<int> pool_order, large_order;
     for (int i = 0; i < nb; ++i)
-        if (!eligible[i]) pool_order.push_back(i);
+        if (slow[i]) pool_order.push_back(i);
+        else if (large[i]) large_order.push_back(i);
     fallback_brains = int(pool_order.size());
+    std::stable_sort(large_order.begin(), large_order.end(),
+                     [&](int a, int b) { return cost(a) > cost(b); });
+    nbig = int(large_order.size());
+    for (int i : large_order) pool_order.push_back(i);
     for (const Bin &bin : bins)
         for (int i : bin.members) pool_order.push_back(i);
 
@@ -364,9 +398,14 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     if (slow_count) up(d_slow, slow_pos);
     view = p;
 
-    // 7. Packs and per-tick ranges.
+    // 7. Large brains, packs (with type-sorted neuron slots) and per-tick ranges.
+    std::vector<int32_t> big_list(std::max(nbig, 1), 0);
+    for (int k = 0; k < nbig; ++k) big_list[k] = fallback_brains + k;
+    d_big = arena.alloc<int32_t>(std::max(nbig, 1));
+    up(d_big, big_list);
+    std::vector<uint16_t> perm(N, 0);
     packs.clear();
-    int pb = fallback_brains;
+    int pb = fallback_brains + nbig;
     for (const Bin &bin : bins) {
         Pack pk{};
         pk.b0 = pb;
@@ -376,9 +415,22 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
         pk.nn = bin.nn;
         pk.ns = bin.ns;
         pk.nm = bin.nb * outs;
+        // slot order: cortical AdEx (type 0), receptor cells (5), other AdEx
+        // (1, 4), LFS (6), then inactive cells
+        std::vector<std::pair<int, int>> key;
+        for (int j = 0; j < pk.nn; ++j) {
+            const uint32_t f = nflags[pk.n0 + j];
+            const uint32_t ty = f & NF_TYPE;
+            const int cls = !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
+            key.push_back({cls, j});
+        }
+        std::stable_sort(key.begin(), key.end());
+        for (int j = 0; j < pk.nn; ++j) perm[pk.n0 + j] = uint16_t(key[j].second);
         packs.push_back(pk);
         pb += bin.nb;
     }
+    d_perm = arena.alloc<uint16_t>(N);
+    up(d_perm, perm);
     d_packs = arena.alloc<Pack>(std::max<size_t>(packs.size(), 1));
     if (!packs.empty()) up(d_packs, packs);
     const int64_t fb_n = fallback_brains ? int64_t(bn0[fallback_brains - 1]) + bnc[fallback_brains - 1] : 0;
@@ -551,11 +603,11 @@ static Tile default_tile(const std::vector<HostBrain> &brains) {
     const int n90 = n[n.size() * 9 / 10], s90 = s[s.size() * 9 / 10];
     for (int k = 0; k < kNumTiles; ++k) {
         const TileShape &t = kTiles[k];
-        if (t.threads * t.neurons_per_thread >= 4 * n90 && t.threads * t.synapses_per_thread >= 4 * s90)
-            return Tile{t.threads, t.neurons_per_thread, t.synapses_per_thread};
+        if (t.synapses_per_thread > 0 && t.threads * t.neurons_per_thread >= 4 * n90 &&
+            t.threads * t.synapses_per_thread >= 4 * s90)
+            return Tile{t.threads, t.neurons_per_thread, t.synapses_per_thread, t.min_waves, 0};
     }
-    const TileShape &t = kTiles[kNumTiles - 2];
-    return Tile{t.threads, t.neurons_per_thread, t.synapses_per_thread};
+    return Tile{1024, 1, 8, 1, 0};
 }
 
 void Batch::create(std::vector<HostBrain> &&owned, int) {
@@ -619,17 +671,17 @@ void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float ga
         if (timer) timer->end(st);
         return;
     }
-    if (!pool.packs.empty()) {
-        const TileShape *shape = nullptr;
-        for (int k = 0; k < kNumTiles; ++k)
-            if (kTiles[k].threads == pool.tile.threads &&
-                kTiles[k].neurons_per_thread == pool.tile.kn &&
-                kTiles[k].synapses_per_thread == pool.tile.ks)
-                shape = &kTiles[k];
+    if (!pool.packs.empty() || pool.nbig) {
+        const TileShape *shape = find_tile(pool.tile);
         if (!shape) throw std::runtime_error("Unknown tile shape");
         if (timer) timer->begin(st, 0, ev);
-        NS_HIP(shape->launch(pool.view, pool.d_packs, int(pool.packs.size()), ticks, d_in, gain,
-                             d_out, homeostasis, st));
+        if (shape->synapses_per_thread > 0)
+            NS_HIP(shape->launch(pool.view, pool.d_packs, int(pool.packs.size()), pool.d_big,
+                                 pool.nbig, pool.d_perm, ticks, d_in, gain, d_out, homeostasis, st));
+        else
+            NS_HIP((pool.plastic ? shape->launch_plastic : shape->launch_static)(
+                pool.view, pool.d_packs, int(pool.packs.size()), pool.d_big, pool.nbig, pool.d_perm,
+                ticks, d_in, gain, d_out, homeostasis, pool.smax_lds, st));
         if (timer) timer->end(st);
     }
     if (pool.fallback_brains) {
@@ -848,15 +900,18 @@ int ns_gpu_set_form(void *p, int form) {
     return 0;
 }
 
-int ns_gpu_set_tiles(void *p, int threads, int kn, int ks) {
+int ns_gpu_set_tiles(void *p, int threads, int kn, int ks, int want_minw, int lds) {
     try {
         auto &b = *static_cast<Batch *>(p);
-        bool known = false;
-        for (int k = 0; k < kNumTiles; ++k)
-            known |= kTiles[k].threads == threads && kTiles[k].neurons_per_thread == kn &&
-                     kTiles[k].synapses_per_thread == ks;
-        if (!known) throw std::runtime_error("Tile shape not compiled");
-        b.relayout(Tile{threads, kn, ks});
+        const TileShape *found = nullptr;
+        for (int k = 0; k < kNumTiles && !found; ++k) {
+            const TileShape &t = kTiles[k];
+            if (t.threads == threads && t.neurons_per_thread == kn && t.synapses_per_thread == ks &&
+                (want_minw == 0 || t.min_waves == want_minw) && (ks > 0 || t.lds_plastic == lds))
+                found = &t;
+        }
+        if (!found) throw std::runtime_error("Tile shape not compiled");
+        b.relayout(Tile{threads, kn, ks, found->min_waves, ks > 0 ? 0 : lds});
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -864,7 +919,7 @@ int ns_gpu_set_tiles(void *p, int threads, int kn, int ks) {
     }
 }
 
-int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks) {
+int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *minw, int *lds) {
     try {
         auto &b = *static_cast<Batch *>(p);
         std::vector<ns_neuron_state> n;
@@ -885,7 +940,8 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks) {
         NS_HIP(hipEventCreate(&e0));
         NS_HIP(hipEventCreate(&e1));
         for (int k = 0; k < kNumTiles; ++k) {
-            Tile t{kTiles[k].threads, kTiles[k].neurons_per_thread, kTiles[k].synapses_per_thread};
+            Tile t{kTiles[k].threads, kTiles[k].neurons_per_thread, kTiles[k].synapses_per_thread,
+                   kTiles[k].min_waves, kTiles[k].synapses_per_thread > 0 ? 0 : kTiles[k].lds_plastic};
             b.pool.build(b.specs, b.outputs, t, b.stream);
             b.pool.upload_state(n, s, h, tk, &stamps);
             b.brun_valid = false;
@@ -909,6 +965,8 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks) {
         if (threads) *threads = best_tile.threads;
         if (kn) *kn = best_tile.kn;
         if (ks) *ks = best_tile.ks;
+        if (minw) *minw = best_tile.minw;
+        if (lds) *lds = best_tile.lds;
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -1019,9 +1077,15 @@ int ns_gpu_info(void *p, ns_batch_info *info) {
         info->resident_events = int64_t(events[0]);
         info->fallback_neurons = b.pool.fallback_range.n_hi - b.pool.fallback_range.n_lo;
         info->fallback_synapses = b.pool.fallback_range.s_hi - b.pool.fallback_range.s_lo;
+        info->big_brains = b.pool.nbig;
+        int64_t big_n = 0;
+        for (int k = 0; k < b.pool.nbig; ++k) big_n += b.pool.bnc[b.pool.fallback_brains + k];
+        info->big_neurons = big_n;
         info->threads = b.pool.tile.threads;
         info->neurons_per_thread = b.pool.tile.kn;
         info->synapses_per_thread = b.pool.tile.ks;
+        info->min_waves = b.pool.tile.minw;
+        info->lds_synapses = b.pool.smax_lds;
         info->form = b.form;
         info->device_bytes = b.pool.arena.bytes + b.io.bytes;
         b.timer.resolve();

@@ -113,12 +113,43 @@ struct NeuronConst {
     float adp_decay;        // expf(-h / afterdepol_tau_ms) when tau > 0
     float rbase, rgain, revent;
     float rdecay;           // expf(-1 / receptor_decay_ms)
+    double rcap, rslope, rtauw;   // correctly rounded 1/c of the three divisors
     uint32_t flags;
 };
 
 __device__ inline float clamp_range(float x, float lo, float hi) {   // physiology.cuh:78-80
     return fminf(hi, fmaxf(lo, x));
 }
+
+// Correctly rounded x / c from rc = RN_double(1/c).  The double product is
+// within 2^-52 (relative) of x/c, while a quotient of two floats that is not
+// itself a float midpoint lies at least 2^-49 (relative) from every float
+// rounding midpoint, so rounding the product to float gives RN(x/c) for all
+// normal, infinite and zero results.  Nonzero results below 2^-126 (where
+// exact subnormal midpoints exist) take the IEEE division.  Checked for
+// every float x by tests/test_division.py.
+__device__ inline float div_const(float x, float c, double rc) {
+    const double q = (double)x * rc;
+    const double a = __builtin_fabs(q);
+    if (a != 0.0 && a < 0x1p-126) return x / c;
+    return (float)q;
+}
+
+// Correctly rounded 5 / d for d >= 1: reciprocal estimate refined by two
+// Newton steps in double (error ~2^-52 from any start within a few float
+// ulps), times 5, rounded once to float; same argument as div_const.
+__device__ inline float div5(float d) {
+    if (!(d >= 1.0f && d < 3.0e38f)) return 5.0f / d;
+    const double dd = d;
+    double r = (double)__builtin_amdgcn_rcpf(d);
+    double e = __builtin_fma(-dd, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-dd, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return (float)(5.0 * r);
+}
+
+constexpr double kRcp04 = 1.0 / double(0.4f);
 
 __device__ inline void load_neuron(const Pool &p, int64_t i, NeuronRegs &s, NeuronConst &g,
                                    const uint64_t *etab) {
@@ -146,6 +177,9 @@ __device__ inline void load_neuron(const Pool &p, int64_t i, NeuronRegs &s, Neur
     g.reset = p.ng[NG_RESET * N + i];
     g.adp_spike = p.ng[NG_ADP_SPIKE * N + i];
     g.leak_slope = g.leak * g.slope;
+    g.rcap = 1.0 / double(g.cap);
+    g.rslope = 1.0 / double(g.slope);
+    g.rtauw = 1.0 / double(g.tauw);
     float tau = p.ng[NG_ADP_TAU * N + i];
     g.adp_decay = tau > 0 ? ns_expf(-0.25f / tau, etab) : 1.0f;
     g.flags = p.nflags[i];
@@ -194,7 +228,7 @@ __device__ inline float receptor_drive(NeuronRegs &s, const NeuronConst &g, floa
 
 // physiology.cuh:226-230
 __device__ inline float branch_io(float c, const uint64_t *etab) {
-    float regen = 5.0f / (1.0f + ns_expf(-(c - 4.0f) / 0.4f, etab));
+    float regen = div5(1.0f + ns_expf(div_const(-(c - 4.0f), 0.4f, kRcp04), etab));
     return fminf(11.0f, c + regen - kBranchOffset);
 }
 
@@ -208,8 +242,8 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
         float v = s.v;
-        s.w += h * (g.a * (v - g.rest) - s.w) / g.tauw;
-        float x = clamp_range((v - g.thr) / g.slope, -20.0f, 10.0f);
+        s.w += div_const(h * (g.a * (v - g.rest) - s.w), g.tauw, g.rtauw);
+        float x = clamp_range(div_const(v - g.thr, g.slope, g.rslope), -20.0f, 10.0f);
         float soma = 0;
         if (cortical) {
             soma += branch_io(s.d0, etab);
@@ -222,7 +256,7 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
         float dv = g.leak * (g.rest - v) + g.leak_slope * ns_expf(x, etab) +
                    soma * (0.0f - v) + s.inh * (-75.0f - v) + current + s.adp - s.w -
                    0.0f * (v - -68.0f);
-        s.v = v + h * dv / g.cap;
+        s.v = v + div_const(h * dv, g.cap, g.rcap);
         if (s.v >= 0.0f) {
             spike = 1;
             ++s.total;

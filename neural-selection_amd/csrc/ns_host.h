@@ -67,11 +67,14 @@ struct BrainSpec {
     bool world;          // src/main.cu neuron_step rules
 };
 
+// Tile of the resident launch: ks > 0 selects the register-slot kernel, ks == 0
+// the LDS kernel with lds synapses per pack.
 struct Tile {
-    int threads = 256, kn = 2, ks = 4;
+    int threads = 256, kn = 2, ks = 4, minw = 1, lds = 0;
     int nmax() const { return threads * kn; }
-    int smax() const { return threads * ks; }
+    int smax() const { return ks > 0 ? threads * ks : lds; }
 };
+const TileShape *find_tile(const Tile &t);
 
 // Structure-of-arrays device pool over a set of brains, laid out in packs.
 class DevicePool {
@@ -102,8 +105,11 @@ public:
     std::vector<Pack> packs;
     Pack *d_packs = nullptr;
     PlainRange fallback_range{}, all_range{};
-    int32_t *d_slow = nullptr;
-    int slow_count = 0, fallback_brains = 0;
+    int32_t *d_slow = nullptr, *d_big = nullptr;
+    uint16_t *d_perm = nullptr;
+    bool plastic = false;            // any valid plastic excitatory pair
+    int smax_lds = 0;                // LDS synapse capacity per pack (LDS tiles)
+    int slow_count = 0, fallback_brains = 0, nbig = 0;
     uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
     hipStream_t stream = nullptr;
 };

@@ -27,11 +27,21 @@ struct PlainRange {
     int slow_count;
 };
 
-using ResidentLauncher = hipError_t (*)(const Pool &, const Pack *, int, int, const float *,
-                                        float, float *, int, hipStream_t);
+using ResidentLauncher = hipError_t (*)(const Pool &, const Pack *, int, const int32_t *, int,
+                                        const uint16_t *, int, const float *, float, float *, int,
+                                        hipStream_t);
+using LdsLauncher = hipError_t (*)(const Pool &, const Pack *, int, const int32_t *, int,
+                                   const uint16_t *, int, const float *, float, float *, int, int,
+                                   hipStream_t);
+// A compiled tile.  Register-slot tiles (synapses_per_thread > 0) keep
+// synapse state in registers; LDS tiles keep it in LDS with room for
+// lds_plastic / lds_static synapses per pack (with / without plasticity
+// traces).
 struct TileShape {
-    int threads, neurons_per_thread, synapses_per_thread;
+    int threads, neurons_per_thread, synapses_per_thread, min_waves;
+    int lds_plastic, lds_static;
     ResidentLauncher launch;
+    LdsLauncher launch_plastic, launch_static;
 };
 extern const TileShape kTiles[];
 extern const int kNumTiles;

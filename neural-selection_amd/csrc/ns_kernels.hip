@@ -114,28 +114,32 @@ __global__ void k_fill_i32(int32_t *x, int64_t n, int32_t v) {
 // ---------------------------------------------------------------------------
 
 // Delivery (src/main.cu:659-708; gym_adapter/core.cuh:37-47): arrival test on
-// the spike shift register, short-term release on arrival.
+// the spike shift register, short-term release on arrival.  Returns arrival.
+__device__ inline unsigned deliver_one(const Pool &p, int64_t q, int32_t b, int t) {
+    uint32_t flags = p.sflags[q];
+    uint32_t route = flags >> SF_ROUTE_SHIFT;
+    float rel = 0;
+    unsigned arr = 0;
+    if (route & R_VALID) {
+        int64_t tick = p.btick[b] + t;
+        int d = flags & SF_DELAY;
+        if (tick >= d) arr = (p.hist[p.spre[q]] >> (d - 1)) & 1u;
+        if (arr) {
+            float w = p.ss[SS_WEIGHT * p.scap + q];
+            rel = (route & R_STP) ? release_at(p, q, flags, w, tick, g_exp_tab) : w;
+        }
+    }
+    p.arrived[q] = uint8_t(arr);
+    p.released[q] = rel;
+    return arr;
+}
+
 __global__ void k_deliver(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t need) {
     int64_t q = s_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned arr = 0;
     if (q < s_hi) {
         int32_t b = p.sbrain[q];
-        if (need_ok(p, b, need)) {
-            uint32_t flags = p.sflags[q];
-            uint32_t route = flags >> SF_ROUTE_SHIFT;
-            float rel = 0;
-            if (route & R_VALID) {
-                int64_t tick = p.btick[b] + t;
-                int d = flags & SF_DELAY;
-                if (tick >= d) arr = (p.hist[p.spre[q]] >> (d - 1)) & 1u;
-                if (arr) {
-                    float w = p.ss[SS_WEIGHT * p.scap + q];
-                    rel = (route & R_STP) ? release_at(p, q, flags, w, tick, g_exp_tab) : w;
-                }
-            }
-            p.arrived[q] = uint8_t(arr);
-            p.released[q] = rel;
-        }
+        if (need_ok(p, b, need)) arr = deliver_one(p, q, b, t);
     }
     uint64_t m = __ballot(arr);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&p.events[1], (unsigned long long)__popcll(m));
@@ -189,16 +193,11 @@ __device__ inline float sensory_current(const Pool &p, int64_t i, NeuronRegs &s,
     return (g.flags & NF_RECEPTOR) ? receptor_drive(s, g, level) : gain * level;
 }
 
-// neuron_step (src/main.cu:710-746) / the neuron loop of core.cuh:49-58.
-__global__ void k_neuron(Pool p, int64_t n_lo, int64_t n_hi, int t, uint8_t need,
-                         const float *input, float gain) {
-    int64_t i = n_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_hi) return;
-    int32_t b = p.nbrain[i];
-    if (!need_ok(p, b, need)) return;
+// neuron_step (src/main.cu:710-746) / the neuron loop of core.cuh:49-58, for
+// the register types (0, 1, 4, 5, 6) and inactive cells of any type.
+__device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, const float *input,
+                                  float gain) {
     uint32_t flags = p.nflags[i];
-    uint32_t type = flags & NF_TYPE;
-    if (type == 2 || type == 3) return;
     if (!(flags & NF_ACTIVE)) {
         uint32_t h = p.hist[i];
         if (flags & NF_WORLD) {
@@ -224,6 +223,17 @@ __global__ void k_neuron(Pool p, int64_t n_lo, int64_t n_hi, int t, uint8_t need
     if (!isfinite(s.v)) p.bstatus[b] = -1;
 }
 
+__global__ void k_neuron(Pool p, int64_t n_lo, int64_t n_hi, int t, uint8_t need,
+                         const float *input, float gain) {
+    int64_t i = n_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_hi) return;
+    int32_t b = p.nbrain[i];
+    if (!need_ok(p, b, need)) return;
+    uint32_t type = p.nflags[i] & NF_TYPE;
+    if ((type == 2 || type == 3) && (p.nflags[i] & NF_ACTIVE)) return;
+    neuron_one(p, i, b, t, input, gain);
+}
+
 // Types 2 (LP, lp_cell.cuh) and 3 (Prinz, prinz_cell.cuh): physiology.cuh:313-347.
 __global__ void k_neuron_slow(Pool p, const int32_t *list, int count, int t, uint8_t need,
                               const float *input, float gain) {
@@ -234,16 +244,7 @@ __global__ void k_neuron_slow(Pool p, const int32_t *list, int count, int t, uin
     if (!need_ok(p, b, need)) return;
     uint32_t flags = p.nflags[i];
     const int64_t N = p.ncap;
-    if (!(flags & NF_ACTIVE)) {
-        uint32_t h = p.hist[i];
-        if (flags & NF_WORLD) {
-            p.hist[i] = (h << 1) | ((h >> (kWorldHistory - 1)) & 1u);
-        } else {
-            p.hist[i] = h << 1;
-            p.spike[i] = 0;
-        }
-        return;
-    }
+    if (!(flags & NF_ACTIVE)) return;   // inactive cells run in k_neuron
     int64_t tick = p.btick[b] + t;
     float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh_in = 0;
     unsigned mod = gather_inputs(p, i, tick, e0, e1, e2, e3, inh_in, g_exp_tab);
@@ -293,20 +294,9 @@ __global__ void k_neuron_slow(Pool p, const int32_t *list, int count, int t, uin
     if (!isfinite(s.v)) p.bstatus[b] = -1;
 }
 
-// Plasticity (src/main.cu:748-782; core.cuh:60-66) plus motor counting
-// (core.cuh:68, main.cu:742-743) for threads below `slots`.
-__global__ void k_plasticity(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t need,
-                             int slots, float *out) {
-    int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid < slots) {
-        int32_t b = int32_t(gid / p.outputs);
-        const int32_t mn = p.motor[gid];
-        if (need_ok(p, b, need) && mn >= 0) out[p.bout0[b] + gid % p.outputs] += float(p.spike[mn]);
-    }
-    int64_t q = s_lo + gid;
-    if (q >= s_hi) return;
-    int32_t b = p.sbrain[q];
-    if (!need_ok(p, b, need) || p.bfrozen[b]) return;
+// Plasticity (src/main.cu:748-782; core.cuh:60-66) of synapse q of brain b.
+__device__ inline void plasticity_one(const Pool &p, int64_t q, int32_t b) {
+    if (p.bfrozen[b]) return;
     uint32_t flags = p.sflags[q];
     uint32_t route = flags >> SF_ROUTE_SHIFT;
     int profile = (flags >> SF_PLAST_SHIFT) & 7;
@@ -318,6 +308,23 @@ __global__ void k_plasticity(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t 
                     p.arrived[q] != 0, p.spike[post] != 0, p.ns[NS_DEPOL * p.ncap + post],
                     p.modbits[post], g_exp_tab);
     store_plast(p, q, profile, r);
+}
+
+// Plasticity plus motor counting (core.cuh:68, main.cu:742-743) for threads
+// below `slots`.
+__global__ void k_plasticity(Pool p, int64_t s_lo, int64_t s_hi, int t, uint8_t need,
+                             int slots, float *out) {
+    int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid < slots) {
+        int32_t b = int32_t(gid / p.outputs);
+        const int32_t mn = p.motor[gid];
+        if (need_ok(p, b, need) && mn >= 0) out[p.bout0[b] + gid % p.outputs] += float(p.spike[mn]);
+    }
+    int64_t q = s_lo + gid;
+    if (q >= s_hi) return;
+    int32_t b = p.sbrain[q];
+    if (!need_ok(p, b, need)) return;
+    plasticity_one(p, q, b);
 }
 
 // Homeostasis at the end of a batched call (core.cuh:70-72): elapsed = ticks,
@@ -353,8 +360,52 @@ __global__ void k_homeostasis_world(Pool p, int64_t n, const int32_t *age) {
 // ---------------------------------------------------------------------------
 // Resident form
 // ---------------------------------------------------------------------------
-template <int WG, int KN, int KS>
-__global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int ticks,
+
+// A brain too large for a pack: one workgroup runs all its ticks, with the
+// per-tick phases separated by workgroup barriers instead of kernel launches
+// and its state kept in global memory (L2-resident for the call).
+template <int WG>
+__device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float *input,
+                                float gain, float *out, int homeostasis) {
+    __shared__ unsigned big_events;
+    if (!(p.brun[b] & 1)) return;
+    const int tid = threadIdx.x;
+    const int64_t n0 = p.bn0[b], n1 = n0 + p.bnc[b];
+    const int64_t s0 = p.bs0[b], s1 = s0 + p.bsc[b];
+    if (tid == 0) big_events = 0;
+    unsigned arrivals = 0;
+    for (int t = 0; t < ticks; ++t) {
+        for (int64_t q = s0 + tid; q < s1; q += WG) arrivals += deliver_one(p, q, b, t);
+        __syncthreads();
+        for (int64_t i = n0 + tid; i < n1; i += WG) neuron_one(p, i, b, t, input, gain);
+        __syncthreads();
+        for (int64_t q = s0 + tid; q < s1; q += WG) plasticity_one(p, q, b);
+        for (int m = tid; m < p.outputs; m += WG) {
+            const int32_t mn = p.motor[int64_t(b) * p.outputs + m];
+            if (mn >= 0) out[p.bout0[b] + m] += float(p.spike[mn]);
+        }
+        __syncthreads();
+    }
+    if (homeostasis) {
+        const int64_t tick = p.btick[b] + ticks;
+        for (int64_t i = n0 + tid; i < n1; i += WG) {
+            const uint32_t flags = p.nflags[i];
+            if (!(flags & NF_ACTIVE) || (flags & NF_TYPE) != 0) continue;
+            const float hz = p.total[i] * 1000.0f / float(ticks > tick ? int64_t(ticks) : tick);
+            p.ns[NS_LAS * p.ncap + i] = homeostasis_las(p.ns[NS_LAS * p.ncap + i], float(ticks), hz);
+        }
+    }
+    if (arrivals) atomicAdd(&big_events, arrivals);
+    __syncthreads();
+    if (tid == 0) {
+        p.btick[b] += ticks;
+        if (big_events) atomicAdd(&p.events[0], (unsigned long long)big_events);
+    }
+}
+
+template <int WG, int KN, int KS, int MINW>
+__global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs, const int32_t *big,
+                                                 int nbig, const uint16_t *perm, int ticks,
                                                  const float *input, float gain, float *out,
                                                  int homeostasis) {
     constexpr int NMAX = WG * KN, SMAX = WG * KS;
@@ -374,7 +425,11 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
     __shared__ int qlen;
     __shared__ unsigned events_l;
 
-    const Pack pk = packs[blockIdx.x];
+    if (int(blockIdx.x) < nbig) {
+        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis);
+        return;
+    }
+    const Pack pk = packs[blockIdx.x - nbig];
     const int tid = threadIdx.x;
     const int64_t N = p.ncap, S = p.scap;
     const int64_t n0 = pk.n0, s0 = pk.s0;
@@ -407,20 +462,24 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
     }
     __syncthreads();
 
-    // Neuron registers.
+    // Neuron registers.  Slot (k, tid) holds the pack's neuron perm[k*WG+tid]:
+    // the pack's neurons sorted by cell type, so wavefronts run one membrane
+    // model at a time.
     NeuronRegs nr[KN];
     NeuronConst nc[KN];
     float nscale[KN], ncur[KN];
-    int nbr[KN];
+    int nbr[KN], nid[KN];
     bool nlive[KN];
 #pragma unroll
     for (int k = 0; k < KN; ++k) {
-        const int n = k * WG + tid;
+        const int slot = k * WG + tid;
         nlive[k] = false;
         nbr[k] = 0;
         nscale[k] = 0;
         ncur[k] = 0;
-        if (n < pk.nn) {
+        nid[k] = slot < pk.nn ? int(perm[n0 + slot]) : pk.nn;
+        const int n = nid[k];
+        if (slot < pk.nn) {
             const int64_t i = n0 + n;
             nbr[k] = p.nbrain[i] - pk.b0;
             load_neuron(p, i, nr[k], nc[k], etab);
@@ -509,7 +568,7 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
         if (tid == 0) qlen = 0;
 #pragma unroll
         for (int k = 0; k < KN; ++k) {
-            const int n = k * WG + tid;
+            const int n = nid[k];
             if (!nlive[k]) continue;
             const uint32_t flags = nc[k].flags;
             if (!(flags & NF_ACTIVE)) {
@@ -579,7 +638,7 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
     // Write back.
 #pragma unroll
     for (int k = 0; k < KN; ++k) {
-        const int n = k * WG + tid;
+        const int n = nid[k];
         if (n >= pk.nn || !brun_l[nbr[k]]) continue;
         const int64_t i = n0 + n;
         if (homeostasis && (nc[k].flags & NF_ACTIVE) && (nc[k].flags & NF_TYPE) == 0) {
@@ -611,6 +670,301 @@ __global__ __launch_bounds__(WG) void k_resident(Pool p, const Pack *packs, int 
     }
     for (int b = tid; b < pk.nb; b += WG)
         if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
+    __syncthreads();
+    if (tid == 0 && events_l) atomicAdd(&p.events[0], (unsigned long long)events_l);
+}
+
+// ---------------------------------------------------------------------------
+// Resident form, synapse state in LDS.
+//
+// Same tick structure as k_resident, but every per-synapse value (weight and
+// plasticity traces, genes, flags, endpoints, this tick's release) lives in
+// dynamically sized LDS and threads stride over the pack's synapses.  Thread
+// registers then hold only neuron state, which keeps the kernel under 128
+// VGPRs without spills.  PLASTIC = false drops the trace arrays for pools
+// without plastic synapses.
+// ---------------------------------------------------------------------------
+template <bool PLASTIC>
+__host__ __device__ constexpr int lds_synapse_bytes() {
+    return PLASTIC ? 4 * 9 + 1 + 1 + 2 : 4 * 5 + 1 + 1 + 2;
+}
+
+template <int WG, int KN, int MINW, bool PLASTIC>
+__global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *packs,
+                                                         const int32_t *big, int nbig,
+                                                         const uint16_t *perm, int ticks,
+                                                         const float *input, float gain,
+                                                         float *out, int homeostasis, int smax) {
+    constexpr int NMAX = WG * KN;
+    extern __shared__ __align__(16) unsigned char lds_syn[];
+    __shared__ uint64_t etab[32];
+    __shared__ uint32_t hist_l[NMAX];
+    __shared__ float depol_l[NMAX];
+    __shared__ uint8_t mod_l[NMAX];
+    __shared__ uint16_t in_l[NMAX + 1];
+    __shared__ int64_t btick_l[kPackBrains];
+    __shared__ uint8_t brun_l[kPackBrains], bfrozen_l[kPackBrains];
+    __shared__ uint16_t mot_l[kPackMotors];
+    __shared__ uint16_t cnt_l[kPackMotors];
+    __shared__ int qlen;
+    __shared__ unsigned events_l;
+
+    if (int(blockIdx.x) < nbig) {
+        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis);
+        return;
+    }
+    const Pack pk = packs[blockIdx.x - nbig];
+    const int tid = threadIdx.x;
+    const int64_t N = p.ncap, S = p.scap;
+    const int64_t n0 = pk.n0, s0 = pk.s0;
+    const int ns = pk.ns;
+
+    // synapse arrays (structure of arrays, smax entries each)
+    float *rel_l = reinterpret_cast<float *>(lds_syn);
+    float *w_l = rel_l + smax;
+    float *w0_l = w_l + smax;
+    uint32_t *fl_l = reinterpret_cast<uint32_t *>(w0_l + smax);
+    uint32_t *pp_l = fl_l + smax;
+    float *x_l = reinterpret_cast<float *>(pp_l + smax);
+    float *y_l = PLASTIC ? x_l + smax : x_l;
+    float *pa_l = PLASTIC ? y_l + smax : x_l;
+    float *qa_l = PLASTIC ? pa_l + smax : x_l;
+    uint16_t *queue_l = reinterpret_cast<uint16_t *>(PLASTIC ? qa_l + smax : x_l);
+    uint8_t *sbr_l = reinterpret_cast<uint8_t *>(queue_l + smax);
+    uint8_t *arr_l = sbr_l + smax;
+
+    if (tid < 32) etab[tid] = g_exp_tab[tid];
+    for (int b = tid; b < pk.nb; b += WG) {
+        btick_l[b] = p.btick[pk.b0 + b];
+        brun_l[b] = p.brun[pk.b0 + b] & 1;
+        bfrozen_l[b] = p.bfrozen[pk.b0 + b];
+    }
+    for (int n = tid; n < pk.nn; n += WG) {
+        hist_l[n] = p.hist[n0 + n];
+        in_l[n] = uint16_t(p.nin[n0 + n] - s0);
+        mod_l[n] = 0;
+        depol_l[n] = 0;
+    }
+    if (tid == 0) {
+        in_l[pk.nn] = uint16_t(ns);
+        qlen = 0;
+        events_l = 0;
+    }
+    for (int m = tid; m < pk.nm; m += WG) {
+        const int32_t mn = p.motor[int64_t(pk.b0) * p.outputs + m];
+        mot_l[m] = mn >= 0 ? uint16_t(mn - n0) : kNoMotor;
+        cnt_l[m] = 0;
+    }
+    __syncthreads();
+    for (int q = tid; q < ns; q += WG) {
+        const int64_t gq = s0 + q;
+        const uint8_t b = uint8_t(p.sbrain[gq] - pk.b0);
+        uint32_t f = p.sflags[gq];
+        if (!brun_l[b]) f &= ~(R_VALID << SF_ROUTE_SHIFT);
+        fl_l[q] = f;
+        sbr_l[q] = b;
+        pp_l[q] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << 16);
+        w0_l[q] = p.sg[SG_W0 * S + gq];
+        rel_l[q] = 0.0f;
+        arr_l[q] = 0;
+        if (PLASTIC) {
+            PlastRegs r;
+            load_plast(p, gq, (f >> SF_PLAST_SHIFT) & 7, r);
+            w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
+        } else {
+            w_l[q] = p.ss[SS_WEIGHT * S + gq];
+        }
+    }
+
+    // Neuron registers, slots in type-sorted order (see k_resident).
+    NeuronRegs nr[KN];
+    NeuronConst nc[KN];
+    float nscale[KN], ncur[KN];
+    int nbr[KN], nid[KN];
+    bool nlive[KN];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+        const int slot = k * WG + tid;
+        nlive[k] = false;
+        nbr[k] = 0;
+        nscale[k] = 0;
+        ncur[k] = 0;
+        nid[k] = slot < pk.nn ? int(perm[n0 + slot]) : pk.nn;
+        const int n = nid[k];
+        if (slot < pk.nn) {
+            const int64_t i = n0 + n;
+            nbr[k] = p.nbrain[i] - pk.b0;
+            load_neuron(p, i, nr[k], nc[k], g_exp_tab);
+            nlive[k] = brun_l[nbr[k]] != 0;
+            nscale[k] = ns_expf(p.ns[NS_LAS * N + i], g_exp_tab);
+            int32_t idx = p.ninput[i];
+            ncur[k] = idx >= 0 ? input[idx] : -1.0f;
+            if (idx < 0) nc[k].flags |= 1u << 30;
+        }
+    }
+    __syncthreads();
+
+    unsigned my_events = 0;
+    for (int t = 0; t < ticks; ++t) {
+        // Phase A: plasticity of tick t-1, arrivals of tick t (uniform trip
+        // count so every lane reaches the ballots).
+        for (int base = 0; base < ns; base += WG) {
+            const int q = base + tid;
+            bool arrive_stp = false;
+            if (q < ns) {
+                const uint32_t f = fl_l[q];
+                const uint32_t route = f >> SF_ROUTE_SHIFT;
+                if (route & R_VALID) {
+                    const uint32_t pp = pp_l[q];
+                    const int pre = pp & 0xffff, post = pp >> 16;
+                    const int b = sbr_l[q];
+                    if (PLASTIC) {
+                        const int profile = (f >> SF_PLAST_SHIFT) & 7;
+                        if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
+                            PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
+                            plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q],
+                                            arr_l[q] != 0, hist_l[post] & 1, depol_l[post],
+                                            mod_l[post], etab);
+                            w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
+                            pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
+                        }
+                    }
+                    const int64_t tick = btick_l[b] + t;
+                    const int d = f & SF_DELAY;
+                    const bool arr = tick >= d && ((hist_l[pre] >> (d - 1)) & 1u);
+                    arr_l[q] = arr;
+                    rel_l[q] = arr ? w_l[q] : 0.0f;
+                    my_events += arr;
+                    arrive_stp = arr && (route & R_STP);
+                }
+            }
+            const uint64_t m = __ballot(arrive_stp);
+            if (m) {
+                const int lane = tid & 63;
+                int qb = 0;
+                if (lane == 0) qb = atomicAdd(&qlen, __popcll(m));
+                qb = __shfl(qb, 0);
+                if (arrive_stp) queue_l[qb + __popcll(m & ((uint64_t(1) << lane) - 1))] = uint16_t(q);
+            }
+        }
+        __syncthreads();
+        // Phase A2: short-term release over the compacted arrivals.
+        {
+            const int n_q = qlen;
+            for (int j = tid; j < n_q; j += WG) {
+                const int q = queue_l[j];
+                const int64_t gq = s0 + q;
+                const int64_t tick = btick_l[sbr_l[q]] + t;
+                rel_l[q] = release_at(p, gq, fl_l[q], rel_l[q], tick, etab);
+            }
+        }
+        __syncthreads();
+        // Phase B: neurons.
+        if (tid == 0) qlen = 0;
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+            const int n = nid[k];
+            if (!nlive[k]) continue;
+            const uint32_t flags = nc[k].flags;
+            if (!(flags & NF_ACTIVE)) {
+                const uint32_t h = hist_l[n];
+                hist_l[n] = (flags & NF_WORLD) ? (h << 1) | ((h >> (kWorldHistory - 1)) & 1u) : h << 1;
+                continue;
+            }
+            const int64_t tick = btick_l[nbr[k]] + t;
+            float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
+            unsigned mod = 0;
+            const int q1 = in_l[n + 1];
+            for (int q = in_l[n]; q < q1; ++q) {
+                const float r = rel_l[q];
+                if (r == 0.0f) continue;
+                const uint32_t route = fl_l[q] >> SF_ROUTE_SHIFT;
+                if (route & R_MOD) {
+                    if (r > 0) {
+                        const unsigned mm = (route >> R_MOD_SHIFT) & 3u;
+                        mod |= 1u << (mm - 1);
+                        uint64_t &st = p.stamps[((mm - 1) * 2 + (tick & 1)) * N + n0 + n];
+                        if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
+                    }
+                } else if (route & R_INH) {
+                    inh += r;
+                } else {
+                    const float v = r * nscale[k];
+                    switch (route & R_BRANCH) {
+                    case 0: e0 += v; break;
+                    case 1: e1 += v; break;
+                    case 2: e2 += v; break;
+                    default: e3 += v; break;
+                    }
+                }
+            }
+            float current = 0.0f;
+            if (!(flags & (1u << 30)))
+                current = (flags & NF_RECEPTOR) ? receptor_drive(nr[k], nc[k], ncur[k])
+                                                : gain * ncur[k];
+            const int spike = neuron_tick(nr[k], nc[k], current, e0, e1, e2, e3, inh, p.cell, N,
+                                          n0 + n, etab);
+            hist_l[n] = (hist_l[n] << 1) | uint32_t(spike);
+            depol_l[n] = nr[k].depol;
+            mod_l[n] = uint8_t(mod);
+            if (!isfinite(nr[k].v)) p.bstatus[pk.b0 + nbr[k]] = -1;
+        }
+        __syncthreads();
+        for (int m = tid; m < pk.nm; m += WG)
+            if (brun_l[m / p.outputs] && mot_l[m] != kNoMotor) cnt_l[m] += hist_l[mot_l[m]] & 1u;
+    }
+    // Plasticity of the last tick.
+    if (PLASTIC && ticks > 0) {
+        for (int q = tid; q < ns; q += WG) {
+            const uint32_t f = fl_l[q];
+            const uint32_t route = f >> SF_ROUTE_SHIFT;
+            const int profile = (f >> SF_PLAST_SHIFT) & 7;
+            if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[sbr_l[q]]) {
+                const int post = pp_l[q] >> 16;
+                PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
+                plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], arr_l[q] != 0,
+                                hist_l[post] & 1, depol_l[post], mod_l[post], etab);
+                w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
+            }
+        }
+    }
+    // Write back.
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+        const int n = nid[k];
+        if (n >= pk.nn || !brun_l[nbr[k]]) continue;
+        const int64_t i = n0 + n;
+        if (homeostasis && (nc[k].flags & NF_ACTIVE) && (nc[k].flags & NF_TYPE) == 0) {
+            const int64_t tick = btick_l[nbr[k]] + ticks;
+            const float hz = nr[k].total * 1000.0f / float(ticks > tick ? int64_t(ticks) : tick);
+            p.ns[NS_LAS * N + i] = homeostasis_las(p.ns[NS_LAS * N + i], float(ticks), hz);
+        }
+        if (nc[k].flags & NF_ACTIVE) {
+            store_neuron(p, i, nr[k]);
+            p.spike[i] = uint8_t(hist_l[n] & 1u);
+            p.modbits[i] = mod_l[n];
+        } else if (!(nc[k].flags & NF_WORLD)) {
+            p.spike[i] = 0;
+        }
+        p.hist[i] = hist_l[n];
+    }
+    for (int q = tid; q < ns; q += WG) {
+        if (!brun_l[sbr_l[q]]) continue;
+        const int64_t gq = s0 + q;
+        if (PLASTIC) {
+            PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
+            store_plast(p, gq, (fl_l[q] >> SF_PLAST_SHIFT) & 7, r);
+        }
+        p.arrived[gq] = arr_l[q];
+    }
+    for (int m = tid; m < pk.nm; m += WG) {
+        const int b = m / p.outputs;
+        out[p.bout0[pk.b0 + b] + m % p.outputs] = brun_l[b] ? float(cnt_l[m]) : 0.0f;
+    }
+    for (int b = tid; b < pk.nb; b += WG)
+        if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
+    for (int off = 32; off > 0; off >>= 1) my_events += __shfl_xor(my_events, off);
+    if ((tid & 63) == 0 && my_events) atomicAdd(&events_l, my_events);
     __syncthreads();
     if (tid == 0 && events_l) atomicAdd(&p.events[0], (unsigned long long)events_l);
 }
@@ -679,37 +1033,93 @@ void launch_world_homeostasis(const Pool &p, int64_t nn, const int32_t *age, hip
     if (nn) hipLaunchKernelGGL(k_homeostasis_world, dim3(blocks_for(nn, 256)), dim3(256), 0, st, p, nn, age);
 }
 
-template <int WG, int KN, int KS>
-static hipError_t launch_resident_t(const Pool &p, const Pack *packs, int npacks, int ticks,
+template <int WG, int KN, int KS, int MINW>
+static hipError_t launch_resident_t(const Pool &p, const Pack *packs, int npacks,
+                                    const int32_t *big, int nbig, const uint16_t *perm, int ticks,
                                     const float *input, float gain, float *out, int homeostasis,
                                     hipStream_t st) {
-    hipLaunchKernelGGL((k_resident<WG, KN, KS>), dim3(npacks), dim3(WG), 0, st, p, packs, ticks,
-                       input, gain, out, homeostasis);
+    hipLaunchKernelGGL((k_resident<WG, KN, KS, MINW>), dim3(nbig + npacks), dim3(WG), 0, st, p, packs,
+                       big, nbig, perm, ticks, input, gain, out, homeostasis);
+    return hipGetLastError();
+}
+
+template <int WG, int KN, int MINW, bool PLASTIC>
+static hipError_t launch_lds_t(const Pool &p, const Pack *packs, int npacks, const int32_t *big,
+                               int nbig, const uint16_t *perm, int ticks, const float *input,
+                               float gain, float *out, int homeostasis, int smax, hipStream_t st) {
+    const size_t bytes = size_t(smax) * lds_synapse_bytes<PLASTIC>() + 16;
+    static size_t allowed = 0;   // dynamic LDS limit raised so far for this instance
+    if (bytes > allowed) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&k_resident_lds<WG, KN, MINW, PLASTIC>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+        if (e != hipSuccess) return e;
+        allowed = bytes;
+    }
+    hipLaunchKernelGGL((k_resident_lds<WG, KN, MINW, PLASTIC>), dim3(nbig + npacks), dim3(WG), bytes,
+                       st, p, packs, big, nbig, perm, ticks, input, gain, out, homeostasis, smax);
     return hipGetLastError();
 }
 
 // Explicit instantiations so the device pass emits every tile's kernel.
-#define NS_INSTANTIATE(WG, KN, KS)                                                             \
-    template __global__ void k_resident<WG, KN, KS>(Pool, const Pack *, int, const float *, float, \
-                                                    float *, int);
-NS_INSTANTIATE(256, 1, 4)
-NS_INSTANTIATE(256, 2, 4)
-NS_INSTANTIATE(256, 2, 8)
-NS_INSTANTIATE(512, 1, 4)
-NS_INSTANTIATE(512, 2, 4)
-NS_INSTANTIATE(512, 1, 8)
-NS_INSTANTIATE(1024, 1, 4)
-NS_INSTANTIATE(1024, 1, 8)
-NS_INSTANTIATE(128, 2, 8)
+// MINW is the launch bound's minimum waves per SIMD (it caps VGPRs).
+#define NS_INSTANTIATE(WG, KN, KS, MINW)                                                       \
+    template __global__ void k_resident<WG, KN, KS, MINW>(Pool, const Pack *, const int32_t *, int, \
+                                                          const uint16_t *, int, const float *,  \
+                                                          float, float *, int);
+#define NS_INSTANTIATE_LDS(WG, KN, MINW)                                                        \
+    template __global__ void k_resident_lds<WG, KN, MINW, true>(                                 \
+        Pool, const Pack *, const int32_t *, int, const uint16_t *, int, const float *, float,  \
+        float *, int, int);                                                                     \
+    template __global__ void k_resident_lds<WG, KN, MINW, false>(                                \
+        Pool, const Pack *, const int32_t *, int, const uint16_t *, int, const float *, float,  \
+        float *, int, int);
+NS_INSTANTIATE(256, 1, 4, 1)
+NS_INSTANTIATE(256, 1, 4, 2)
+NS_INSTANTIATE(256, 1, 4, 3)
+NS_INSTANTIATE(256, 2, 4, 1)
+NS_INSTANTIATE(256, 2, 8, 1)
+NS_INSTANTIATE(512, 1, 4, 1)
+NS_INSTANTIATE(512, 1, 4, 2)
+NS_INSTANTIATE(512, 2, 4, 1)
+NS_INSTANTIATE(512, 1, 8, 1)
+NS_INSTANTIATE(1024, 1, 4, 1)
+NS_INSTANTIATE(1024, 1, 8, 1)
+NS_INSTANTIATE(128, 2, 8, 1)
+NS_INSTANTIATE(128, 1, 4, 4)
+NS_INSTANTIATE_LDS(1024, 1, 1)
+NS_INSTANTIATE_LDS(512, 1, 1)
+NS_INSTANTIATE_LDS(512, 2, 1)
+NS_INSTANTIATE_LDS(256, 1, 1)
+NS_INSTANTIATE_LDS(256, 2, 1)
 
 // Host-only table: clang would otherwise promote this const array of host
 // function pointers into device constant memory.
 #if !defined(__HIP_DEVICE_COMPILE__)
-#define NS_TILE(WG, KN, KS) {WG, KN, KS, &launch_resident_t<WG, KN, KS>}
+#define NS_TILE(WG, KN, KS, MINW) {WG, KN, KS, MINW, 0, 0, &launch_resident_t<WG, KN, KS, MINW>, nullptr, nullptr}
+#define NS_TILE_LDS(WG, KN, MINW, SP, SS)                                                      \
+    {WG, KN, 0, MINW, SP, SS, nullptr, &launch_lds_t<WG, KN, MINW, true>,                      \
+     &launch_lds_t<WG, KN, MINW, false>}
 const TileShape kTiles[] = {
-    NS_TILE(256, 1, 4), NS_TILE(256, 2, 4), NS_TILE(256, 2, 8),
-    NS_TILE(512, 1, 4), NS_TILE(512, 2, 4), NS_TILE(512, 1, 8),
-    NS_TILE(1024, 1, 4), NS_TILE(1024, 1, 8), NS_TILE(128, 2, 8),
+    NS_TILE(256, 1, 4, 1),
+    NS_TILE(256, 1, 4, 2),
+    NS_TILE(256, 1, 4, 3),
+    NS_TILE(256, 2, 4, 1),
+    NS_TILE(256, 2, 8, 1),
+    NS_TILE(512, 1, 4, 1),
+    NS_TILE(512, 1, 4, 2),
+    NS_TILE(512, 2, 4, 1),
+    NS_TILE(512, 1, 8, 1),
+    NS_TILE(1024, 1, 4, 1),
+    NS_TILE(1024, 1, 8, 1),
+    NS_TILE(128, 2, 8, 1),
+    NS_TILE(128, 1, 4, 4),
+    NS_TILE_LDS(1024, 1, 1, 3584, 5632),
+    NS_TILE_LDS(1024, 1, 1, 2048, 3072),
+    NS_TILE_LDS(512, 1, 1, 1792, 2944),
+    NS_TILE_LDS(512, 2, 1, 3584, 5632),
+    NS_TILE_LDS(256, 1, 1, 896, 1408),
+    NS_TILE_LDS(256, 2, 1, 1792, 2944),
 };
 const int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
 #endif

@@ -35,7 +35,9 @@ class BatchInfo(ct.Structure):
                 ('device_bytes', ct.c_int64), ('resident_ms', ct.c_double),
                 ('per_tick_ms', ct.c_double), ('resident_launches', ct.c_int64),
                 ('per_tick_calls', ct.c_int64), ('resident_events', ct.c_int64),
-                ('fallback_neurons', ct.c_int64), ('fallback_synapses', ct.c_int64)]
+                ('fallback_neurons', ct.c_int64), ('fallback_synapses', ct.c_int64),
+                ('big_brains', ct.c_int64), ('big_neurons', ct.c_int64),
+                ('min_waves', ct.c_int), ('lds_synapses', ct.c_int)]
 
 
 class ServeConfig(ct.Structure):
@@ -72,8 +74,8 @@ SIGNATURES = {
     'ns_gpu_step': ([_vp, _f, _u8, ct.c_int, ct.c_float, _f, _i32], ct.c_int),
     'ns_gpu_step_device': ([_vp, _vp, _vp, ct.c_int, ct.c_float, _vp, _vp, _vp], ct.c_int),
     'ns_gpu_set_form': ([_vp, ct.c_int], ct.c_int),
-    'ns_gpu_set_tiles': ([_vp, ct.c_int, ct.c_int, ct.c_int], ct.c_int),
-    'ns_gpu_autotune': ([_vp, ct.c_int, _i32, _i32, _i32], ct.c_int),
+    'ns_gpu_set_tiles': ([_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int], ct.c_int),
+    'ns_gpu_autotune': ([_vp, ct.c_int, _i32, _i32, _i32, _i32, _i32], ct.c_int),
     'ns_gpu_export': ([_vp, _vp, _vp, _u8, _i64], ct.c_int),
     'ns_gpu_import': ([_vp, _vp, _vp, _u8, _i64], ct.c_int),
     'ns_gpu_export_spike_bits': ([_vp, _u32], ct.c_int),
@@ -224,14 +226,14 @@ class GpuBatch:
     def set_form(self, form):
         self._check(self.lib.ns_gpu_set_form(self.ptr, form))
 
-    def set_tiles(self, threads, kn, ks):
-        self._check(self.lib.ns_gpu_set_tiles(self.ptr, threads, kn, ks))
+    def set_tiles(self, threads, kn, ks, min_waves=0, lds_synapses=0):
+        self._check(self.lib.ns_gpu_set_tiles(self.ptr, threads, kn, ks, min_waves, lds_synapses))
 
     def autotune(self, ticks=16):
-        t, kn, ks = ct.c_int(), ct.c_int(), ct.c_int()
+        t, kn, ks, mw, ld = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
         self._check(self.lib.ns_gpu_autotune(self.ptr, ticks, ct.byref(t), ct.byref(kn),
-                                             ct.byref(ks)))
-        return t.value, kn.value, ks.value
+                                             ct.byref(ks), ct.byref(mw), ct.byref(ld)))
+        return t.value, kn.value, ks.value, mw.value, ld.value
 
     def set_timing(self, on=True):
         self._check(self.lib.ns_gpu_set_timing(self.ptr, int(on)))

@@ -52,7 +52,26 @@ def test_masks_and_resets_match():
                  mask_fn=lambda c: np.array([1, c % 2, 1, c % 3 == 0, 0, 1], np.uint8))
 
 
-@pytest.mark.parametrize('tiles', [(256, 1, 4), (512, 2, 4), (1024, 1, 8), (128, 2, 8)])
+@pytest.mark.parametrize('tiles', [(256, 1, 4), (512, 2, 4), (1024, 1, 8), (128, 2, 8),
+                                   (256, 1, 4, 3), (1024, 1, 0, 1, 3584), (256, 1, 0, 1, 896),
+                                   (512, 2, 0, 1, 3584)])
 def test_tile_shapes_agree(tiles):
     w = W.small_mix(seed=31, brains=10, calls=12, types=(0, 1, 4, 5, 6), lo=20, hi=120)
     _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=tiles)
+
+
+def test_big_brains_bit_exact():
+    """Brains larger than a pack (here > 128 neurons or > 512 synapses) run in
+    a workgroup of their own inside the resident launch."""
+    w = W.small_mix(seed=33, brains=14, calls=20, types=(0, 1, 4, 5, 6), lo=16, hi=400,
+                    fanout=3.0)
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=20, check_every=5, tiles=(128, 1, 4, 4))
+    assert info['big_brains'] > 0 and info['resident_packs'] > 0
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=20, check_every=5, tiles=(256, 1, 0, 1, 896))
+    assert info['big_brains'] > 0 and info['resident_packs'] > 0 and info['lds_synapses'] > 0
+
+
+def test_plasticity_free_pools_use_the_static_lds_kernel():
+    w = W.small_mix(seed=35, brains=10, calls=10, types=(0, 1, 4, 5), plasticity=False)
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=10, check_every=5, tiles=(256, 1, 0, 1, 896))
+    assert info['lds_synapses'] == 1408
