@@ -233,6 +233,10 @@ __device__ inline float branch_io(float c, const uint64_t *etab) {
 }
 
 // physiology.cuh:290-374, AdEx branch (types 0, 1, 4, 5).  Returns the spike.
+// Two exact shortcuts: branch_io(+0) is exactly +0 (the regenerative term at
+// zero conductance equals the subtracted offset), so zero dendrites add +0 to
+// a non-negative sum and are skipped; and with adapt_ns == 0, w == +0 and a
+// finite v - rest the adaptation update adds a signed zero to +0, an identity.
 __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float current,
                                 const uint64_t *etab) {
     const float h = 0.25f;
@@ -242,14 +246,15 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
         float v = s.v;
-        s.w += div_const(h * (g.a * (v - g.rest) - s.w), g.tauw, g.rtauw);
+        if (!(g.a == 0.0f && __float_as_uint(s.w) == 0u && __builtin_isfinite(v - g.rest)))
+            s.w += div_const(h * (g.a * (v - g.rest) - s.w), g.tauw, g.rtauw);
         float x = clamp_range(div_const(v - g.thr, g.slope, g.rslope), -20.0f, 10.0f);
         float soma = 0;
         if (cortical) {
-            soma += branch_io(s.d0, etab);
-            soma += branch_io(s.d1, etab);
-            soma += branch_io(s.d2, etab);
-            soma += branch_io(s.d3, etab);
+            if (s.d0 != 0.0f) soma += branch_io(s.d0, etab);
+            if (s.d1 != 0.0f) soma += branch_io(s.d1, etab);
+            if (s.d2 != 0.0f) soma += branch_io(s.d2, etab);
+            if (s.d3 != 0.0f) soma += branch_io(s.d3, etab);
         } else {
             soma += s.d0; soma += s.d1; soma += s.d2; soma += s.d3;
         }

@@ -580,28 +580,25 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
             float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
             unsigned mod = 0;
             const int q1 = in_l[n + 1];
+            // Every release is >= +0 and the sums start at +0, so adding +0
+            // for the other routes leaves them bit-identical: branch-free.
             for (int q = in_l[n]; q < q1; ++q) {
                 const float r = rel_l[q];
-                if (r == 0.0f) continue;
                 const uint32_t route = route_l[q];
-                if (route & R_MOD) {
-                    if (r > 0) {
-                        const unsigned m = (route >> R_MOD_SHIFT) & 3u;
-                        mod |= 1u << (m - 1);
-                        uint64_t &st = p.stamps[((m - 1) * 2 + (tick & 1)) * N + n0 + n];
-                        if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
-                    }
-                } else if (route & R_INH) {
-                    inh += r;
-                } else {
-                    const float v = r * nscale[k];
-                    switch (route & R_BRANCH) {
-                    case 0: e0 += v; break;
-                    case 1: e1 += v; break;
-                    case 2: e2 += v; break;
-                    default: e3 += v; break;
-                    }
+                if ((route & R_MOD) && r > 0) {
+                    const unsigned m = (route >> R_MOD_SHIFT) & 3u;
+                    mod |= 1u << (m - 1);
+                    uint64_t &st = p.stamps[((m - 1) * 2 + (tick & 1)) * N + n0 + n];
+                    if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
                 }
+                const bool exc = !(route & (R_MOD | R_INH));
+                const float v = exc ? r * nscale[k] : 0.0f;
+                const uint32_t br = route & R_BRANCH;
+                inh += (route & R_MOD) ? 0.0f : ((route & R_INH) ? r : 0.0f);
+                e0 += br == 0 ? v : 0.0f;
+                e1 += br == 1 ? v : 0.0f;
+                e2 += br == 2 ? v : 0.0f;
+                e3 += br == 3 ? v : 0.0f;
             }
             float current = 0.0f;
             if (!(flags & (1u << 30)))
@@ -877,26 +874,21 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int q1 = in_l[n + 1];
             for (int q = in_l[n]; q < q1; ++q) {
                 const float r = rel_l[q];
-                if (r == 0.0f) continue;
                 const uint32_t route = fl_l[q] >> SF_ROUTE_SHIFT;
-                if (route & R_MOD) {
-                    if (r > 0) {
-                        const unsigned mm = (route >> R_MOD_SHIFT) & 3u;
-                        mod |= 1u << (mm - 1);
-                        uint64_t &st = p.stamps[((mm - 1) * 2 + (tick & 1)) * N + n0 + n];
-                        if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
-                    }
-                } else if (route & R_INH) {
-                    inh += r;
-                } else {
-                    const float v = r * nscale[k];
-                    switch (route & R_BRANCH) {
-                    case 0: e0 += v; break;
-                    case 1: e1 += v; break;
-                    case 2: e2 += v; break;
-                    default: e3 += v; break;
-                    }
+                if ((route & R_MOD) && r > 0) {
+                    const unsigned mm = (route >> R_MOD_SHIFT) & 3u;
+                    mod |= 1u << (mm - 1);
+                    uint64_t &st = p.stamps[((mm - 1) * 2 + (tick & 1)) * N + n0 + n];
+                    if (st < uint64_t(tick + 1)) st = uint64_t(tick + 1);
                 }
+                const bool exc = !(route & (R_MOD | R_INH));
+                const float v = exc ? r * nscale[k] : 0.0f;
+                const uint32_t br = route & R_BRANCH;
+                inh += (route & R_MOD) ? 0.0f : ((route & R_INH) ? r : 0.0f);
+                e0 += br == 0 ? v : 0.0f;
+                e1 += br == 1 ? v : 0.0f;
+                e2 += br == 2 ? v : 0.0f;
+                e3 += br == 3 ? v : 0.0f;
             }
             float current = 0.0f;
             if (!(flags & (1u << 30)))

@@ -186,9 +186,9 @@ def c3_sizes(rng, brains=32768, total=4194304, alpha=1.5, xm=32, cap=4096):
     return sizes
 
 
-def c3(seed=3, brains=32768, total_neurons=4194304, fanout=2.0, calls=100):
+def c3(seed=3, brains=32768, total_neurons=4194304, fanout=2.0, calls=100, cap=4096):
     rng = np.random.default_rng(seed)
-    sizes = c3_sizes(rng, brains, total_neurons)
+    sizes = c3_sizes(rng, brains, total_neurons, cap=cap)
     return _assemble('c3', rng, sizes, fanout, _uniform_types([0, 1, 4, 5]), 0.2, 8, 19, 5, 16,
                      -1.0, 0.5, _stdp_mix, 0.06, seed, POISSON_20HZ_16MS, 230.0, 16, calls)
 
