@@ -130,9 +130,9 @@ __device__ inline float clamp_range(float x, float lo, float hi) {   // physiolo
 // every float x by tests/test_division.py.
 __device__ inline float div_const(float x, float c, double rc) {
     const double q = (double)x * rc;
-    const double a = __builtin_fabs(q);
-    if (a != 0.0 && a < 0x1p-126) return x / c;
-    return (float)q;
+    float r = (float)q;
+    if (__builtin_expect(__builtin_fabs(q) < 0x1p-126 && x != 0.0f, 0)) r = x / c;
+    return r;
 }
 
 // Correctly rounded 5 / d for d >= 1: reciprocal estimate refined by two
@@ -226,6 +226,9 @@ __device__ inline float receptor_drive(NeuronRegs &s, const NeuronConst &g, floa
     return s.rns * (0.0f - s.v);
 }
 
+// Dendritic conductances below this give branch_io exactly +0 (see adex_tick).
+constexpr float kBranchQuiet = 0x1p-37f;
+
 // physiology.cuh:226-230
 __device__ inline float branch_io(float c, const uint64_t *etab) {
     float regen = div5(1.0f + ns_expf(div_const(-(c - 4.0f), 0.4f, kRcp04), etab));
@@ -233,10 +236,12 @@ __device__ inline float branch_io(float c, const uint64_t *etab) {
 }
 
 // physiology.cuh:290-374, AdEx branch (types 0, 1, 4, 5).  Returns the spike.
-// Two exact shortcuts: branch_io(+0) is exactly +0 (the regenerative term at
-// zero conductance equals the subtracted offset), so zero dendrites add +0 to
-// a non-negative sum and are skipped; and with adapt_ns == 0, w == +0 and a
-// finite v - rest the adaptation update adds a signed zero to +0, an identity.
+// Two exact shortcuts.  branch_io(c) is exactly +0 for 0 <= c < 2^-37: c - 4
+// rounds to -4, so the regenerative term equals its value at c = 0, which is
+// the subtracted offset (a float in [2^-13, 2^-12), whose half ulp is 2^-37),
+// and c + offset rounds to offset.  Such a dendrite adds +0 to a sum that is
+// never -0, so it is skipped.  With adapt_ns == 0, w == +0 and a finite
+// v - rest, the adaptation update adds a signed zero to +0, an identity.
 __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float current,
                                 const uint64_t *etab) {
     const float h = 0.25f;
@@ -251,10 +256,10 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
         float x = clamp_range(div_const(v - g.thr, g.slope, g.rslope), -20.0f, 10.0f);
         float soma = 0;
         if (cortical) {
-            if (s.d0 != 0.0f) soma += branch_io(s.d0, etab);
-            if (s.d1 != 0.0f) soma += branch_io(s.d1, etab);
-            if (s.d2 != 0.0f) soma += branch_io(s.d2, etab);
-            if (s.d3 != 0.0f) soma += branch_io(s.d3, etab);
+            if (!(s.d0 >= 0.0f && s.d0 < kBranchQuiet)) soma += branch_io(s.d0, etab);
+            if (!(s.d1 >= 0.0f && s.d1 < kBranchQuiet)) soma += branch_io(s.d1, etab);
+            if (!(s.d2 >= 0.0f && s.d2 < kBranchQuiet)) soma += branch_io(s.d2, etab);
+            if (!(s.d3 >= 0.0f && s.d3 < kBranchQuiet)) soma += branch_io(s.d3, etab);
         } else {
             soma += s.d0; soma += s.d1; soma += s.d2; soma += s.d3;
         }

@@ -674,12 +674,20 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // ---------------------------------------------------------------------------
 // Resident form, synapse state in LDS.
 //
-// Same tick structure as k_resident, but every per-synapse value (weight and
+// Same tick order as k_resident, but every per-synapse value (weight and
 // plasticity traces, genes, flags, endpoints, this tick's release) lives in
 // dynamically sized LDS and threads stride over the pack's synapses.  Thread
 // registers then hold only neuron state, which keeps the kernel under 128
-// VGPRs without spills.  PLASTIC = false drops the trace arrays for pools
-// without plastic synapses.
+// VGPRs.  PLASTIC = false drops the trace arrays for pools without plastic
+// synapses.
+//
+// Per tick: phase A (synapse-parallel) applies plasticity of tick t-1, detects
+// the arrivals of tick t, marks the post neuron of every arrival and queues
+// the arrivals that take the short-term release (wave ballot, one LDS atomic
+// per wave); phase A2 runs those releases over the compacted queue; phase B
+// (neuron-parallel) sums the inputs of marked neurons in ascending synapse
+// order and integrates every neuron.  A neuron with no arrival this tick has
+// only +0 releases, so its sums are +0 without reading them.
 // ---------------------------------------------------------------------------
 template <bool PLASTIC>
 __host__ __device__ constexpr int lds_synapse_bytes() {
@@ -698,6 +706,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __shared__ uint32_t hist_l[NMAX];
     __shared__ float depol_l[NMAX];
     __shared__ uint8_t mod_l[NMAX];
+    __shared__ uint8_t hit_l[NMAX];
     __shared__ uint16_t in_l[NMAX + 1];
     __shared__ int64_t btick_l[kPackBrains];
     __shared__ uint8_t brun_l[kPackBrains], bfrozen_l[kPackBrains];
@@ -740,6 +749,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         hist_l[n] = p.hist[n0 + n];
         in_l[n] = uint16_t(p.nin[n0 + n] - s0);
         mod_l[n] = 0;
+        hit_l[n] = 0;
         depol_l[n] = 0;
     }
     if (tid == 0) {
@@ -831,6 +841,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                     const bool arr = tick >= d && ((hist_l[pre] >> (d - 1)) & 1u);
                     arr_l[q] = arr;
                     rel_l[q] = arr ? w_l[q] : 0.0f;
+                    if (arr) hit_l[post] = 1;
                     my_events += arr;
                     arrive_stp = arr && (route & R_STP);
                 }
@@ -850,9 +861,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int n_q = qlen;
             for (int j = tid; j < n_q; j += WG) {
                 const int q = queue_l[j];
-                const int64_t gq = s0 + q;
                 const int64_t tick = btick_l[sbr_l[q]] + t;
-                rel_l[q] = release_at(p, gq, fl_l[q], rel_l[q], tick, etab);
+                rel_l[q] = release_at(p, s0 + q, fl_l[q], rel_l[q], tick, etab);
             }
         }
         __syncthreads();
@@ -871,7 +881,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int64_t tick = btick_l[nbr[k]] + t;
             float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
             unsigned mod = 0;
-            const int q1 = in_l[n + 1];
+            const int q1 = hit_l[n] ? int(in_l[n + 1]) : 0;
+            hit_l[n] = 0;
             for (int q = in_l[n]; q < q1; ++q) {
                 const float r = rel_l[q];
                 const uint32_t route = fl_l[q] >> SF_ROUTE_SHIFT;

@@ -909,6 +909,21 @@ void ora_fn_homeostasis(const ns_neuron_gene *g, ns_neuron_state *s, float elaps
     homeostasis(*g, *s, elapsed, hz);
 }
 float ora_fn_expf(float x) { return expf(x); }
+float ora_fn_branch_io(float c) { return branch_io(c); }
+// Number of float bit patterns b in [lo, hi), stepping by `step`, for which
+// branch_io(b) is not bitwise +0.
+uint64_t ora_branch_io_nonzero(uint32_t lo, uint32_t hi, uint32_t step) {
+    uint64_t bad = 0;
+    for (uint64_t b = lo; b < hi; b += step) {
+        uint32_t u = uint32_t(b), r;
+        float c;
+        std::memcpy(&c, &u, 4);
+        const float y = branch_io(c);
+        std::memcpy(&r, &y, 4);
+        bad += r != 0;
+    }
+    return bad;
+}
 float ora_fn_logf(float x) { return logf(x); }
 
 }  // extern "C"

@@ -94,9 +94,11 @@ def oracle_lib():
         lib.ora_batch_step.argtypes = [ct.POINTER(_vp), ct.c_int, _f, ct.c_int, _u8, ct.c_int,
                                        ct.c_float, _f, ct.c_int, _i32, ct.c_int]
         lib.ora_batch_step.restype = ct.c_int
-        for name in ('ora_fn_expf', 'ora_fn_logf'):
+        for name in ('ora_fn_expf', 'ora_fn_logf', 'ora_fn_branch_io'):
             getattr(lib, name).argtypes = [ct.c_float]
             getattr(lib, name).restype = ct.c_float
+        lib.ora_branch_io_nonzero.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_uint32]
+        lib.ora_branch_io_nonzero.restype = ct.c_uint64
         _fn_signatures(lib, 'ora_fn_')
         _oracle = lib
     return _oracle
@@ -142,6 +144,8 @@ def ref_lib():
         lib.ref_brain_genes.argtypes = [_vp, _vp, _vp]
         _fn_signatures(lib, 'ref_fn_')
         lib.ref_fn_plasticity_quiet.argtypes = [_vp, _vp, ct.c_int]
+        lib.ref_fn_dendritic_transfer.argtypes = [ct.c_float]
+        lib.ref_fn_dendritic_transfer.restype = ct.c_float
         _ref = lib
     return _ref
 

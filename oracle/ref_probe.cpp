@@ -29,6 +29,7 @@ void ref_brain_genes(void *p, NeuronGene *ng, SynapseGene *sg) {
         std::memcpy(sg, b.connections.data(), b.connections.size() * sizeof(SynapseGene));
 }
 
+float ref_fn_dendritic_transfer(float c) { return dendritic_transfer(c); }
 void ref_fn_profile(int type, NeuronGene *out) { *out = initial_profile((unsigned char)type); }
 void ref_fn_initial_neuron(const NeuronGene *g, uint32_t seed, NeuronState *out) {
     *out = initial_neuron_state(*g, seed);

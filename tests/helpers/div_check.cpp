@@ -14,9 +14,9 @@
 
 static inline float div_const(float x, float c, double rc) {
     const double q = (double)x * rc;
-    const double a = std::fabs(q);
-    if (a != 0.0 && a < 0x1p-126) return x / c;
-    return (float)q;
+    float r = (float)q;
+    if (std::fabs(q) < 0x1p-126 && x != 0.0f) r = x / c;
+    return r;
 }
 
 static inline float div5(float d, int perturb) {
