@@ -28,7 +28,7 @@ sys.path[:0] = [str(ROOT / 'neural-selection_amd'), str(ROOT / 'oracle')]
 import ns_amd  # noqa: E402
 import ns_workloads as W  # noqa: E402
 
-HBM_PEAK = 8.0e12     # MI355X HBM3E, bytes/s
+HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 # Algorithmic bytes per unit of work (SoA-minimal streaming form, one tick):
 NEURON_UPDATE_BYTES = 178
 SYNAPSE_UPDATE_BYTES_PLASTIC = 66
@@ -69,7 +69,7 @@ def describe(name, w):
     return base
 
 
-def cpu_baseline(w, seconds=12.0, max_neurons=65536):
+def cpu_baseline(w, seconds=15.0, max_neurons=65536):
     """The CPU oracle (bit-identical to the reference's CPU path) on the host
     cores, over the first brains of the same workload."""
     import ctypes as ct
@@ -93,7 +93,7 @@ def cpu_baseline(w, seconds=12.0, max_neurons=65536):
     status = np.zeros(n_b, np.int32)
     neurons = int(w.neuron_counts[:n_b].sum())
     calls, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds and calls < 400:
+    while time.perf_counter() - t0 < seconds and calls < 20000:
         x = np.ascontiguousarray(w.inputs(calls)[:n_b])
         lib.ora_batch_step(arr, n_b, x.ctypes.data_as(O._f), w.channels, None, w.ticks, w.gain,
                            out.ctypes.data_as(O._f), w.outputs, status.ctypes.data_as(O._i32),
@@ -106,14 +106,20 @@ def cpu_baseline(w, seconds=12.0, max_neurons=65536):
                       f'x {w.ticks} ticks, {dt:.1f} s wall, oracle/liboracle.so threaded over brains'}
 
 
-def load_traffic(name):
-    f = ROOT / 'profiles' / f'k_resident_{name}_hbm.json'
+def load_traffic(kernel, workload, tile):
+    """Measured HBM bytes per launch of `kernel` from its committed rocprofv3
+    summary, if that summary was taken on this workload and tile."""
+    f = ROOT / 'profiles' / f'{kernel}_{workload}.md'
     if not f.exists():
         return None
-    try:
-        return json.loads(f.read_text()).get('bytes_per_launch')
-    except (OSError, ValueError):
+    text = f.read_text().splitlines()
+    want = ','.join(str(x) for x in tile)
+    if not text or not text[0].rstrip().endswith(f'tile {want}'):
         return None
+    for line in text:
+        if line.startswith('bytes_per_launch:'):
+            return float(line.split(':')[1])
+    return None
 
 
 def main():
@@ -153,12 +159,14 @@ def main():
         f'({time.perf_counter() - t0:.1f} s)')
     batch = ns_amd.GpuBatch(workload=w)
     batch.reset(w.seeds())
+    tile_key = None          # the tile as set_tiles names it
     if args.tiles:
-        batch.set_tiles(*[int(x) for x in args.tiles.split(',')])
+        tile_key = tuple(int(x) for x in args.tiles.split(','))
+        batch.set_tiles(*tile_key)
     elif not args.no_autotune:
         log('autotuning tile shape')
-        tiles = batch.autotune(w.ticks)
-        log(f'tile {tiles}')
+        tile_key = tuple(batch.autotune(w.ticks))
+        log(f'tile {tile_key}')
     info = batch.info()
     log(f'packs {info["resident_packs"]} fallback brains {info["fallback_brains"]} '
         f'({info["fallback_neurons"]} neurons) device {info["device_bytes"] / 2**30:.2f} GiB')
@@ -204,7 +212,7 @@ def main():
     syn_bytes = SYNAPSE_UPDATE_BYTES_PLASTIC if plastic else SYNAPSE_UPDATE_BYTES_STATIC
     alg_bytes = (NEURON_UPDATE_BYTES * res_neurons * w.ticks + syn_bytes * res_synapses * w.ticks
                  + EVENT_BYTES * res_events)
-    achieved = alg_bytes / (res_ms * 1e-3) if res_ms > 0 else 0.0
+    achieved = alg_bytes / (res_ms * 1e-3) / 1e9 if res_ms > 0 else 0.0
     per_tick_share = (i1['per_tick_ms'] - i0['per_tick_ms']) / max(args.steps, 1)
 
     # plain per-tick form on the same pools (the 5x denominator)
@@ -219,6 +227,8 @@ def main():
     torch.cuda.synchronize()
     plain_wall = time.perf_counter() - t1
     plain_value = w.total_neurons * w.ticks * args.plain_steps / plain_wall
+    plain_alg = (NEURON_UPDATE_BYTES * w.total_neurons * w.ticks
+                 + syn_bytes * w.total_synapses * w.ticks + EVENT_BYTES * ev / args.steps)
     batch.set_form(ns_amd.FORM_RESIDENT)
     log(f'resident {value:.3e} vs plain per-tick {plain_value:.3e} neuron-updates/s')
 
@@ -229,6 +239,13 @@ def main():
 
     tile = (i1['threads'], i1['neurons_per_thread'], i1['synapses_per_thread'], i1['min_waves'],
             i1['lds_synapses'])
+    if tile[2] == 0:
+        kernel = 'k_resident_lds'
+        kernel_full = f'k_resident_lds<{tile[0]}, {tile[1]}, {tile[3]}, {str(plastic).lower()}>'
+    else:
+        kernel = 'k_resident'
+        kernel_full = f'k_resident<{tile[0]}, {tile[1]}, {tile[2]}, {tile[3]}>'
+    traffic = load_traffic(kernel, args.workload, tile_key) if tile_key else None
     line = {
         'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
         'value': value,
@@ -249,17 +266,21 @@ def main():
                             'p99': float(np.percentile(step_ms, 99)),
                             'max': float(step_ms.max())},
         'plain_per_tick': {'value': plain_value, 'speedup': value / plain_value,
-                           'steps': args.plain_steps},
-        'tile': {'threads': tile[0], 'neurons_per_thread': tile[1], 'synapses_per_thread': tile[2],
+                           'steps': args.plain_steps,
+                           'ms_per_step': plain_wall * 1e3 / max(args.plain_steps, 1),
+                           'effective_GBps': plain_alg / (plain_wall / max(args.plain_steps, 1)) / 1e9},
+        'tile': {'key': list(tile_key) if tile_key else None,
+                 'threads': tile[0], 'neurons_per_thread': tile[1], 'synapses_per_thread': tile[2],
                  'min_waves_per_simd': tile[3], 'lds_synapses': tile[4],
                  'packs': i1['resident_packs'],
                  'big_brains': i1['big_brains'], 'big_neurons': i1['big_neurons'],
                  'fallback_brains': i1['fallback_brains'],
                  'fallback_neurons': i1['fallback_neurons'],
                  'per_tick_ms_per_step': per_tick_share},
-        'roofline': {'bound': 'hbm', 'kernel': 'k_resident', 'achieved': achieved,
-                     'peak': HBM_PEAK, 'unit': 'B/s', 'frac': achieved / HBM_PEAK,
-                     'traffic': load_traffic(args.workload),
+        'roofline': {'bound': 'hbm', 'kernel': kernel_full, 'achieved': achieved,
+                     'peak': HBM_PEAK_GBPS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBPS,
+                     'traffic': traffic,
+                     'measured_GBps': traffic / (res_ms * 1e-3) / 1e9 if traffic and res_ms else None,
                      'launch_ms': res_ms, 'algorithmic_bytes_per_launch': alg_bytes},
         'cpu_baseline': cpu,
     }
