@@ -122,6 +122,85 @@ def load_traffic(kernel, workload, tile):
     return None
 
 
+def serving_genomes(count, seed, lo=16, hi=4096):
+    """C4 episodes: sizes log-uniform on [lo, hi], 8 sensors, 2 motors."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        size = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        nr, sr, mo = W.episode_genome(rng, size)
+        out.append((nr, sr, mo, 1000 + i, 5000 + i))
+    return out
+
+
+def bench_serving(args):
+    """Continuous batching under Poisson arrivals: per-episode latency (submit
+    to finish) and whole-run throughput, one run per arrival rate."""
+    import ctypes as ct
+    L = ns_amd.lib()
+    rates = [float(x) for x in args.rates.split(',')]
+    eps = serving_genomes(args.episodes, seed=17)
+    total_neurons = sum(len(e[0]) for e in eps)
+    f = ct.POINTER(ct.c_float)
+    runs = []
+    for rate in rates:
+        cfg = ns_amd.ServeConfig(neuron_capacity=1 << 22, synapse_capacity=1 << 24,
+                                 episode_capacity=8192, steps_per_episode=args.episode_steps,
+                                 ticks_per_step=16, steps_per_launch=25, noise_sigma=0.01,
+                                 input_gain=230.0)
+        srv = L.ns_serve_create(ct.byref(cfg))
+        if not srv:
+            raise SystemExit(ns_amd.error())
+        gaps = np.random.default_rng(int(rate)).exponential(1.0 / rate, len(eps))
+        log(f'serving: {len(eps)} episodes at {rate:g}/s')
+        t0 = time.perf_counter()
+        due = t0
+        for k, (nr, sr, mo, seed, tseed) in enumerate(eps):
+            due += gaps[k]
+            delay = due - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            if L.ns_serve_submit(srv, len(nr), len(sr), nr.ctypes.data_as(f), sr.ctypes.data_as(f),
+                                 mo.ctypes.data_as(ct.POINTER(ct.c_int)), seed, tseed) < 0:
+                raise SystemExit(ns_amd.error())
+        if L.ns_serve_drain(srv):
+            raise SystemExit(ns_amd.error())
+        wall = time.perf_counter() - t0
+        res = (ns_amd.ServeResult * len(eps))()
+        n = L.ns_serve_poll(srv, res, len(eps))
+        stats = ns_amd.ServeStats()
+        L.ns_serve_stats_get(srv, ct.byref(stats))
+        L.ns_serve_destroy(srv)
+        lat = np.array([r.finish_s - r.submit_s for r in res[:n]]) * 1e3
+        wait = np.array([r.admit_s - r.submit_s for r in res[:n]]) * 1e3
+        nu = total_neurons * args.episode_steps * 16
+        runs.append({'rate_per_s': rate, 'episodes': n, 'wall_s': wall,
+                     'episodes_per_s': n / wall, 'neuron_updates_per_s': nu / wall,
+                     'latency_ms': {'p50': float(np.percentile(lat, 50)),
+                                    'p99': float(np.percentile(lat, 99)),
+                                    'max': float(lat.max())},
+                     'admission_wait_ms': {'p50': float(np.percentile(wait, 50)),
+                                           'p99': float(np.percentile(wait, 99))},
+                     'launch_rounds': stats.launches, 'resident_cohorts_peak': stats.resident_peak,
+                     'device_busy_s': stats.busy_s})
+        log(f'  p50 {runs[-1]["latency_ms"]["p50"]:.1f} ms, p99 {runs[-1]["latency_ms"]["p99"]:.1f} ms, '
+            f'{runs[-1]["neuron_updates_per_s"]:.3e} neuron-updates/s')
+    top = runs[-1]
+    line = {
+        'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
+        'value': top['neuron_updates_per_s'], 'unit': 'neuron-updates/s', 'n_gpus': 1,
+        'steps': args.episodes, 'warmup': 0, 'ms_per_step': top['wall_s'] * 1e3 / max(top['episodes'], 1),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic (seeded episode genomes and tasks)',
+        'config': {'workload': 'c4', 'desc': 'serving: Poisson arrivals of genome-evaluation '
+                   'episodes, sizes log-uniform 16..4096 neurons, 4-d linear task, 8 sensors, '
+                   f'2 motors, {args.episode_steps} steps x 16 ticks', 'episodes': len(eps),
+                   'neurons_total': total_neurons, 'rates_per_s': rates},
+        'serving': runs, 'roofline': None, 'cpu_baseline': None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -132,6 +211,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-autotune', action='store_true')
     ap.add_argument('--tiles', default='')
+    ap.add_argument('--rates', default='100,1000,5000', help='c4: Poisson arrival rates (1/s)')
+    ap.add_argument('--episodes', type=int, default=300, help='c4: episodes per rate')
+    ap.add_argument('--episode-steps', type=int, default=500, help='c4: env steps per episode')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy MIB MiB device to device (HBM counter calibration)')
     args = ap.parse_args()
@@ -149,6 +231,9 @@ def main():
         print(f'bench.py measures one GPU only (asked for {args.gpus}); brains are independent, '
               'so the update shards by brain with no exchange (see DESIGN.md)', file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'c4':
+        bench_serving(args)
+        return
     import torch
     torch.cuda.init()
 

@@ -171,7 +171,10 @@ struct Server {
         for (int i = 0; i < count; ++i)
             c.specs.push_back(BrainSpec{&c.reqs[i].brain, i, i * kChannels, i * kOutputs, false});
         c.stream = streams[stream_rr++ % streams.size()];
-        c.pool.build(c.specs, kOutputs, Tile{256, 2, 4}, c.stream);
+        // LDS-resident tick kernel (the fastest measured shape for pools of
+        // this kind); episodes carry no plasticity, so packs take up to 5632
+        // synapses each.
+        c.pool.build(c.specs, kOutputs, Tile{1024, 1, 0, 1, 3584}, c.stream);
         c.d_input = c.io.alloc<float>(int64_t(count) * kChannels);
         c.d_out = c.io.alloc<float>(int64_t(count) * kOutputs);
         c.d_task = c.io.alloc<float>(int64_t(count) * kTaskFloats);
