@@ -241,8 +241,33 @@ typedef struct ns_batch_info {
     int64_t big_brains, big_neurons;   /* one workgroup each in the resident launch */
     int min_waves;
     int lds_synapses;           /* LDS synapse capacity per pack, 0 = register slots */
+    /* slots (ns_gpu_reserve / ns_gpu_replace) */
+    int64_t slots, free_slots;  /* pool brain slots; slots without a brain */
+    int64_t slot_neurons, slot_synapses;   /* total slot capacity */
+    int64_t placements, relayouts;         /* replacements placed in free slots / by rebuild */
 } ns_batch_info;
 int ns_gpu_info(void *batch, ns_batch_info *info);
+
+/* Brain turnover (births and deaths) without rebuilding the pools.  These
+ * have no counterpart in the reference's batch ABI; they serve the world's
+ * births/deaths (src/main.cu:364-454, 1302-1338: slot claims and genome
+ * lists) and continuous batching.
+ *
+ * ns_gpu_reserve: rebuild the pools (state kept) with slot capacities rounded
+ * up a geometric ladder of ratio 1 + slack and spare empty slots for a
+ * fraction `spare` of the batch, spread over the brain sizes.
+ *
+ * ns_gpu_replace: brains indices[0..count) take the genomes `handles` (made by
+ * ns_create; copied, the caller keeps them) with fresh state seeded like
+ * ns_gpu_reset with seeds[j].  Each newborn goes to the best-fitting free slot
+ * (spares, vacancies, or the slot it replaces); the pools are rebuilt (state
+ * kept) only when none fits, or for LP/Prinz cells.  Enqueued on `stream`
+ * (a hipStream_t, NULL = the batch's own stream) after earlier work there;
+ * later ns_gpu_step_device calls on that stream see the new brains.  Results
+ * are those of a batch created with the new genomes and reset with seeds. */
+int ns_gpu_reserve(void *batch, float slack, float spare);
+int ns_gpu_replace(void *batch, int count, const int *indices, void **handles,
+                   const uint32_t *seeds, void *stream);
 
 /* Record HIP events around every tick-kernel launch of this batch (adds one
  * event pair per launch; totals read back by ns_gpu_info). */

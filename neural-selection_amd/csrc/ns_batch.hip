@@ -8,6 +8,8 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <set>
+#include <tuple>
 
 #include "ns_host.h"
 
@@ -159,6 +161,122 @@ const TileShape *find_tile(const Tile &t) {
     return nullptr;
 }
 
+// A brain plastic pair exists: a valid excitatory-to-excitatory synapse with a
+// plasticity profile (the pairs the plastic kernels keep traces for).
+static bool has_plastic_pair(const HostBrain &b) {
+    for (const auto &g : b.edges) {
+        const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
+        if (g.plasticity && g.active && a.active && z.active && !a.inhibitory_transmitter &&
+            excitatory_cell(a.type) && !a.modulator && !z.inhibitory_transmitter &&
+            excitatory_cell(z.type))
+            return true;
+    }
+    return false;
+}
+
+static bool has_slow_cell(const HostBrain &b) {
+    for (const auto &g : b.genes)
+        if (g.type == 2 || g.type == 3) return true;
+    return false;
+}
+
+// Structure-of-arrays image of one slot: the occupant (or nothing) laid out
+// at pool positions n0.., s0.. in a slot of cap_n neurons and cap_s synapses.
+// Neurons past the occupant are inactive with empty incoming runs; synapses
+// past it are invalid and point at the slot's first neuron, so the tail never
+// changes a result.
+struct SlotImage {
+    int cap_n = 0, cap_s = 0;
+    std::vector<float> ng, nx, sg;
+    std::vector<uint32_t> nflags, norig, sflags, sorig;
+    std::vector<int32_t> ninput, nin, spre, spost, motor, syn_perm;
+    int32_t bout0 = 0;
+
+    void make(const BrainSpec *spec, int64_t n0, int64_t s0, int cn, int cs, int outs) {
+        cap_n = cn; cap_s = cs;
+        ng.assign(size_t(NG_FIELDS) * cn, 0.0f);
+        nx.assign(size_t(NX_FIELDS) * cn, 0.0f);
+        nflags.assign(cn, 0u); norig.assign(cn, 0u);
+        ninput.assign(cn, -1); nin.assign(cn, int32_t(s0));
+        sg.assign(size_t(SG_FIELDS) * cs, 0.0f);
+        sflags.assign(cs, 1u); sorig.assign(cs, 0u);          // delay 1, invalid route
+        spre.assign(cs, int32_t(n0)); spost.assign(cs, int32_t(n0));
+        syn_perm.assign(cs, -1);
+        motor.assign(outs, -1);
+        bout0 = 0;
+        if (!spec || !spec->brain) return;
+        const HostBrain &b = *spec->brain;
+        const int nc = int(b.genes.size()), sc = int(b.edges.size());
+        bout0 = spec->output_base;
+        for (int j = 0; j < outs; ++j) motor[j] = b.motors[j] >= 0 ? int32_t(n0 + b.motors[j]) : -1;
+        for (int i = 0; i < nc; ++i) {
+            const ns_neuron_gene &g = b.genes[i];
+            const float f[NG_FIELDS] = {g.capacitance_pf, g.leak_ns, g.rest_mv, g.threshold_mv,
+                                        g.slope_mv, g.adapt_ns, g.adapt_tau_ms, g.spike_adapt_pa,
+                                        g.reset_mv, g.afterdepol_spike_pa, g.afterdepol_tau_ms,
+                                        g.receptor_baseline_hz, g.receptor_gain_hz,
+                                        g.receptor_event_ns, g.receptor_decay_ms};
+            for (int k = 0; k < NG_FIELDS; ++k) ng[size_t(k) * cn + i] = f[k];
+            for (int k = 0; k < 4; ++k) nx[size_t(NX_LP + k) * cn + i] = g.lp_scale[k];
+            for (int k = 0; k < 8; ++k) nx[size_t(NX_PRINZ + k) * cn + i] = g.prinz_g[k];
+            nx[size_t(NX_AMBIENT) * cn + i] = g.prinz_ambient_leak_ns;
+            nflags[i] = uint32_t(g.type) | (g.active ? NF_ACTIVE : 0) |
+                        (b.receptor[i] ? NF_RECEPTOR : 0) | (spec->world ? NF_WORLD : 0);
+            ninput[i] = b.sensory[i] >= 0 ? spec->input_base + b.sensory[i] : -1;
+            norig[i] = b.orig_n.empty() ? uint32_t(i) : b.orig_n[i];
+        }
+        // synapses sorted by (post, edge index): each neuron's incoming run is
+        // contiguous and in the reference's accumulation order
+        std::vector<int> perm(sc);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int x, int y) { return b.edges[x].post < b.edges[y].post; });
+        std::vector<int> incoming(nc + 1, 0);
+        for (int j = 0; j < sc; ++j) incoming[b.edges[j].post + 1]++;
+        for (int i = 0; i < nc; ++i) incoming[i + 1] += incoming[i];
+        for (int i = 0; i < cn; ++i) nin[i] = int32_t(s0 + (i < nc ? incoming[i] : sc));
+        for (int k = 0; k < sc; ++k) {
+            const int j = perm[k];
+            const ns_synapse_gene &g = b.edges[j];
+            const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
+            const float f[SG_FIELDS] = {g.initial_weight_ns, g.utilization, g.recovery_ms,
+                                        g.facilitation_ms, g.slow_recovery_ms, g.slow_depletion};
+            for (int m = 0; m < SG_FIELDS; ++m) sg[size_t(m) * cs + k] = f[m];
+            uint32_t route = uint32_t(g.branch & 3);
+            if (a.modulator) route |= uint32_t(a.modulator & 3) << R_MOD_SHIFT;
+            else if (a.inhibitory_transmitter) route |= R_INH;
+            if (g.active && a.active && z.active) route |= R_VALID;
+            if (!a.inhibitory_transmitter || a.modulator) route |= R_STP;
+            if (!a.inhibitory_transmitter && excitatory_cell(a.type) && !a.modulator &&
+                !z.inhibitory_transmitter && excitatory_cell(z.type))
+                route |= R_PAIR;
+            sflags[k] = uint32_t(g.delay) | (uint32_t(g.plasticity & 7) << SF_PLAST_SHIFT) |
+                        (uint32_t(g.mod_receptors & 7) << SF_MODREC_SHIFT) |
+                        (uint32_t(site_count(g)) << SF_SITES_SHIFT) | (route << SF_ROUTE_SHIFT);
+            spre[k] = int32_t(n0 + g.pre);
+            spost[k] = int32_t(n0 + g.post);
+            sorig[k] = b.orig_s.empty() ? uint32_t(j) : b.orig_s[j];
+            syn_perm[k] = j;
+        }
+    }
+};
+
+// Type-sorted slot order of one pack (cortical AdEx (type 0), receptor cells
+// (5), other AdEx (1, 4), LFS (6), then inactive cells): waves of one type
+// take the same branches.
+static void pack_order(const Pack &pk, const uint32_t *nflags, uint16_t *perm) {
+    std::vector<std::pair<int, int>> key;
+    key.reserve(pk.nn);
+    for (int j = 0; j < pk.nn; ++j) {
+        const uint32_t f = nflags[j];
+        const uint32_t ty = f & NF_TYPE;
+        const int cls = !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
+        key.push_back({cls, j});
+    }
+    std::stable_sort(key.begin(), key.end());
+    for (int j = 0; j < pk.nn; ++j) perm[j] = uint16_t(key[j].second);
+}
+
 void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile &t,
                        hipStream_t st) {
     arena.release();
@@ -169,47 +287,43 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     const TileShape *shape = find_tile(t);
     if (!shape) throw std::runtime_error("Tile shape not compiled");
     plastic = false;
+    nbatch = 0;
     for (const auto &spec : specs) {
-        const HostBrain &b = *spec.brain;
-        for (const auto &g : b.edges) {
-            const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
-            if (g.plasticity && g.active && a.active && z.active && !a.inhibitory_transmitter &&
-                excitatory_cell(a.type) && !a.modulator && !z.inhibitory_transmitter &&
-                excitatory_cell(z.type))
-                plastic = true;
-        }
+        if (!spec.brain) continue;
+        if (has_plastic_pair(*spec.brain)) plastic = true;
+        nbatch = std::max(nbatch, spec.batch_index + 1);
     }
     const int nmax = t.nmax();
     const int smax = t.ks > 0 ? t.threads * t.ks : (plastic ? shape->lds_plastic : shape->lds_static);
     smax_lds = t.ks > 0 ? 0 : smax;
     const int brains_per_pack = std::min(kPackBrains, std::max(1, kPackMotors / std::max(1, outs)));
 
-    // 1. Classes: brains with LP/Prinz cells take the per-tick kernels; brains
-    // too large for a pack get a workgroup of their own in the resident
-    // launch; the rest are packed several per workgroup.
+    // 1. Slot sizes and classes: slots whose brain has LP/Prinz cells take the
+    // per-tick kernels; slots too large for a pack get a workgroup of their
+    // own in the resident launch; the rest are packed several per workgroup.
+    std::vector<int> capn(nb), caps(nb);
     std::vector<uint8_t> slow(nb, 0), eligible(nb, 0), large(nb, 0);
     for (int i = 0; i < nb; ++i) {
-        const HostBrain &b = *specs[i].brain;
-        for (const auto &g : b.genes)
-            if (g.type == 2 || g.type == 3) slow[i] = 1;
-        const int nc = int(b.genes.size()), sc = int(b.edges.size());
-        eligible[i] = !slow[i] && nc <= nmax && sc <= smax && outs <= kPackMotors;
+        const BrainSpec &spec = specs[i];
+        const int on = spec.brain ? int(spec.brain->genes.size()) : 0;
+        const int os = spec.brain ? int(spec.brain->edges.size()) : 0;
+        capn[i] = std::max(std::max(on, spec.cap_n), 1);
+        caps[i] = std::max(os, spec.cap_s);
+        slow[i] = spec.brain && has_slow_cell(*spec.brain);
+        eligible[i] = !slow[i] && capn[i] <= nmax && caps[i] <= smax && outs <= kPackMotors;
         large[i] = !slow[i] && !eligible[i];
     }
-    // 2. First-fit-decreasing packing of resident brains by cost.
+    // 2. First-fit-decreasing packing of resident slots by cost.
     std::vector<int> order;
     for (int i = 0; i < nb; ++i)
         if (eligible[i]) order.push_back(i);
-    auto cost = [&](int i) {
-        const HostBrain &b = *specs[i].brain;
-        return int64_t(b.genes.size()) * 24 + int64_t(b.edges.size());
-    };
+    auto cost = [&](int i) { return int64_t(capn[i]) * 24 + int64_t(caps[i]); };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
     std::vector<Bin> bins;
     const size_t window = 64;
     size_t open_from = 0;
     for (int i : order) {
-        const int nc = int(specs[i].brain->genes.size()), sc = int(specs[i].brain->edges.size());
+        const int nc = capn[i], sc = caps[i];
         bool placed = false;
         for (size_t k = open_from; k < bins.size(); ++k) {
             Bin &bin = bins[k];
@@ -226,7 +340,7 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
             if (bins.size() - open_from > window) ++open_from;
         }
     }
-    // 3. Pool order: per-tick brains, then large brains (largest first, so
+    // 3. Pool order: per-tick slots, then large slots (largest first, so
     // their long-running workgroups start early), then bins.
     std::vector<int> pool_order, large_order;
     for (int i = 0; i < nb; ++i)
@@ -242,21 +356,29 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
 
     // 4. Positions.
     bn0.assign(nb, 0); bnc.assign(nb, 0); bs0.assign(nb, 0); bsc.assign(nb, 0);
-    batch_of.assign(nb, 0); pool_of.assign(nb, 0); fallback.assign(nb, 0);
+    occ_n.assign(nb, 0); occ_s.assign(nb, 0);
+    batch_of.assign(nb, -1); pool_of.assign(nbatch, -1); fallback.assign(nb, 0);
+    slot_kind.assign(nb, 2); pack_of.assign(nb, -1);
     nn = 0; ns = 0;
     for (int pb = 0; pb < nb; ++pb) {
         const int i = pool_order[pb];
-        const HostBrain &b = *specs[i].brain;
-        bn0[pb] = int32_t(nn); bnc[pb] = int32_t(b.genes.size());
-        bs0[pb] = int32_t(ns); bsc[pb] = int32_t(b.edges.size());
+        const BrainSpec &spec = specs[i];
+        bn0[pb] = int32_t(nn); bnc[pb] = capn[i];
+        bs0[pb] = int32_t(ns); bsc[pb] = caps[i];
         nn += bnc[pb]; ns += bsc[pb];
-        batch_of[pb] = specs[i].batch_index;
-        pool_of[specs[i].batch_index] = pb;
+        if (spec.brain) {
+            occ_n[pb] = int32_t(spec.brain->genes.size());
+            occ_s[pb] = int32_t(spec.brain->edges.size());
+            batch_of[pb] = spec.batch_index;
+            pool_of[spec.batch_index] = pb;
+        }
         fallback[pb] = pb < fallback_brains;
+        slot_kind[pb] = pb < fallback_brains ? 0 : pb < fallback_brains + nbig ? 1 : 2;
     }
-    if (nn > INT32_MAX || ns > INT32_MAX) throw std::runtime_error("Pool exceeds 2^31 elements");
+    if (nn >= (int64_t(1) << 30) || ns >= (int64_t(1) << 30))
+        throw std::runtime_error("Pool exceeds 2^30 elements");
 
-    // 5. Host structure-of-arrays images.
+    // 5. Host structure-of-arrays images, one slot at a time.
     const int64_t N = std::max<int64_t>(nn, 1), S = std::max<int64_t>(ns, 1);
     std::vector<float> ng(NG_FIELDS * N, 0.0f), nx(NX_FIELDS * N, 0.0f);
     std::vector<uint32_t> nflags(N, 0), norig(N, 0);
@@ -266,71 +388,41 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     std::vector<int32_t> spre(S, 0), spost(S, 0), sbrain(S, 0), motor(int64_t(nb) * outs, 0);
     std::vector<int32_t> bout0(nb, 0);
     syn_perm.assign(S, 0);
+    SlotImage img;
     for (int pb = 0; pb < nb; ++pb) {
         const BrainSpec &spec = specs[pool_order[pb]];
-        const HostBrain &b = *spec.brain;
         const int64_t n0 = bn0[pb], s0 = bs0[pb];
-        const int nc = bnc[pb], sc = bsc[pb];
-        bout0[pb] = spec.output_base;
-        for (int j = 0; j < outs; ++j)
-            motor[int64_t(pb) * outs + j] = b.motors[j] >= 0 ? int32_t(n0 + b.motors[j]) : -1;
-        for (int i = 0; i < nc; ++i) {
-            const ns_neuron_gene &g = b.genes[i];
+        const int cn = bnc[pb], cs = bsc[pb];
+        img.make(&spec, n0, s0, cn, cs, outs);
+        bout0[pb] = img.bout0;
+        for (int j = 0; j < outs; ++j) motor[int64_t(pb) * outs + j] = img.motor[j];
+        for (int i = 0; i < cn; ++i) {
             const int64_t n = n0 + i;
-            const float f[NG_FIELDS] = {g.capacitance_pf, g.leak_ns, g.rest_mv, g.threshold_mv,
-                                        g.slope_mv, g.adapt_ns, g.adapt_tau_ms, g.spike_adapt_pa,
-                                        g.reset_mv, g.afterdepol_spike_pa, g.afterdepol_tau_ms,
-                                        g.receptor_baseline_hz, g.receptor_gain_hz,
-                                        g.receptor_event_ns, g.receptor_decay_ms};
-            for (int k = 0; k < NG_FIELDS; ++k) ng[k * N + n] = f[k];
-            for (int k = 0; k < 4; ++k) nx[(NX_LP + k) * N + n] = g.lp_scale[k];
-            for (int k = 0; k < 8; ++k) nx[(NX_PRINZ + k) * N + n] = g.prinz_g[k];
-            nx[NX_AMBIENT * N + n] = g.prinz_ambient_leak_ns;
-            nflags[n] = uint32_t(g.type) | (g.active ? NF_ACTIVE : 0) |
-                        (b.receptor[i] ? NF_RECEPTOR : 0) | (spec.world ? NF_WORLD : 0);
-            ninput[n] = b.sensory[i] >= 0 ? spec.input_base + b.sensory[i] : -1;
+            for (int k = 0; k < NG_FIELDS; ++k) ng[k * N + n] = img.ng[size_t(k) * cn + i];
+            for (int k = 0; k < NX_FIELDS; ++k) nx[k * N + n] = img.nx[size_t(k) * cn + i];
+            nflags[n] = img.nflags[i];
+            ninput[n] = img.ninput[i];
+            nin[n] = img.nin[i];
             nbrain[n] = pb;
-            norig[n] = b.orig_n.empty() ? uint32_t(i) : b.orig_n[i];
-            if (g.type == 2 || g.type == 3) slow_pos.push_back(int32_t(n));
+            norig[n] = img.norig[i];
+            const uint32_t ty = img.nflags[i] & NF_TYPE;
+            if (i < occ_n[pb] && (ty == 2 || ty == 3)) slow_pos.push_back(int32_t(n));
         }
-        // synapses sorted by (post, edge index): each neuron's incoming run is
-        // contiguous and in the reference's accumulation order
-        std::vector<int> perm(sc);
-        std::iota(perm.begin(), perm.end(), 0);
-        std::stable_sort(perm.begin(), perm.end(),
-                         [&](int a, int c) { return b.edges[a].post < b.edges[c].post; });
-        std::vector<int> incoming(nc + 1, 0);
-        for (int j = 0; j < sc; ++j) incoming[b.edges[j].post + 1]++;
-        for (int i = 0; i < nc; ++i) incoming[i + 1] += incoming[i];
-        for (int i = 0; i < nc; ++i) nin[n0 + i] = int32_t(s0 + incoming[i]);
-        for (int k = 0; k < sc; ++k) {
-            const int j = perm[k];
-            const ns_synapse_gene &g = b.edges[j];
+        for (int k = 0; k < cs; ++k) {
             const int64_t q = s0 + k;
-            const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
-            const float f[SG_FIELDS] = {g.initial_weight_ns, g.utilization, g.recovery_ms,
-                                        g.facilitation_ms, g.slow_recovery_ms, g.slow_depletion};
-            for (int m = 0; m < SG_FIELDS; ++m) sg[m * S + q] = f[m];
-            uint32_t route = uint32_t(g.branch & 3);
-            if (a.modulator) route |= uint32_t(a.modulator & 3) << R_MOD_SHIFT;
-            else if (a.inhibitory_transmitter) route |= R_INH;
-            if (g.active && a.active && z.active) route |= R_VALID;
-            if (!a.inhibitory_transmitter || a.modulator) route |= R_STP;
-            if (!a.inhibitory_transmitter && excitatory_cell(a.type) && !a.modulator &&
-                !z.inhibitory_transmitter && excitatory_cell(z.type))
-                route |= R_PAIR;
-            sflags[q] = uint32_t(g.delay) | (uint32_t(g.plasticity & 7) << SF_PLAST_SHIFT) |
-                        (uint32_t(g.mod_receptors & 7) << SF_MODREC_SHIFT) |
-                        (uint32_t(site_count(g)) << SF_SITES_SHIFT) | (route << SF_ROUTE_SHIFT);
-            spre[q] = int32_t(n0 + g.pre);
-            spost[q] = int32_t(n0 + g.post);
+            for (int m = 0; m < SG_FIELDS; ++m) sg[m * S + q] = img.sg[size_t(m) * cs + k];
+            sflags[q] = img.sflags[k];
+            spre[q] = img.spre[k];
+            spost[q] = img.spost[k];
             sbrain[q] = pb;
-            sorig[q] = b.orig_s.empty() ? uint32_t(j) : b.orig_s[j];
-            syn_perm[q] = j;
+            sorig[q] = img.sorig[k];
+            syn_perm[q] = img.syn_perm[k];
         }
     }
     nin[nn] = int32_t(ns);
     for (int64_t n = nn; n < N; ++n) nin[n + 1] = int32_t(ns);
+    h_nflags = nflags;
+    h_bout0 = bout0;
 
     // 6. Device arrays.
     Pool p{};
@@ -398,12 +490,12 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     if (slow_count) up(d_slow, slow_pos);
     view = p;
 
-    // 7. Large brains, packs (with type-sorted neuron slots) and per-tick ranges.
+    // 7. Large slots, packs (with type-sorted neuron slots) and per-tick ranges.
     std::vector<int32_t> big_list(std::max(nbig, 1), 0);
     for (int k = 0; k < nbig; ++k) big_list[k] = fallback_brains + k;
     d_big = arena.alloc<int32_t>(std::max(nbig, 1));
     up(d_big, big_list);
-    std::vector<uint16_t> perm(N, 0);
+    h_perm.assign(N, 0);
     packs.clear();
     int pb = fallback_brains + nbig;
     for (const Bin &bin : bins) {
@@ -415,28 +507,114 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
         pk.nn = bin.nn;
         pk.ns = bin.ns;
         pk.nm = bin.nb * outs;
-        // slot order: cortical AdEx (type 0), receptor cells (5), other AdEx
-        // (1, 4), LFS (6), then inactive cells
-        std::vector<std::pair<int, int>> key;
-        for (int j = 0; j < pk.nn; ++j) {
-            const uint32_t f = nflags[pk.n0 + j];
-            const uint32_t ty = f & NF_TYPE;
-            const int cls = !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
-            key.push_back({cls, j});
-        }
-        std::stable_sort(key.begin(), key.end());
-        for (int j = 0; j < pk.nn; ++j) perm[pk.n0 + j] = uint16_t(key[j].second);
+        pack_order(pk, &nflags[pk.n0], &h_perm[pk.n0]);
+        for (int k = 0; k < bin.nb; ++k) pack_of[pb + k] = int(packs.size());
         packs.push_back(pk);
         pb += bin.nb;
     }
     d_perm = arena.alloc<uint16_t>(N);
-    up(d_perm, perm);
+    up(d_perm, h_perm);
     d_packs = arena.alloc<Pack>(std::max<size_t>(packs.size(), 1));
     if (!packs.empty()) up(d_packs, packs);
     const int64_t fb_n = fallback_brains ? int64_t(bn0[fallback_brains - 1]) + bnc[fallback_brains - 1] : 0;
     const int64_t fb_s = fallback_brains ? int64_t(bs0[fallback_brains - 1]) + bsc[fallback_brains - 1] : 0;
     fallback_range = PlainRange{0, fb_n, 0, fb_s, nb, fallback_brains * outs, d_slow, slow_count};
     all_range = PlainRange{0, nn, 0, ns, nb, nb * outs, d_slow, slow_count};
+}
+
+// ---------------------------------------------------------------------------
+// Slot placement
+// ---------------------------------------------------------------------------
+namespace {
+// Collects (device range, host bytes) pairs into one staged buffer.
+struct Stager {
+    std::vector<ScatterEntry> entries;
+    std::vector<unsigned char> data;
+    void add(const void *dst, const void *src, size_t bytes) {
+        if (!bytes) return;
+        const size_t off = (data.size() + 15) & ~size_t(15);
+        data.resize(off + bytes);
+        std::memcpy(data.data() + off, src, bytes);
+        entries.push_back(ScatterEntry{uint64_t(reinterpret_cast<uintptr_t>(dst)), uint64_t(off),
+                                       uint64_t(bytes)});
+    }
+    // entries first (offsets rebased past them), then the data
+    size_t bytes() const { return head() + data.size(); }
+    size_t head() const { return (entries.size() * sizeof(ScatterEntry) + 15) & ~size_t(15); }
+    void write(unsigned char *out) const {
+        std::vector<ScatterEntry> e = entries;
+        for (auto &x : e) x.src += head();
+        if (!e.empty()) std::memcpy(out, e.data(), e.size() * sizeof(ScatterEntry));
+        if (!data.empty()) std::memcpy(out + head(), data.data(), data.size());
+    }
+};
+}  // namespace
+
+void DevicePool::scatter(PinnedStaging::Buf &buf, size_t bytes, int n, hipStream_t st) {
+    if (d_stage_cap < bytes) {
+        NS_HIP(hipStreamSynchronize(st));
+        stage_arena.release();
+        d_stage_cap = std::max(bytes, size_t(1) << 20);
+        d_stage = stage_arena.alloc<unsigned char>(int64_t(d_stage_cap));
+    }
+    NS_HIP(hipMemcpyAsync(d_stage, buf.p, bytes, hipMemcpyHostToDevice, st));
+    NS_HIP(hipEventRecord(buf.done, st));
+    launch_scatter(reinterpret_cast<const ScatterEntry *>(d_stage), n, d_stage, st);
+    NS_HIP(hipGetLastError());
+}
+
+void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &moves, hipStream_t st) {
+    if (moves.empty()) return;
+    const int64_t N = view.ncap, S = view.scap;
+    Stager sg;
+    std::vector<int> touched;
+    SlotImage img;
+    for (const auto &mv : moves) {
+        const int pb = mv.first;
+        const BrainSpec *spec = mv.second;
+        // per-tick-class slots may be vacated, not filled (their LP/Prinz
+        // neuron list is fixed at build)
+        if (pb < 0 || pb >= nb || (slot_kind[pb] == 0 && spec && spec->brain))
+            throw std::runtime_error("Slot cannot take a placement");
+        const int64_t n0 = bn0[pb], s0 = bs0[pb];
+        const int cn = bnc[pb], cs = bsc[pb];
+        const HostBrain *b = spec ? spec->brain : nullptr;
+        if (b && (int(b->genes.size()) > cn || int(b->edges.size()) > cs))
+            throw std::runtime_error("Brain exceeds its slot");
+        img.make(spec, n0, s0, cn, cs, outputs);
+        for (int k = 0; k < NG_FIELDS; ++k) sg.add(view.ng + k * N + n0, &img.ng[size_t(k) * cn], 4u * cn);
+        for (int k = 0; k < NX_FIELDS; ++k) sg.add(view.nx + k * N + n0, &img.nx[size_t(k) * cn], 4u * cn);
+        sg.add(view.nflags + n0, img.nflags.data(), 4u * cn);
+        sg.add(view.ninput + n0, img.ninput.data(), 4u * cn);
+        sg.add(view.nin + n0, img.nin.data(), 4u * cn);
+        sg.add(d_norig + n0, img.norig.data(), 4u * cn);
+        for (int m = 0; m < SG_FIELDS; ++m) sg.add(view.sg + m * S + s0, &img.sg[size_t(m) * cs], 4u * cs);
+        sg.add(view.sflags + s0, img.sflags.data(), 4u * cs);
+        sg.add(view.spre + s0, img.spre.data(), 4u * cs);
+        sg.add(view.spost + s0, img.spost.data(), 4u * cs);
+        sg.add(d_sorig + s0, img.sorig.data(), 4u * cs);
+        if (outputs) sg.add(view.motor + int64_t(pb) * outputs, img.motor.data(), 4u * outputs);
+        sg.add(view.bout0 + pb, &img.bout0, 4);
+        std::memcpy(&h_nflags[n0], img.nflags.data(), 4u * cn);
+        for (int k = 0; k < cs; ++k) syn_perm[s0 + k] = img.syn_perm[k];
+        h_bout0[pb] = img.bout0;
+        if (batch_of[pb] >= 0 && pool_of[batch_of[pb]] == pb) pool_of[batch_of[pb]] = -1;
+        occ_n[pb] = b ? int32_t(b->genes.size()) : 0;
+        occ_s[pb] = b ? int32_t(b->edges.size()) : 0;
+        batch_of[pb] = b ? spec->batch_index : -1;
+        if (b) pool_of[spec->batch_index] = pb;
+        if (slot_kind[pb] == 2) touched.push_back(pack_of[pb]);
+    }
+    std::sort(touched.begin(), touched.end());
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    for (int k : touched) {
+        const Pack &pk = packs[k];
+        pack_order(pk, &h_nflags[pk.n0], &h_perm[pk.n0]);
+        sg.add(d_perm + pk.n0, &h_perm[pk.n0], 2u * pk.nn);
+    }
+    PinnedStaging::Buf &buf = staging.take(sg.bytes());
+    sg.write(buf.p);
+    scatter(buf, sg.bytes(), int(sg.entries.size()), st);
 }
 
 // Neuron/synapse state in batch order, genome order within each brain.
@@ -460,20 +638,20 @@ void DevicePool::download_state(std::vector<ns_neuron_state> &out_n,
     down(ready, view.ready);
     if (stamps) down(st, view.stamps);
     // batch-order offsets
-    std::vector<int64_t> noff(nb + 1, 0), soff(nb + 1, 0);
-    for (int i = 0; i < nb; ++i) {
-        noff[i + 1] = noff[i] + bnc[pool_of[i]];
-        soff[i + 1] = soff[i] + bsc[pool_of[i]];
+    std::vector<int64_t> noff(nbatch + 1, 0), soff(nbatch + 1, 0);
+    for (int i = 0; i < nbatch; ++i) {
+        noff[i + 1] = noff[i] + occ_n[pool_of[i]];
+        soff[i + 1] = soff[i] + occ_s[pool_of[i]];
     }
-    out_n.assign(noff[nb], ns_neuron_state{});
-    out_s.assign(soff[nb], ns_synapse_state{});
-    hist.assign(noff[nb], 0);
-    ticks.assign(nb, 0);
-    if (stamps) stamps->assign(noff[nb] * 6, 0);
-    for (int i = 0; i < nb; ++i) {
+    out_n.assign(noff[nbatch], ns_neuron_state{});
+    out_s.assign(soff[nbatch], ns_synapse_state{});
+    hist.assign(noff[nbatch], 0);
+    ticks.assign(nbatch, 0);
+    if (stamps) stamps->assign(noff[nbatch] * 6, 0);
+    for (int i = 0; i < nbatch; ++i) {
         const int pb = pool_of[i];
         ticks[i] = btick[pb];
-        for (int k = 0; k < bnc[pb]; ++k) {
+        for (int k = 0; k < occ_n[pb]; ++k) {
             const int64_t n = int64_t(bn0[pb]) + k;
             ns_neuron_state &r = out_n[noff[i] + k];
             std::memset(&r, 0, sizeof r);
@@ -493,7 +671,7 @@ void DevicePool::download_state(std::vector<ns_neuron_state> &out_n,
             if (stamps)
                 for (int e = 0; e < 6; ++e) (*stamps)[(noff[i] + k) * 6 + e] = st[e * N + n];
         }
-        for (int k = 0; k < bsc[pb]; ++k) {
+        for (int k = 0; k < occ_s[pb]; ++k) {
             const int64_t q = int64_t(bs0[pb]) + k;
             ns_synapse_state &r = out_s[soff[i] + syn_perm[q]];
             std::memset(&r, 0, sizeof r);
@@ -528,18 +706,18 @@ void DevicePool::upload_state(const std::vector<ns_neuron_state> &in_n,
     std::vector<uint8_t> spike(N, 0), arrived(S, 0);
     std::vector<int64_t> last(S, 0), btick(std::max(nb, 1), 0);
     std::vector<uint64_t> ready(S, 0), st(6 * N, 0);
-    std::vector<int64_t> noff(nb + 1, 0), soff(nb + 1, 0);
-    for (int i = 0; i < nb; ++i) {
-        noff[i + 1] = noff[i] + bnc[pool_of[i]];
-        soff[i + 1] = soff[i] + bsc[pool_of[i]];
+    std::vector<int64_t> noff(nbatch + 1, 0), soff(nbatch + 1, 0);
+    for (int i = 0; i < nbatch; ++i) {
+        noff[i + 1] = noff[i] + occ_n[pool_of[i]];
+        soff[i + 1] = soff[i] + occ_s[pool_of[i]];
     }
-    if (int64_t(in_n.size()) != noff[nb] || int64_t(in_s.size()) != soff[nb] ||
-        int64_t(hist.size()) != noff[nb] || int(ticks.size()) != nb)
+    if (int64_t(in_n.size()) != noff[nbatch] || int64_t(in_s.size()) != soff[nbatch] ||
+        int64_t(hist.size()) != noff[nbatch] || int(ticks.size()) != nbatch)
         throw std::runtime_error("State dimensions differ");
-    for (int i = 0; i < nb; ++i) {
+    for (int i = 0; i < nbatch; ++i) {
         const int pb = pool_of[i];
         btick[pb] = ticks[i];
-        for (int k = 0; k < bnc[pb]; ++k) {
+        for (int k = 0; k < occ_n[pb]; ++k) {
             const int64_t n = int64_t(bn0[pb]) + k;
             const ns_neuron_state &r = in_n[noff[i] + k];
             fns[NS_V * N + n] = r.voltage_mv;
@@ -558,7 +736,7 @@ void DevicePool::upload_state(const std::vector<ns_neuron_state> &in_n,
             if (stamps)
                 for (int e = 0; e < 6; ++e) st[e * N + n] = (*stamps)[(noff[i] + k) * 6 + e];
         }
-        for (int k = 0; k < bsc[pb]; ++k) {
+        for (int k = 0; k < occ_s[pb]; ++k) {
             const int64_t q = int64_t(bs0[pb]) + k;
             const ns_synapse_state &r = in_s[soff[i] + syn_perm[q]];
             fss[SS_WEIGHT * S + q] = r.weight_ns;
@@ -622,11 +800,165 @@ void Batch::create(std::vector<HostBrain> &&owned, int) {
     specs.clear();
     for (int i = 0; i < count; ++i)
         specs.push_back(BrainSpec{&brains[i], i, i * channels, i * outputs, world});
-    pool.build(specs, outputs, default_tile(brains), stream);
+    pool.build(slot_specs(), outputs, default_tile(brains), stream);
     d_input = io.alloc<float>(int64_t(count) * channels);
     d_output = io.alloc<float>(int64_t(count) * outputs);
-    h_brun.assign(count, 0);
-    h_status.assign(count, 0);
+    last_mask.assign(count, 1);
+}
+
+// Capacity rungs ceil(16 * r^k), r = 1 + slack: brains of nearby sizes share
+// a rung, so a freed slot fits most brains of its size class.
+static int ladder(int n, float slack) {
+    if (!(slack > 0) || n <= 0) return n;
+    const double r = 1.0 + slack;
+    double c = 16.0;
+    while (std::ceil(c) < n) c *= r;
+    return int(std::ceil(c));
+}
+
+std::vector<BrainSpec> Batch::slot_specs() const {
+    std::vector<BrainSpec> out = specs;
+    for (auto &sp : out) {
+        sp.cap_n = ladder(int(sp.brain->genes.size()), slack);
+        sp.cap_s = ladder(int(sp.brain->edges.size()), slack);
+    }
+    if (spare > 0 && count > 0) {
+        // spare slots at evenly spaced quantiles of the slot sizes, plus two
+        // of the largest
+        std::vector<int> order(count);
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return std::make_pair(out[a].cap_n, out[a].cap_s) < std::make_pair(out[b].cap_n, out[b].cap_s);
+        });
+        const int nsp = std::max(1, int(std::ceil(double(spare) * count)));
+        std::vector<BrainSpec> extra;
+        for (int j = 0; j < nsp; ++j) {
+            const BrainSpec &q = out[order[std::min(count - 1, int((j + 0.5) * count / nsp))]];
+            extra.push_back(BrainSpec{nullptr, -1, 0, 0, world, q.cap_n, q.cap_s});
+        }
+        for (int j = 0; j < 2; ++j) {
+            const BrainSpec &q = out[order[count - 1]];
+            extra.push_back(BrainSpec{nullptr, -1, 0, 0, world, q.cap_n, q.cap_s});
+        }
+        out.insert(out.end(), extra.begin(), extra.end());
+    }
+    return out;
+}
+
+void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const uint32_t *seeds,
+                    hipStream_t st) {
+    if (k <= 0) return;
+    if (int(fresh.size()) != k) throw std::runtime_error("Replacement count differs");
+    std::vector<uint8_t> seen(count, 0);
+    for (int j = 0; j < k; ++j) {
+        if (idx[j] < 0 || idx[j] >= count || seen[idx[j]]) throw std::runtime_error("Invalid brain index");
+        seen[idx[j]] = 1;
+        if (fresh[j].channels != channels || int(fresh[j].motors.size()) != outputs)
+            throw std::runtime_error("Batch ports differ");
+    }
+    std::vector<int> old_slot(k), new_slot(k, -1);
+    for (int j = 0; j < k; ++j) old_slot[j] = pool.pool_of[idx[j]];
+    bool rebuild = false;
+    for (int j = 0; j < k && !rebuild; ++j)
+        if (has_slow_cell(fresh[j]) || (!pool.plastic && has_plastic_pair(fresh[j]))) rebuild = true;
+    if (!rebuild) {
+        // best fit by neuron capacity among free slots (spares, earlier
+        // vacancies and the replaced brains' own slots)
+        std::set<std::tuple<int, int, int>> freeset;
+        for (int pb = 0; pb < pool.nb; ++pb)
+            if (pool.batch_of[pb] < 0 && pool.slot_kind[pb] != 0)
+                freeset.insert({pool.bnc[pb], pool.bsc[pb], pb});
+        for (int j = 0; j < k; ++j)
+            if (pool.slot_kind[old_slot[j]] != 0)
+                freeset.insert({pool.bnc[old_slot[j]], pool.bsc[old_slot[j]], old_slot[j]});
+        for (int j = 0; j < k; ++j) {
+            const int n = int(fresh[j].genes.size()), e = int(fresh[j].edges.size());
+            auto it = freeset.lower_bound({n, e, -1});
+            while (it != freeset.end() && std::get<1>(*it) < e) ++it;
+            if (it == freeset.end()) { rebuild = true; break; }
+            new_slot[j] = std::get<2>(*it);
+            freeset.erase(it);
+        }
+    }
+    if (rebuild) {
+        // survivors keep their state; newborns start empty and are reset below
+        NS_HIP(hipStreamSynchronize(st));
+        std::vector<ns_neuron_state> on, nn_;
+        std::vector<ns_synapse_state> os, ns_;
+        std::vector<uint32_t> oh, nh;
+        std::vector<int64_t> ot;
+        std::vector<uint64_t> ost, nst;
+        pool.download_state(on, os, oh, ot, &ost);
+        std::vector<int64_t> noff(count + 1, 0), soff(count + 1, 0);
+        for (int i = 0; i < count; ++i) {
+            noff[i + 1] = noff[i] + int64_t(brains[i].genes.size());
+            soff[i + 1] = soff[i] + int64_t(brains[i].edges.size());
+        }
+        for (int j = 0; j < k; ++j) brains[idx[j]] = std::move(fresh[j]);
+        for (int i = 0; i < count; ++i) {
+            const int64_t n = int64_t(brains[i].genes.size()), e = int64_t(brains[i].edges.size());
+            if (seen[i]) {
+                nn_.insert(nn_.end(), size_t(n), ns_neuron_state{});
+                ns_.insert(ns_.end(), size_t(e), ns_synapse_state{});
+                nh.insert(nh.end(), size_t(n), 0u);
+                nst.insert(nst.end(), size_t(n) * 6, 0ull);
+                ot[i] = 0;
+            } else {
+                nn_.insert(nn_.end(), on.begin() + noff[i], on.begin() + noff[i + 1]);
+                ns_.insert(ns_.end(), os.begin() + soff[i], os.begin() + soff[i + 1]);
+                nh.insert(nh.end(), oh.begin() + noff[i], oh.begin() + noff[i + 1]);
+                nst.insert(nst.end(), ost.begin() + noff[i] * 6, ost.begin() + noff[i + 1] * 6);
+            }
+        }
+        pool.build(slot_specs(), outputs, pool.tile, stream);
+        pool.upload_state(nn_, ns_, nh, ot, &nst);
+        NS_HIP(hipStreamSynchronize(stream));
+        ++relayouts;
+    } else {
+        for (int j = 0; j < k; ++j) brains[idx[j]] = std::move(fresh[j]);
+        std::vector<std::pair<int, const BrainSpec *>> moves;
+        std::vector<uint8_t> reused(pool.nb, 0);
+        for (int j = 0; j < k; ++j) {
+            moves.push_back({new_slot[j], &specs[idx[j]]});
+            reused[new_slot[j]] = 1;
+        }
+        for (int j = 0; j < k; ++j)
+            if (!reused[old_slot[j]]) moves.push_back({old_slot[j], nullptr});
+        pool.place(moves, st);
+        placements += k;
+    }
+    // Fresh state for the newborns (initial states seeded as ns_gpu_reset),
+    // then the run flags of the last mask under the new slot mapping.
+    std::vector<uint8_t> run(pool.nb, 0), next(pool.nb, 0);
+    std::vector<uint32_t> bseed(pool.nb, 0);
+    for (int j = 0; j < k; ++j) {
+        const int pb = pool.pool_of[idx[j]];
+        run[pb] = uint8_t(1 | (pool.fallback[pb] ? 2 : 0));
+        bseed[pb] = seeds ? seeds[j] : 0u;
+    }
+    for (int pb = 0; pb < pool.nb; ++pb) {
+        const int i = pool.batch_of[pb];
+        next[pb] = uint8_t((i >= 0 && last_mask[i] ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
+    }
+    {
+        Stager sg;
+        sg.add(pool.view.brun, run.data(), size_t(pool.nb));
+        sg.add(pool.d_bseed, bseed.data(), 4u * pool.nb);
+        PinnedStaging::Buf &buf = pool.staging.take(sg.bytes());
+        sg.write(buf.p);
+        pool.scatter(buf, sg.bytes(), int(sg.entries.size()), st);
+    }
+    launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig, pool.d_sorig, st);
+    NS_HIP(hipGetLastError());
+    {
+        Stager sg;
+        sg.add(pool.view.brun, next.data(), size_t(pool.nb));
+        PinnedStaging::Buf &buf = pool.staging.take(sg.bytes());
+        sg.write(buf.p);
+        pool.scatter(buf, sg.bytes(), int(sg.entries.size()), st);
+    }
+    h_brun = next;
+    brun_valid = true;
 }
 
 void Batch::relayout(const Tile &tile) {
@@ -636,7 +968,7 @@ void Batch::relayout(const Tile &tile) {
     std::vector<int64_t> ticks;
     std::vector<uint64_t> stamps;
     pool.download_state(n, s, h, ticks, &stamps);
-    pool.build(specs, outputs, tile, stream);
+    pool.build(slot_specs(), outputs, tile, stream);
     pool.upload_state(n, s, h, ticks, &stamps);
     brun_valid = false;
 }
@@ -644,13 +976,16 @@ void Batch::relayout(const Tile &tile) {
 // Uploads the per-brain run flags only when they change; the upload waits
 // for earlier launches that may still read them.
 void Batch::set_run(const uint8_t *mask) {
-    std::vector<uint8_t> next(count);
-    for (int pb = 0; pb < count; ++pb)
-        next[pb] = uint8_t((mask[pool.batch_of[pb]] ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
+    last_mask.assign(mask, mask + count);
+    std::vector<uint8_t> next(pool.nb);
+    for (int pb = 0; pb < pool.nb; ++pb) {
+        const int i = pool.batch_of[pb];
+        next[pb] = uint8_t((i >= 0 && mask[i] ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
+    }
     if (brun_valid && next == h_brun) return;
     NS_HIP(hipDeviceSynchronize());
     h_brun = next;
-    NS_HIP(hipMemcpy(pool.view.brun, h_brun.data(), count, hipMemcpyHostToDevice));
+    NS_HIP(hipMemcpy(pool.view.brun, h_brun.data(), pool.nb, hipMemcpyHostToDevice));
     brun_valid = true;
 }
 
@@ -798,12 +1133,13 @@ void ns_gpu_destroy(void *p) { delete static_cast<Batch *>(p); }
 int ns_gpu_reset(void *p, const uint8_t *mask, const uint32_t *seeds) {
     try {
         auto &b = *static_cast<Batch *>(p);
-        std::vector<uint32_t> bseed(b.count);
-        for (int pb = 0; pb < b.count; ++pb) bseed[pb] = seeds[b.pool.batch_of[pb]];
+        std::vector<uint32_t> bseed(b.pool.nb, 0);
+        for (int pb = 0; pb < b.pool.nb; ++pb)
+            if (b.pool.batch_of[pb] >= 0) bseed[pb] = seeds[b.pool.batch_of[pb]];
         b.set_run(mask);
-        NS_HIP(hipMemcpyAsync(b.pool.d_bseed, bseed.data(), sizeof(uint32_t) * b.count,
+        NS_HIP(hipMemcpyAsync(b.pool.d_bseed, bseed.data(), sizeof(uint32_t) * b.pool.nb,
                               hipMemcpyHostToDevice, b.stream));
-        launch_reset(b.pool.view, b.pool.nn, b.pool.ns, b.count, b.pool.d_bseed, b.pool.d_norig,
+        launch_reset(b.pool.view, b.pool.nn, b.pool.ns, b.pool.nb, b.pool.d_bseed, b.pool.d_norig,
                      b.pool.d_sorig, b.stream);
         NS_HIP(hipGetLastError());
         NS_HIP(hipStreamSynchronize(b.stream));
@@ -834,10 +1170,12 @@ int ns_gpu_step(void *p, const float *input, const uint8_t *mask, int ticks, flo
         NS_HIP(hipGetLastError());
         NS_HIP(hipMemcpyAsync(output, b.d_output, sizeof(float) * b.count * b.outputs,
                               hipMemcpyDeviceToHost, b.stream));
-        NS_HIP(hipMemcpyAsync(b.h_status.data(), b.pool.view.bstatus, sizeof(int32_t) * b.count,
+        b.h_status.resize(b.pool.nb);
+        NS_HIP(hipMemcpyAsync(b.h_status.data(), b.pool.view.bstatus, sizeof(int32_t) * b.pool.nb,
                               hipMemcpyDeviceToHost, b.stream));
         NS_HIP(hipStreamSynchronize(b.stream));
-        for (int pb = 0; pb < b.count; ++pb) status[b.pool.batch_of[pb]] = b.h_status[pb];
+        for (int pb = 0; pb < b.pool.nb; ++pb)
+            if (b.pool.batch_of[pb] >= 0) status[b.pool.batch_of[pb]] = b.h_status[pb];
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -863,12 +1201,49 @@ int ns_gpu_step_device(void *p, const float *input, const uint8_t *mask, int tic
         NS_HIP(hipGetLastError());
         if (status) {
             NS_HIP(hipStreamSynchronize(st));
-            NS_HIP(hipMemcpy(b.h_status.data(), b.pool.view.bstatus, sizeof(int32_t) * b.count,
+            b.h_status.resize(b.pool.nb);
+            NS_HIP(hipMemcpy(b.h_status.data(), b.pool.view.bstatus, sizeof(int32_t) * b.pool.nb,
                              hipMemcpyDeviceToHost));
-            std::vector<int32_t> s(b.count);
-            for (int pb = 0; pb < b.count; ++pb) s[b.pool.batch_of[pb]] = b.h_status[pb];
+            std::vector<int32_t> s(b.count, 0);
+            for (int pb = 0; pb < b.pool.nb; ++pb)
+                if (b.pool.batch_of[pb] >= 0) s[b.pool.batch_of[pb]] = b.h_status[pb];
             NS_HIP(hipMemcpy(status, s.data(), sizeof(int32_t) * b.count, hipMemcpyHostToDevice));
         }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_reserve(void *p, float slack, float spare) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        if (!(slack >= 0 && slack <= 4) || !(spare >= 0 && spare <= 4))
+            throw std::runtime_error("Invalid slot reserve");
+        b.slack = slack;
+        b.spare = spare;
+        b.relayout(b.pool.tile);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_replace(void *p, int count, const int *indices, void **handles, const uint32_t *seeds,
+                   void *stream) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        if (count < 0 || (count && (!indices || !handles))) throw std::runtime_error("Invalid replacement");
+        std::vector<HostBrain> fresh;
+        fresh.reserve(count);
+        for (int j = 0; j < count; ++j) {
+            if (!handles[j]) throw std::runtime_error("Invalid genome handle");
+            fresh.push_back(*static_cast<HostBrain *>(handles[j]));
+        }
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b.stream;
+        b.replace(count, indices, std::move(fresh), seeds, st);
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -946,7 +1321,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
         for (int k = 0; k < kNumTiles; ++k) {
             Tile t{kTiles[k].threads, kTiles[k].neurons_per_thread, kTiles[k].synapses_per_thread,
                    kTiles[k].min_waves, kTiles[k].synapses_per_thread > 0 ? 0 : kTiles[k].lds_plastic};
-            b.pool.build(b.specs, b.outputs, t, b.stream);
+            b.pool.build(b.slot_specs(), b.outputs, t, b.stream);
             b.pool.upload_state(n, s, h, tk, &stamps);
             b.brun_valid = false;
             b.set_run(ones.data());
@@ -964,7 +1339,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        b.pool.build(b.specs, b.outputs, best_tile, b.stream);
+        b.pool.build(b.slot_specs(), b.outputs, best_tile, b.stream);
         b.pool.upload_state(n, s, h, tk, &stamps);
         b.brun_valid = false;
         NS_HIP(hipMemcpy(b.pool.view.events, events0, sizeof events0, hipMemcpyHostToDevice));
@@ -996,7 +1371,7 @@ int ns_gpu_export(void *p, ns_neuron_state *neurons, ns_synapse_state *synapses,
             int64_t off = 0;
             for (int i = 0; i < b.count; ++i) {
                 const int64_t T = tk[i];
-                const int nc = b.pool.bnc[b.pool.pool_of[i]];
+                const int nc = b.pool.occ_n[b.pool.pool_of[i]];
                 for (int k = 0; k < nc; ++k) {
                     const uint32_t bits = h[off + k];
                     for (int slot = 0; slot < 9; ++slot) {
@@ -1032,7 +1407,7 @@ int ns_gpu_import(void *p, const ns_neuron_state *neurons, const ns_synapse_stat
             int64_t off = 0;
             for (int i = 0; i < b.count; ++i) {
                 const int64_t T = tk[i];
-                const int nc = b.pool.bnc[b.pool.pool_of[i]];
+                const int nc = b.pool.occ_n[b.pool.pool_of[i]];
                 for (int k = 0; k < nc; ++k) {
                     uint32_t bits = 0;
                     for (int j = 0; j < 9; ++j) {
@@ -1084,6 +1459,14 @@ int ns_gpu_info(void *p, ns_batch_info *info) {
         info->fallback_neurons = b.pool.fallback_range.n_hi - b.pool.fallback_range.n_lo;
         info->fallback_synapses = b.pool.fallback_range.s_hi - b.pool.fallback_range.s_lo;
         info->big_brains = b.pool.nbig;
+        info->slots = b.pool.nb;
+        int64_t free_slots = 0;
+        for (int pb = 0; pb < b.pool.nb; ++pb) free_slots += b.pool.batch_of[pb] < 0;
+        info->free_slots = free_slots;
+        info->slot_neurons = b.pool.nn;
+        info->slot_synapses = b.pool.ns;
+        info->placements = b.placements;
+        info->relayouts = b.relayouts;
         int64_t big_n = 0;
         for (int k = 0; k < b.pool.nbig; ++k) big_n += b.pool.bnc[b.pool.fallback_brains + k];
         info->big_neurons = big_n;

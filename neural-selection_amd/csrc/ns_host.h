@@ -58,13 +58,43 @@ struct DeviceArena {
     ~DeviceArena() { release(); }
 };
 
-// One brain as the pool builder sees it.
+// One pool slot as the builder sees it: a brain, or no brain (a spare slot),
+// with an optional capacity larger than the brain.
 struct BrainSpec {
-    const HostBrain *brain;
-    int batch_index;     // input / output row
-    int input_base;      // absolute input index of channel 0
-    int output_base;     // absolute output index of motor 0
-    bool world;          // src/main.cu neuron_step rules
+    const HostBrain *brain;   // nullptr: an empty slot
+    int batch_index;          // input / output row (-1 for an empty slot)
+    int input_base;           // absolute input index of channel 0
+    int output_base;          // absolute output index of motor 0
+    bool world;               // src/main.cu neuron_step rules
+    int cap_n = 0, cap_s = 0; // slot capacity; at least the brain's size
+};
+
+// Pinned host staging for slot placements, double buffered so one buffer can
+// be filled while the other's copy is still in flight.
+struct PinnedStaging {
+    struct Buf { unsigned char *p = nullptr; size_t cap = 0; hipEvent_t done = nullptr; };
+    Buf bufs[2];
+    int next = 0;
+    Buf &take(size_t bytes) {
+        Buf &b = bufs[next];
+        next ^= 1;
+        if (b.done) NS_HIP(hipEventSynchronize(b.done));
+        if (b.cap < bytes) {
+            if (b.p) (void)hipHostFree(b.p);
+            b.p = nullptr;
+            b.cap = 0;
+            NS_HIP(hipHostMalloc(&b.p, bytes));
+            b.cap = bytes;
+        }
+        if (!b.done) NS_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+        return b;
+    }
+    ~PinnedStaging() {
+        for (auto &b : bufs) {
+            if (b.done) { (void)hipEventSynchronize(b.done); (void)hipEventDestroy(b.done); }
+            if (b.p) (void)hipHostFree(b.p);
+        }
+    }
 };
 
 // Tile of the resident launch: ks > 0 selects the register-slot kernel, ks == 0
@@ -97,7 +127,7 @@ public:
     int nb = 0, outputs = 0;
     Tile tile;
     // per pool brain
-    std::vector<int32_t> bn0, bnc, bs0, bsc, batch_of;
+    std::vector<int32_t> bn0, bnc, bs0, bsc, batch_of;   // bnc/bsc: slot capacities
     std::vector<uint8_t> fallback;   // brun bit 1
     std::vector<int32_t> pool_of;    // batch index -> pool brain
     // genome order -> pool position, per pool brain (offsets bs0)
@@ -110,6 +140,25 @@ public:
     bool plastic = false;            // any valid plastic excitatory pair
     int smax_lds = 0;                // LDS synapse capacity per pack (LDS tiles)
     int slow_count = 0, fallback_brains = 0, nbig = 0;
+    // Slots.  Every pool brain index is a slot; its capacity (bnc, bsc) may
+    // exceed its occupant (occ_n, occ_s; 0 = free, batch_of = -1).  Neurons
+    // past the occupant are inactive and synapses past it invalid.
+    int nbatch = 0;                  // batch brains (pool_of's size)
+    std::vector<int32_t> occ_n, occ_s, pack_of, h_bout0;
+    std::vector<uint8_t> slot_kind;  // 0 per-tick class, 1 own workgroup, 2 pack
+    std::vector<uint32_t> h_nflags;  // host copies behind pack slot orders
+    std::vector<uint16_t> h_perm;
+    PinnedStaging staging;
+    DeviceArena stage_arena;         // device copy of a staged placement
+    unsigned char *d_stage = nullptr;
+    size_t d_stage_cap = 0;
+    // Copy `bytes` of pinned host data to the device staging buffer on `st`
+    // and scatter its `n` entries (the first n * sizeof(ScatterEntry) bytes).
+    void scatter(PinnedStaging::Buf &buf, size_t bytes, int n, hipStream_t st);
+    // Write the images of `moves` (slot, occupant or nullptr) into the pool on
+    // stream `st`: genes, flags, routes, endpoints, motors and the affected
+    // packs' slot orders.  State is left alone (the caller resets newborns).
+    void place(const std::vector<std::pair<int, const BrainSpec *>> &moves, hipStream_t st);
     uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
     hipStream_t stream = nullptr;
 };
@@ -168,15 +217,29 @@ struct Batch {
     uint8_t *d_run_host_order = nullptr;
     DeviceArena io;
     hipStream_t stream = nullptr;
-    std::vector<uint8_t> h_brun;
+    std::vector<uint8_t> h_brun;                       // per slot
     bool brun_valid = false;
     std::vector<int32_t> h_status;
     bool timing = false;
     KernelTimer timer;
 
+    // Slot reserve for brain replacement: capacities rounded up a geometric
+    // ladder of ratio 1 + slack, plus spare empty slots (a fraction `spare`
+    // of the batch) spread over the brain-size distribution.
+    float slack = 0.0f, spare = 0.0f;
+    int64_t relayouts = 0, placements = 0;
+    std::vector<uint8_t> last_mask;                    // batch order, last run mask
+
     void create(std::vector<HostBrain> &&owned, int max_delay_unused);
     void relayout(const Tile &tile);
     void set_run(const uint8_t *mask);                 // host mask, batch order
+    // Replace brains idx[0..k) by new genomes with fresh state (seeds as
+    // ns_gpu_reset): each goes to the best-fitting free slot and the pool is
+    // rebuilt only when none fits.  Stream-ordered on `st`.
+    void replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const uint32_t *seeds,
+                 hipStream_t st);
+    // Pool slots for the current brains under the reserve policy.
+    std::vector<BrainSpec> slot_specs() const;
     void run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
              hipStream_t st);
     ~Batch();

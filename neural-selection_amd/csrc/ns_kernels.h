@@ -48,6 +48,12 @@ extern const int kNumTiles;
 
 void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t *bseed,
                   const uint32_t *norig, const uint32_t *sorig, hipStream_t st);
+// One staged copy of a slot placement: `bytes` bytes from offset `src` of the
+// staged buffer to device address `dst`.
+struct ScatterEntry {
+    uint64_t dst, src, bytes;
+};
+void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st);
 void launch_fill(float *x, int64_t n, float v, hipStream_t st);
 void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st);
 void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,

@@ -94,6 +94,24 @@ __global__ void k_reset_synapses(Pool p, int64_t n, const uint32_t *bseed, const
     p.released[q] = 0;
 }
 
+// Slot placement: each entry copies `bytes` from the staged image (device
+// copy of the pinned staging buffer) to one pool array range.  One workgroup
+// per entry; 4-byte words when both ends and the length allow it.
+__global__ void k_scatter(const ScatterEntry *entries, int n, const unsigned char *base) {
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    const ScatterEntry en = entries[e];
+    unsigned char *dst = reinterpret_cast<unsigned char *>(en.dst);
+    const unsigned char *src = base + en.src;
+    if (((en.dst | en.src | en.bytes) & 3u) == 0) {
+        uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+        const uint32_t *s4 = reinterpret_cast<const uint32_t *>(src);
+        for (uint64_t i = threadIdx.x; i < en.bytes / 4; i += blockDim.x) d[i] = s4[i];
+    } else {
+        for (uint64_t i = threadIdx.x; i < en.bytes; i += blockDim.x) dst[i] = src[i];
+    }
+}
+
 __global__ void k_set_ticks(Pool p, int nb, int64_t add, int64_t set, uint8_t need) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !need_ok(p, b, need)) return;
@@ -663,7 +681,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
     }
     for (int m = tid; m < pk.nm; m += WG) {
         const int b = m / p.outputs;
-        out[p.bout0[pk.b0 + b] + m % p.outputs] = brun_l[b] ? float(cnt_l[m]) : 0.0f;
+        // outputs start at zero (run_pool clears them); a brain that did not
+        // run may share its output row with a slot's new occupant
+        if (brun_l[b]) out[p.bout0[pk.b0 + b] + m % p.outputs] = float(cnt_l[m]);
     }
     for (int b = tid; b < pk.nb; b += WG)
         if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
@@ -962,7 +982,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     }
     for (int m = tid; m < pk.nm; m += WG) {
         const int b = m / p.outputs;
-        out[p.bout0[pk.b0 + b] + m % p.outputs] = brun_l[b] ? float(cnt_l[m]) : 0.0f;
+        // outputs start at zero (run_pool clears them); a brain that did not
+        // run may share its output row with a slot's new occupant
+        if (brun_l[b]) out[p.bout0[pk.b0 + b] + m % p.outputs] = float(cnt_l[m]);
     }
     for (int b = tid; b < pk.nb; b += WG)
         if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
@@ -1004,6 +1026,10 @@ void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t 
     if (nn) hipLaunchKernelGGL(k_reset_neurons, dim3(blocks_for(nn, 256)), dim3(256), 0, st, p, nn, bseed, norig);
     if (ns) hipLaunchKernelGGL(k_reset_synapses, dim3(blocks_for(ns, 256)), dim3(256), 0, st, p, ns, bseed, sorig);
     hipLaunchKernelGGL(k_set_ticks, dim3(blocks_for(nb, 256)), dim3(256), 0, st, p, nb, int64_t(0), int64_t(0), uint8_t(1));
+}
+
+void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_scatter, dim3(n), dim3(256), 0, st, entries, n, base);
 }
 
 void launch_fill(float *x, int64_t n, float v, hipStream_t st) {

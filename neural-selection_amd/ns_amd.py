@@ -37,7 +37,10 @@ class BatchInfo(ct.Structure):
                 ('per_tick_calls', ct.c_int64), ('resident_events', ct.c_int64),
                 ('fallback_neurons', ct.c_int64), ('fallback_synapses', ct.c_int64),
                 ('big_brains', ct.c_int64), ('big_neurons', ct.c_int64),
-                ('min_waves', ct.c_int), ('lds_synapses', ct.c_int)]
+                ('min_waves', ct.c_int), ('lds_synapses', ct.c_int),
+                ('slots', ct.c_int64), ('free_slots', ct.c_int64), ('slot_neurons', ct.c_int64),
+                ('slot_synapses', ct.c_int64), ('placements', ct.c_int64),
+                ('relayouts', ct.c_int64)]
 
 
 class ServeConfig(ct.Structure):
@@ -81,6 +84,8 @@ SIGNATURES = {
     'ns_gpu_export_spike_bits': ([_vp, _u32], ct.c_int),
     'ns_gpu_info': ([_vp, ct.POINTER(BatchInfo)], ct.c_int),
     'ns_gpu_set_timing': ([_vp, ct.c_int], ct.c_int),
+    'ns_gpu_reserve': ([_vp, ct.c_float, ct.c_float], ct.c_int),
+    'ns_gpu_replace': ([_vp, ct.c_int, _i32, ct.POINTER(_vp), _u32, _vp], ct.c_int),
     'ns_device_math_digest': ([ct.c_int, ct.c_uint64, ct.c_uint64, ct.c_int64,
                                ct.POINTER(ct.c_uint64)], ct.c_int),
     'ns_pool_create': ([ct.c_int, ct.c_int, ct.c_int], _vp),
@@ -134,6 +139,37 @@ def device_math_digest(fn, lo, hi, chunk):
     if rc:
         raise RuntimeError(error())
     return out
+
+
+class GenomeHandles:
+    """ns_create handles for a list of gym-encoded genomes (neuron rows,
+    synapse rows, motors), kept for ns_gpu_replace."""
+
+    def __init__(self, genomes, channels, max_delay=8):
+        L = lib()
+        self.lib = L
+        self.ptrs = []
+        self.counts, self.edges = [], []
+        try:
+            for nr, sr, mo in genomes:
+                nr = np.ascontiguousarray(nr, np.float32)
+                sr = np.ascontiguousarray(np.asarray(sr, np.float32).reshape(-1, 11))
+                mo = np.ascontiguousarray(mo, np.int32)
+                h = L.ns_create_ex(len(nr), len(sr), channels, len(mo), _ptr(nr, _f), _ptr(sr, _f),
+                                   _ptr(mo, _i32), max_delay)
+                if not h:
+                    raise RuntimeError(error())
+                self.ptrs.append(h)
+                self.counts.append(len(nr))
+                self.edges.append(len(sr))
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        for h in self.ptrs:
+            self.lib.ns_destroy(h)
+        self.ptrs = []
 
 
 class GpuBatch:
@@ -237,6 +273,23 @@ class GpuBatch:
 
     def set_timing(self, on=True):
         self._check(self.lib.ns_gpu_set_timing(self.ptr, int(on)))
+
+    def reserve(self, slack=0.19, spare=0.03):
+        """Slot reserve for replacements (ns_gpu_reserve); state is kept."""
+        self._check(self.lib.ns_gpu_reserve(self.ptr, slack, spare))
+
+    def replace(self, indices, handles, which, seeds, stream=None):
+        """Brains `indices` take genomes handles.ptrs[which[j]] with fresh state
+        seeded by seeds[j] (ns_gpu_replace), stream-ordered on `stream`."""
+        idx = np.ascontiguousarray(indices, np.int32)
+        which = list(which)
+        arr = (_vp * len(idx))(*[handles.ptrs[w] for w in which])
+        sd = np.ascontiguousarray(seeds, np.uint32)
+        self._check(self.lib.ns_gpu_replace(self.ptr, len(idx), _ptr(idx, _i32), arr, _ptr(sd, _u32),
+                                            stream))
+        for i, w in zip(idx, which):
+            self.neuron_counts[i] = handles.counts[w]
+            self.edge_counts[i] = handles.edges[w]
 
     def info(self):
         i = BatchInfo()
