@@ -50,15 +50,18 @@ def make_workload(name):
     raise SystemExit(f'unknown workload {name}')
 
 
-def describe(name, w):
+def describe(name, w, churn=0.0):
     base = {'workload': name, 'brains': w.count, 'neurons': w.total_neurons,
             'synapses': w.total_synapses, 'ticks_per_step': w.ticks, 'max_delay': w.max_delay}
     if name == 'c3':
         base.update(desc='capacity scale with plasticity: 32768 brains, Pareto(1.5, 32) sizes '
                          'truncated at 4096 and rescaled to 4194304 neurons, 2 synapses/neuron, '
                          'STDP profiles 1/2 with eligibility (3), D1 dopamine (4) and spine (5) '
-                         'profiles, modulator neurons; births/deaths churn not included',
-                    plasticity=True)
+                         'profiles, modulator neurons; '
+                         + (f'births/deaths: {churn:.0%} of brains replaced by newborn genomes '
+                            'every step (best-fit free slots, fresh state), inside the timed steps'
+                            if churn > 0 else 'no births/deaths'),
+                    plasticity=True, churn_per_step=churn)
     elif name == 'c2':
         base.update(desc='2048 brains, lognormal(ln 96, 0.75) sizes clipped 16..1024, 20 '
                          'synapses/neuron, delays 1..16, STP on, plasticity off; the world '
@@ -211,6 +214,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-autotune', action='store_true')
     ap.add_argument('--tiles', default='')
+    ap.add_argument('--churn', type=float, default=None,
+                    help='fraction of brains replaced by newborns every step (c3 default 0.01)')
     ap.add_argument('--rates', default='100,1000,5000', help='c4: Poisson arrival rates (1/s)')
     ap.add_argument('--episodes', type=int, default=300, help='c4: episodes per rate')
     ap.add_argument('--episode-steps', type=int, default=500, help='c4: env steps per episode')
@@ -242,6 +247,7 @@ def main():
     w, plastic = make_workload(args.workload)
     log(f'{w.count} brains, {w.total_neurons} neurons, {w.total_synapses} synapses '
         f'({time.perf_counter() - t0:.1f} s)')
+    churn = args.churn if args.churn is not None else (0.01 if args.workload == 'c3' else 0.0)
     batch = ns_amd.GpuBatch(workload=w)
     batch.reset(w.seeds())
     tile_key = None          # the tile as set_tiles names it
@@ -252,6 +258,25 @@ def main():
         log('autotuning tile shape')
         tile_key = tuple(batch.autotune(w.ticks))
         log(f'tile {tile_key}')
+    births = None
+    if churn > 0:
+        import ctypes as ct
+        t1 = time.perf_counter()
+        batch.reserve(0.10, 0.03)
+        newborn = W.c3_newborns(count=2048)
+        births = ns_amd.GenomeHandles([newborn.brain(i) for i in range(newborn.count)], w.channels,
+                                      max_delay=w.max_delay)
+        per_step = max(1, int(round(churn * w.count)))
+        rng = np.random.default_rng(77)
+        plans = []
+        for k in range(args.warmup + args.steps):
+            idx = np.ascontiguousarray(rng.choice(w.count, per_step, replace=False), np.int32)
+            which = rng.integers(0, newborn.count, per_step)
+            arr = (ct.c_void_p * per_step)(*[births.ptrs[j] for j in which])
+            seeds = np.ascontiguousarray(rng.integers(1, 2**32 - 1, per_step), np.uint32)
+            plans.append((idx, arr, seeds))
+        log(f'churn: {per_step} brains replaced per step; slot reserve built in '
+            f'{time.perf_counter() - t1:.1f} s')
     info = batch.info()
     log(f'packs {info["resident_packs"]} fallback brains {info["fallback_brains"]} '
         f'({info["fallback_neurons"]} neurons) device {info["device_bytes"] / 2**30:.2f} GiB')
@@ -264,6 +289,14 @@ def main():
     stream = torch.cuda.Stream()
 
     def step(k):
+        if births is not None:
+            import ctypes as ct
+            idx, arr, seeds = plans[k]
+            rc = batch.lib.ns_gpu_replace(batch.ptr, len(idx), idx.ctypes.data_as(ct.POINTER(ct.c_int)),
+                                          arr, seeds.ctypes.data_as(ct.POINTER(ct.c_uint32)),
+                                          stream.cuda_stream)
+            if rc:
+                raise RuntimeError(ns_amd.error())
         batch.step_device(inputs[k % patterns].data_ptr(), w.ticks, w.gain, out.data_ptr(),
                           stream=stream.cuda_stream)
 
@@ -344,7 +377,7 @@ def main():
         'vs_baseline': None,
         'dtype': 'f32',
         'data': 'synthetic (seeded genomes and Poisson sensory input from ns_workloads.py)',
-        'config': describe(args.workload, w),
+        'config': describe(args.workload, w, churn),
         'synaptic_events_per_s': ev / wall,
         'synapse_updates_per_s': su / wall,
         'step_latency_ms': {'p50': float(np.percentile(step_ms, 50)),
@@ -369,8 +402,15 @@ def main():
                      'launch_ms': res_ms, 'algorithmic_bytes_per_launch': alg_bytes},
         'cpu_baseline': cpu,
     }
+    if births is not None:
+        line['churn'] = {'fraction_per_step': churn, 'brains_per_step': len(plans[0][0]),
+                         'placements': i1['placements'], 'relayouts': i1['relayouts'],
+                         'slots': i1['slots'], 'free_slots': i1['free_slots'],
+                         'slot_neurons': i1['slot_neurons'], 'slot_synapses': i1['slot_synapses']}
     print(json.dumps(line), flush=True)
     batch.close()
+    if births is not None:
+        births.close()
 
 
 if __name__ == '__main__':

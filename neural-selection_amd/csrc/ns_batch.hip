@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <set>
 #include <tuple>
 
@@ -265,16 +266,14 @@ struct SlotImage {
 // (5), other AdEx (1, 4), LFS (6), then inactive cells): waves of one type
 // take the same branches.
 static void pack_order(const Pack &pk, const uint32_t *nflags, uint16_t *perm) {
-    std::vector<std::pair<int, int>> key;
-    key.reserve(pk.nn);
-    for (int j = 0; j < pk.nn; ++j) {
-        const uint32_t f = nflags[j];
+    auto cls = [](uint32_t f) {
         const uint32_t ty = f & NF_TYPE;
-        const int cls = !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
-        key.push_back({cls, j});
-    }
-    std::stable_sort(key.begin(), key.end());
-    for (int j = 0; j < pk.nn; ++j) perm[j] = uint16_t(key[j].second);
+        return !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
+    };
+    int start[6] = {0, 0, 0, 0, 0, 0};     // stable counting sort by class
+    for (int j = 0; j < pk.nn; ++j) start[cls(nflags[j]) + 1]++;
+    for (int c = 0; c < 5; ++c) start[c + 1] += start[c];
+    for (int j = 0; j < pk.nn; ++j) perm[start[cls(nflags[j])]++] = uint16_t(j);
 }
 
 void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile &t,
@@ -525,50 +524,120 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
 // ---------------------------------------------------------------------------
 // Slot placement
 // ---------------------------------------------------------------------------
-namespace {
-// Collects (device range, host bytes) pairs into one staged buffer.
-struct Stager {
-    std::vector<ScatterEntry> entries;
-    std::vector<unsigned char> data;
-    void add(const void *dst, const void *src, size_t bytes) {
-        if (!bytes) return;
-        const size_t off = (data.size() + 15) & ~size_t(15);
-        data.resize(off + bytes);
-        std::memcpy(data.data() + off, src, bytes);
-        entries.push_back(ScatterEntry{uint64_t(reinterpret_cast<uintptr_t>(dst)), uint64_t(off),
-                                       uint64_t(bytes)});
-    }
-    // entries first (offsets rebased past them), then the data
-    size_t bytes() const { return head() + data.size(); }
-    size_t head() const { return (entries.size() * sizeof(ScatterEntry) + 15) & ~size_t(15); }
-    void write(unsigned char *out) const {
-        std::vector<ScatterEntry> e = entries;
-        for (auto &x : e) x.src += head();
-        if (!e.empty()) std::memcpy(out, e.data(), e.size() * sizeof(ScatterEntry));
-        if (!data.empty()) std::memcpy(out + head(), data.data(), data.size());
-    }
-};
-}  // namespace
 
-void DevicePool::scatter(PinnedStaging::Buf &buf, size_t bytes, int n, hipStream_t st) {
+// Write one newborn's slot image into its reserved chunks (thread-safe across
+// different slots: every write goes to the slot's own ranges).
+static void fill_slot(DevicePool &d, const PlaceChunks &c, unsigned char *base) {
+    const HostBrain *b = c.spec->brain;
+    const int pb = c.pb;
+    const int64_t n0 = d.bn0[pb], s0 = d.bs0[pb];
+    const int cn = d.bnc[pb], cs = d.bsc[pb];
+    const int nc = int(b->genes.size()), sc = int(b->edges.size());
+    for (int k = 0; k < NG_FIELDS; ++k) {
+        float *f = reinterpret_cast<float *>(base + c.ng[k]);
+        for (int i = 0; i < nc; ++i) {
+            const ns_neuron_gene &g = b->genes[i];
+            const float v[NG_FIELDS] = {g.capacitance_pf, g.leak_ns, g.rest_mv, g.threshold_mv,
+                                        g.slope_mv, g.adapt_ns, g.adapt_tau_ms, g.spike_adapt_pa,
+                                        g.reset_mv, g.afterdepol_spike_pa, g.afterdepol_tau_ms,
+                                        g.receptor_baseline_hz, g.receptor_gain_hz,
+                                        g.receptor_event_ns, g.receptor_decay_ms};
+            f[i] = v[k];
+        }
+    }
+    std::memcpy(base + c.fl, &d.h_nflags[n0], 4u * cn);   // flags set by place()
+    int32_t *inp = reinterpret_cast<int32_t *>(base + c.inp);
+    for (int i = 0; i < nc; ++i) inp[i] = b->sensory[i] >= 0 ? c.spec->input_base + b->sensory[i] : -1;
+    uint32_t *no = reinterpret_cast<uint32_t *>(base + c.no);
+    for (int i = 0; i < nc; ++i) no[i] = b->orig_n.empty() ? uint32_t(i) : b->orig_n[i];
+    // synapses by (post, edge index): a stable counting sort by post
+    std::vector<int> start(nc + 1, 0), order(sc);
+    for (int j = 0; j < sc; ++j) start[b->edges[j].post + 1]++;
+    for (int i = 0; i < nc; ++i) start[i + 1] += start[i];
+    int32_t *ni = reinterpret_cast<int32_t *>(base + c.ni);
+    for (int i = 0; i < c.ninx; ++i) ni[i] = int32_t(s0 + start[i]);   // the first tail neuron bounds the last run
+    for (int j = 0; j < sc; ++j) order[start[b->edges[j].post]++] = j;
+    for (int m = 0; m < SG_FIELDS; ++m) {
+        float *f = reinterpret_cast<float *>(base + c.sg[m]);
+        for (int k = 0; k < sc; ++k) {
+            const ns_synapse_gene &g = b->edges[order[k]];
+            const float v[SG_FIELDS] = {g.initial_weight_ns, g.utilization, g.recovery_ms,
+                                        g.facilitation_ms, g.slow_recovery_ms, g.slow_depletion};
+            f[k] = v[m];
+        }
+    }
+    uint32_t *sf = reinterpret_cast<uint32_t *>(base + c.sf);
+    for (int k = 0; k < cs; ++k) {
+        if (k >= sc) { sf[k] = 1u; d.syn_perm[s0 + k] = -1; continue; }   // invalid tail, delay 1
+        const int j = order[k];
+        const ns_synapse_gene &g = b->edges[j];
+        const ns_neuron_gene &a = b->genes[g.pre], &z = b->genes[g.post];
+        uint32_t route = uint32_t(g.branch & 3);
+        if (a.modulator) route |= uint32_t(a.modulator & 3) << R_MOD_SHIFT;
+        else if (a.inhibitory_transmitter) route |= R_INH;
+        if (g.active && a.active && z.active) route |= R_VALID;
+        if (!a.inhibitory_transmitter || a.modulator) route |= R_STP;
+        if (!a.inhibitory_transmitter && excitatory_cell(a.type) && !a.modulator &&
+            !z.inhibitory_transmitter && excitatory_cell(z.type))
+            route |= R_PAIR;
+        sf[k] = uint32_t(g.delay) | (uint32_t(g.plasticity & 7) << SF_PLAST_SHIFT) |
+                (uint32_t(g.mod_receptors & 7) << SF_MODREC_SHIFT) |
+                (uint32_t(site_count(g)) << SF_SITES_SHIFT) | (route << SF_ROUTE_SHIFT);
+        d.syn_perm[s0 + k] = j;
+    }
+    int32_t *pre = reinterpret_cast<int32_t *>(base + c.pre);
+    int32_t *post = reinterpret_cast<int32_t *>(base + c.post);
+    uint32_t *so = reinterpret_cast<uint32_t *>(base + c.so);
+    for (int k = 0; k < sc; ++k) {
+        const ns_synapse_gene &g = b->edges[order[k]];
+        pre[k] = int32_t(n0 + g.pre);
+        post[k] = int32_t(n0 + g.post);
+        so[k] = b->orig_s.empty() ? uint32_t(order[k]) : b->orig_s[order[k]];
+    }
+    int32_t *mo = reinterpret_cast<int32_t *>(base + c.mo);
+    for (int j = 0; j < d.outputs; ++j) mo[j] = b->motors[j] >= 0 ? int32_t(n0 + b->motors[j]) : -1;
+}
+
+void DevicePool::upload_stage(const Stager &sg, hipStream_t st) {
+    const size_t bytes = sg.bytes();
+    PinnedStaging::Buf &buf = staging.take(bytes);
+    unsigned char *base = sg.write(buf.p);
+    // slot images, in parallel over newborns (disjoint slots)
+    const int nthreads = int(std::min<size_t>(8, std::max<size_t>(1, fill_list.size() / 16)));
+    if (nthreads <= 1) {
+        for (const auto &c : fill_list) fill_slot(*this, c, base);
+    } else {
+        std::vector<std::thread> workers;
+        for (int t = 0; t < nthreads; ++t)
+            workers.emplace_back([&, t] {
+                for (size_t j = t; j < fill_list.size(); j += nthreads) fill_slot(*this, fill_list[j], base);
+            });
+        for (auto &th : workers) th.join();
+    }
+    fill_list.clear();
     if (d_stage_cap < bytes) {
         NS_HIP(hipStreamSynchronize(st));
         stage_arena.release();
-        d_stage_cap = std::max(bytes, size_t(1) << 20);
+        d_stage_cap = std::max(bytes + bytes / 2, size_t(1) << 20);
         d_stage = stage_arena.alloc<unsigned char>(int64_t(d_stage_cap));
     }
     NS_HIP(hipMemcpyAsync(d_stage, buf.p, bytes, hipMemcpyHostToDevice, st));
     NS_HIP(hipEventRecord(buf.done, st));
-    launch_scatter(reinterpret_cast<const ScatterEntry *>(d_stage), n, d_stage, st);
+}
+
+void DevicePool::scatter_range(int lo, int hi, hipStream_t st) {
+    if (hi <= lo) return;
+    launch_scatter(reinterpret_cast<const ScatterEntry *>(d_stage) + lo, hi - lo, d_stage, st);
     NS_HIP(hipGetLastError());
 }
 
-void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &moves, hipStream_t st) {
+void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &moves, Stager &sg) {
     if (moves.empty()) return;
     const int64_t N = view.ncap, S = view.scap;
-    Stager sg;
     std::vector<int> touched;
-    SlotImage img;
+    std::vector<PlaceChunks> chunks;
+    auto off = [&](const void *dst, size_t bytes) { return sg.alloc(dst, bytes); };
+    // 1. Slot tables and chunk reservations, in order.
     for (const auto &mv : moves) {
         const int pb = mv.first;
         const BrainSpec *spec = mv.second;
@@ -576,35 +645,59 @@ void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &mov
         // neuron list is fixed at build)
         if (pb < 0 || pb >= nb || (slot_kind[pb] == 0 && spec && spec->brain))
             throw std::runtime_error("Slot cannot take a placement");
+        const HostBrain *b = spec ? spec->brain : nullptr;
+        if (batch_of[pb] >= 0 && pool_of[batch_of[pb]] == pb) pool_of[batch_of[pb]] = -1;
+        if (!b) {
+            // A vacated slot never runs (its run flag stays clear), so its
+            // old image is left in place until a newborn overwrites it.
+            occ_n[pb] = 0;
+            occ_s[pb] = 0;
+            batch_of[pb] = -1;
+            continue;
+        }
         const int64_t n0 = bn0[pb], s0 = bs0[pb];
         const int cn = bnc[pb], cs = bsc[pb];
-        const HostBrain *b = spec ? spec->brain : nullptr;
-        if (b && (int(b->genes.size()) > cn || int(b->edges.size()) > cs))
-            throw std::runtime_error("Brain exceeds its slot");
-        img.make(spec, n0, s0, cn, cs, outputs);
-        for (int k = 0; k < NG_FIELDS; ++k) sg.add(view.ng + k * N + n0, &img.ng[size_t(k) * cn], 4u * cn);
-        for (int k = 0; k < NX_FIELDS; ++k) sg.add(view.nx + k * N + n0, &img.nx[size_t(k) * cn], 4u * cn);
-        sg.add(view.nflags + n0, img.nflags.data(), 4u * cn);
-        sg.add(view.ninput + n0, img.ninput.data(), 4u * cn);
-        sg.add(view.nin + n0, img.nin.data(), 4u * cn);
-        sg.add(d_norig + n0, img.norig.data(), 4u * cn);
-        for (int m = 0; m < SG_FIELDS; ++m) sg.add(view.sg + m * S + s0, &img.sg[size_t(m) * cs], 4u * cs);
-        sg.add(view.sflags + s0, img.sflags.data(), 4u * cs);
-        sg.add(view.spre + s0, img.spre.data(), 4u * cs);
-        sg.add(view.spost + s0, img.spost.data(), 4u * cs);
-        sg.add(d_sorig + s0, img.sorig.data(), 4u * cs);
-        if (outputs) sg.add(view.motor + int64_t(pb) * outputs, img.motor.data(), 4u * outputs);
-        sg.add(view.bout0 + pb, &img.bout0, 4);
-        std::memcpy(&h_nflags[n0], img.nflags.data(), 4u * cn);
-        for (int k = 0; k < cs; ++k) syn_perm[s0 + k] = img.syn_perm[k];
-        h_bout0[pb] = img.bout0;
-        if (batch_of[pb] >= 0 && pool_of[batch_of[pb]] == pb) pool_of[batch_of[pb]] = -1;
-        occ_n[pb] = b ? int32_t(b->genes.size()) : 0;
-        occ_s[pb] = b ? int32_t(b->edges.size()) : 0;
-        batch_of[pb] = b ? spec->batch_index : -1;
-        if (b) pool_of[spec->batch_index] = pb;
+        const int nc = int(b->genes.size()), sc = int(b->edges.size());
+        if (nc > cn || sc > cs) throw std::runtime_error("Brain exceeds its slot");
+        PlaceChunks c;
+        c.pb = pb;
+        c.spec = spec;
+        c.ninx = std::min(cn, nc + 1);
+        // The extra LP/Prinz genes are read only by per-tick-class slots,
+        // which never take placements; a tail past the occupant needs only
+        // its flags (inactive neurons, invalid synapses).
+        for (int k = 0; k < NG_FIELDS; ++k) c.ng[k] = off(view.ng + k * N + n0, 4u * nc);
+        c.fl = off(view.nflags + n0, 4u * cn);
+        c.inp = off(view.ninput + n0, 4u * nc);
+        c.no = off(d_norig + n0, 4u * nc);
+        c.ni = off(view.nin + n0, 4u * c.ninx);
+        for (int m = 0; m < SG_FIELDS; ++m) c.sg[m] = off(view.sg + m * S + s0, 4u * sc);
+        c.sf = off(view.sflags + s0, 4u * cs);
+        c.pre = off(view.spre + s0, 4u * sc);
+        c.post = off(view.spost + s0, 4u * sc);
+        c.so = off(d_sorig + s0, 4u * sc);
+        c.mo = off(view.motor + int64_t(pb) * outputs, 4u * outputs);
+        sg.add(view.bout0 + pb, &spec->output_base, 4);
+        h_bout0[pb] = spec->output_base;
+        occ_n[pb] = nc;
+        occ_s[pb] = sc;
+        batch_of[pb] = spec->batch_index;
+        pool_of[spec->batch_index] = pb;
         if (slot_kind[pb] == 2) touched.push_back(pack_of[pb]);
+        for (int i = 0; i < cn; ++i) {
+            if (i < nc) {
+                const ns_neuron_gene &g = b->genes[i];
+                h_nflags[n0 + i] = uint32_t(g.type) | (g.active ? NF_ACTIVE : 0) |
+                                   (b->receptor[i] ? NF_RECEPTOR : 0) | (spec->world ? NF_WORLD : 0);
+            } else {
+                h_nflags[n0 + i] = 0;   // inactive tail
+            }
+        }
+        chunks.push_back(c);
     }
+    fill_list.insert(fill_list.end(), chunks.begin(), chunks.end());
+    // 2. Slot orders of the packs whose neuron types changed (the images
+    // themselves are written by upload_stage, straight into pinned memory).
     std::sort(touched.begin(), touched.end());
     touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
     for (int k : touched) {
@@ -612,9 +705,6 @@ void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &mov
         pack_order(pk, &h_nflags[pk.n0], &h_perm[pk.n0]);
         sg.add(d_perm + pk.n0, &h_perm[pk.n0], 2u * pk.nn);
     }
-    PinnedStaging::Buf &buf = staging.take(sg.bytes());
-    sg.write(buf.p);
-    scatter(buf, sg.bytes(), int(sg.entries.size()), st);
 }
 
 // Neuron/synapse state in batch order, genome order within each brain.
@@ -812,7 +902,7 @@ static int ladder(int n, float slack) {
     if (!(slack > 0) || n <= 0) return n;
     const double r = 1.0 + slack;
     double c = 16.0;
-    while (std::ceil(c) < n) c *= r;
+    while (std::ceil(c) < n) c *= c < 1024.0 ? r : r * r;   // coarser rungs for large brains
     return int(std::ceil(c));
 }
 
@@ -836,10 +926,13 @@ std::vector<BrainSpec> Batch::slot_specs() const {
             const BrainSpec &q = out[order[std::min(count - 1, int((j + 0.5) * count / nsp))]];
             extra.push_back(BrainSpec{nullptr, -1, 0, 0, world, q.cap_n, q.cap_s});
         }
-        for (int j = 0; j < 2; ++j) {
-            const BrainSpec &q = out[order[count - 1]];
-            extra.push_back(BrainSpec{nullptr, -1, 0, 0, world, q.cap_n, q.cap_s});
-        }
+        // "jumbo" spares as large as the largest slot in both dimensions:
+        // best fit reaches them only when nothing smaller fits, and they
+        // return to the free list when their brain is replaced
+        int jn = 0, js = 0;
+        for (int i = 0; i < count; ++i) { jn = std::max(jn, out[i].cap_n); js = std::max(js, out[i].cap_s); }
+        const int jumbos = std::max(8, std::min(256, count / 256));   // free ones cost no compute
+        for (int j = 0; j < jumbos; ++j) extra.push_back(BrainSpec{nullptr, -1, 0, 0, world, jn, js});
         out.insert(out.end(), extra.begin(), extra.end());
     }
     return out;
@@ -858,6 +951,7 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
     }
     std::vector<int> old_slot(k), new_slot(k, -1);
     for (int j = 0; j < k; ++j) old_slot[j] = pool.pool_of[idx[j]];
+    stage.clear();
     bool rebuild = false;
     for (int j = 0; j < k && !rebuild; ++j)
         if (has_slow_cell(fresh[j]) || (!pool.plastic && has_plastic_pair(fresh[j]))) rebuild = true;
@@ -924,11 +1018,12 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
         }
         for (int j = 0; j < k; ++j)
             if (!reused[old_slot[j]]) moves.push_back({old_slot[j], nullptr});
-        pool.place(moves, st);
+        pool.place(moves, stage);
         placements += k;
     }
     // Fresh state for the newborns (initial states seeded as ns_gpu_reset),
-    // then the run flags of the last mask under the new slot mapping.
+    // then the run flags of the last mask under the new slot mapping; one
+    // upload carries the images and both flag sets.
     std::vector<uint8_t> run(pool.nb, 0), next(pool.nb, 0);
     std::vector<uint32_t> bseed(pool.nb, 0);
     for (int j = 0; j < k; ++j) {
@@ -940,23 +1035,15 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
         const int i = pool.batch_of[pb];
         next[pb] = uint8_t((i >= 0 && last_mask[i] ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
     }
-    {
-        Stager sg;
-        sg.add(pool.view.brun, run.data(), size_t(pool.nb));
-        sg.add(pool.d_bseed, bseed.data(), 4u * pool.nb);
-        PinnedStaging::Buf &buf = pool.staging.take(sg.bytes());
-        sg.write(buf.p);
-        pool.scatter(buf, sg.bytes(), int(sg.entries.size()), st);
-    }
+    stage.add(pool.view.brun, run.data(), size_t(pool.nb));
+    stage.add(pool.d_bseed, bseed.data(), 4u * pool.nb);
+    const int first_pass = int(stage.entries.size());
+    stage.add(pool.view.brun, next.data(), size_t(pool.nb));
+    pool.upload_stage(stage, st);
+    pool.scatter_range(0, first_pass, st);
     launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig, pool.d_sorig, st);
     NS_HIP(hipGetLastError());
-    {
-        Stager sg;
-        sg.add(pool.view.brun, next.data(), size_t(pool.nb));
-        PinnedStaging::Buf &buf = pool.staging.take(sg.bytes());
-        sg.write(buf.p);
-        pool.scatter(buf, sg.bytes(), int(sg.entries.size()), st);
-    }
+    pool.scatter_range(first_pass, int(stage.entries.size()), st);
     h_brun = next;
     brun_valid = true;
 }

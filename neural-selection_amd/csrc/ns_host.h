@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -69,6 +71,51 @@ struct BrainSpec {
     int cap_n = 0, cap_s = 0; // slot capacity; at least the brain's size
 };
 
+// Plans one staged upload: each entry is a device range and an offset into
+// the staged data.  Data is written when the buffer exists: `add` records a
+// copy from host memory that must stay valid until write(); `alloc` only
+// reserves room that a filler writes later.  Chunks are 16-byte aligned.
+struct Stager {
+    struct Copy { size_t off; const void *src; size_t bytes; };
+    std::vector<ScatterEntry> entries;
+    std::vector<Copy> copies;
+    size_t size = 0;
+    void clear() { entries.clear(); copies.clear(); size = 0; }
+    size_t alloc(const void *dst, size_t bytes) {
+        const size_t off = (size + 15) & ~size_t(15);
+        size = off + bytes;
+        entries.push_back(ScatterEntry{uint64_t(reinterpret_cast<uintptr_t>(dst)), uint64_t(off),
+                                       uint64_t(bytes)});
+        return off;
+    }
+    void add(const void *dst, const void *src, size_t bytes) {
+        if (!bytes) return;
+        copies.push_back(Copy{alloc(dst, bytes), src, bytes});
+    }
+    size_t head() const { return (entries.size() * sizeof(ScatterEntry) + 15) & ~size_t(15); }
+    size_t bytes() const { return head() + size; }
+    // entries (offsets rebased past them) and the recorded copies; returns
+    // the start of the data region for fillers
+    unsigned char *write(unsigned char *out) const {
+        const size_t h = head();
+        ScatterEntry *e = reinterpret_cast<ScatterEntry *>(out);
+        for (size_t i = 0; i < entries.size(); ++i) {
+            e[i] = entries[i];
+            e[i].src += h;
+        }
+        for (const Copy &c : copies) std::memcpy(out + h + c.off, c.src, c.bytes);
+        return out + h;
+    }
+};
+
+// Offsets (into the staged data) of one newborn's slot-image chunks.
+struct PlaceChunks {
+    int pb = 0;
+    const BrainSpec *spec = nullptr;
+    size_t ng[NG_FIELDS], fl, inp, no, ni, sg[SG_FIELDS], sf, pre, post, so, mo;
+    int ninx = 0;
+};
+
 // Pinned host staging for slot placements, double buffered so one buffer can
 // be filled while the other's copy is still in flight.
 struct PinnedStaging {
@@ -83,8 +130,8 @@ struct PinnedStaging {
             if (b.p) (void)hipHostFree(b.p);
             b.p = nullptr;
             b.cap = 0;
-            NS_HIP(hipHostMalloc(&b.p, bytes));
-            b.cap = bytes;
+            NS_HIP(hipHostMalloc(&b.p, bytes + bytes / 2));
+            b.cap = bytes + bytes / 2;
         }
         if (!b.done) NS_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
         return b;
@@ -152,13 +199,16 @@ public:
     DeviceArena stage_arena;         // device copy of a staged placement
     unsigned char *d_stage = nullptr;
     size_t d_stage_cap = 0;
-    // Copy `bytes` of pinned host data to the device staging buffer on `st`
-    // and scatter its `n` entries (the first n * sizeof(ScatterEntry) bytes).
-    void scatter(PinnedStaging::Buf &buf, size_t bytes, int n, hipStream_t st);
-    // Write the images of `moves` (slot, occupant or nullptr) into the pool on
-    // stream `st`: genes, flags, routes, endpoints, motors and the affected
-    // packs' slot orders.  State is left alone (the caller resets newborns).
-    void place(const std::vector<std::pair<int, const BrainSpec *>> &moves, hipStream_t st);
+    // Copy the staged buffer to the device on `st` (one upload); entries
+    // [lo, hi) are then scattered by scatter_range.
+    void upload_stage(const struct Stager &sg, hipStream_t st);
+    std::vector<PlaceChunks> fill_list;   // images place() reserved, filled by upload_stage
+    void scatter_range(int lo, int hi, hipStream_t st);
+    // Stage the images of `moves` (slot, occupant or nullptr = vacate) into
+    // `sg`: genes, flags, routes, endpoints, motors and the affected packs'
+    // slot orders, and update the host-side slot tables.  State is left
+    // alone (the caller resets newborns).
+    void place(const std::vector<std::pair<int, const BrainSpec *>> &moves, struct Stager &sg);
     uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
     hipStream_t stream = nullptr;
 };
@@ -229,6 +279,7 @@ struct Batch {
     float slack = 0.0f, spare = 0.0f;
     int64_t relayouts = 0, placements = 0;
     std::vector<uint8_t> last_mask;                    // batch order, last run mask
+    Stager stage;                                      // reused across replacements
 
     void create(std::vector<HostBrain> &&owned, int max_delay_unused);
     void relayout(const Tile &tile);

@@ -193,6 +193,15 @@ def c3(seed=3, brains=32768, total_neurons=4194304, fanout=2.0, calls=100, cap=4
                      -1.0, 0.5, _stdp_mix, 0.06, seed, POISSON_20HZ_16MS, 230.0, 16, calls)
 
 
+def c3_newborns(seed=33, count=2048):
+    """Newborn genomes for C3's births: the same generator and size
+    distribution as c3(), for `count` brains."""
+    rng = np.random.default_rng(seed)
+    sizes = c3_sizes(rng, count, 128 * count)
+    return _assemble('c3-births', rng, sizes, 2.0, _uniform_types([0, 1, 4, 5]), 0.2, 8, 19, 5, 16,
+                     -1.0, 0.5, _stdp_mix, 0.06, seed, POISSON_20HZ_16MS, 230.0, 16, 1)
+
+
 def small_mix(seed=11, brains=8, calls=40, plasticity=True, max_delay=8, types=(0, 1, 4, 5, 6),
               modulator_p=0.1, fanout=4.0, lo=8, hi=48, active_p=0.9):
     """Small ragged brains with every tick-path feature on (parity cases)."""
