@@ -322,9 +322,13 @@ int ns_world_homeostasis(void *pool, const int *age, const uint8_t *frozen);
 
 /* Episode task: a seeded 4-d linear system x' = A x + B u + N(0, sigma^2)
  * observed through an 8-sensor opponent rate code; two motor neurons drive
- * u.  The brain and task state stay resident in the device pools until the
- * episode ends.  Requests are admitted when their neurons and synapses fit
- * the free pool space. */
+ * u.  One device pool of slots holds every episode: size classes
+ * ceil(16 * sqrt2^k) up to max_episode_neurons, the same number of slots per
+ * class (at least one) within neuron_capacity and episode_capacity, synapse
+ * capacity per slot in the ratio synapse_capacity / neuron_capacity (clamped
+ * to [1, 32]).  A
+ * request is admitted at a launch boundary into the best-fitting free slot;
+ * brain and task state stay resident there until the episode ends. */
 typedef struct ns_serve_config {
     int neuron_capacity, synapse_capacity, episode_capacity;
     int steps_per_episode;      /* env steps, 16 ticks each */
@@ -332,6 +336,7 @@ typedef struct ns_serve_config {
     int steps_per_launch;       /* env steps per device launch */
     float noise_sigma;
     float input_gain;
+    int max_episode_neurons;    /* largest slot class (0: 4096) */
 } ns_serve_config;
 
 void *ns_serve_create(const ns_serve_config *config);

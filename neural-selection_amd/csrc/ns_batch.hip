@@ -285,7 +285,7 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     nb = int(specs.size());
     const TileShape *shape = find_tile(t);
     if (!shape) throw std::runtime_error("Tile shape not compiled");
-    plastic = false;
+    plastic = force_plastic;
     nbatch = 0;
     for (const auto &spec : specs) {
         if (!spec.brain) continue;

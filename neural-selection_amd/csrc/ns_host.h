@@ -185,6 +185,7 @@ public:
     int32_t *d_slow = nullptr, *d_big = nullptr;
     uint16_t *d_perm = nullptr;
     bool plastic = false;            // any valid plastic excitatory pair
+    bool force_plastic = false;      // build the plastic layout even without one (serving)
     int smax_lds = 0;                // LDS synapse capacity per pack (LDS tiles)
     int slow_count = 0, fallback_brains = 0, nbig = 0;
     // Slots.  Every pool brain index is a slot; its capacity (bnc, bsc) may

@@ -449,6 +449,11 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
     }
     const Pack pk = packs[blockIdx.x - nbig];
     const int tid = threadIdx.x;
+    {   // a pack whose brains all sit out this call (free slots, masks) does nothing
+        int any = 0;
+        for (int b = tid; b < pk.nb; b += WG) any |= p.brun[pk.b0 + b] & 1;
+        if (!__syncthreads_or(any)) return;
+    }
     const int64_t N = p.ncap, S = p.scap;
     const int64_t n0 = pk.n0, s0 = pk.s0;
 
@@ -741,6 +746,11 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     }
     const Pack pk = packs[blockIdx.x - nbig];
     const int tid = threadIdx.x;
+    {   // a pack whose brains all sit out this call (free slots, masks) does nothing
+        int any = 0;
+        for (int b = tid; b < pk.nb; b += WG) any |= p.brun[pk.b0 + b] & 1;
+        if (!__syncthreads_or(any)) return;
+    }
     const int64_t N = p.ncap, S = p.scap;
     const int64_t n0 = pk.n0, s0 = pk.s0;
     const int ns = pk.ns;

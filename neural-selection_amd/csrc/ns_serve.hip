@@ -8,14 +8,18 @@
 // observed through an opponent rate code on 8 channels (2 per dimension).
 // Return = -sum over steps of |x|^2.
 //
-// A serving thread admits queued requests at every launch boundary while
-// their neurons and synapses fit the free pool capacity.  Requests admitted
-// together form a cohort: one device pool (resident packs plus per-tick
-// fallback), its own stream, and its task state in device memory.  Cohorts
-// admitted at different times run concurrently on different streams, so new
-// work overlaps work already on the device.  Each episode's arithmetic is
-// independent of its cohort's composition (per-brain sequential sums, seeds
-// per episode), so results are bit-identical alone and in any mix.
+// One device pool of slots serves every episode.  Slot capacities follow a
+// ladder of size classes (ratio sqrt 2) with the same number of slots per
+// class, within the configured neuron and synapse capacity.  At every launch
+// boundary the serving thread admits queued requests into the best-fitting
+// free slots (their images and fresh state written on the device, like a
+// brain replacement), then advances every live episode by `steps_per_launch`
+// environment steps: per step one encode, one resident tick launch over the
+// whole pool and one environment update, all for every live slot at once.
+// Finished episodes free their slots for the next boundary.  Each episode's
+// arithmetic is independent of what else is in the pool (per-brain
+// sequential sums, seeds per episode), so results are bit-identical alone
+// and in any mix.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -25,7 +29,9 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <thread>
+#include <tuple>
 
 #include "ns_host.h"
 
@@ -33,13 +39,13 @@ namespace nsa {
 
 namespace {
 constexpr int kChannels = 8, kOutputs = 2, kDims = 4;
-// task record per episode (floats): x[4], A[16], B[8], return, motor totals[2], steps
+// task record per slot (floats): x[4], A[16], B[8], return, motor totals[2], steps
 constexpr int kTaskFloats = 32;
 enum { T_X = 0, T_A = 4, T_B = 20, T_RET = 28, T_M0 = 29, T_M1 = 30, T_STEPS = 31 };
 
-__global__ void k_encode(const float *task, int count, float *input) {
+__global__ void k_encode(const float *task, const uint8_t *run, int slots, float *input) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
+    if (e >= slots || !(run[e] & 1)) return;
     const float *x = task + size_t(e) * kTaskFloats + T_X;
     for (int d = 0; d < kDims; ++d) {
         input[e * kChannels + 2 * d] = fminf(1.0f, fmaxf(0.0f, x[d]));
@@ -53,11 +59,13 @@ __device__ inline float uniform01(uint32_t &s) {
 }
 
 // x' = A x + B u + sigma * (sum of 12 uniforms - 6); fixed evaluation order.
-__global__ void k_env(float *task, uint32_t *rng, const float *out, int count, float ticks,
-                      float sigma) {
+// A slot stops updating once its episode has `steps` steps.
+__global__ void k_env(float *task, uint32_t *rng, const float *out, const uint8_t *run, int slots,
+                      float ticks, float sigma, float steps) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count) return;
+    if (e >= slots || !(run[e] & 1)) return;
     float *t = task + size_t(e) * kTaskFloats;
+    if (!(t[T_STEPS] < steps)) return;
     const float u0 = out[e * kOutputs] / ticks, u1 = out[e * kOutputs + 1] / ticks;
     uint32_t s = rng[e];
     float nx[kDims];
@@ -89,23 +97,6 @@ struct Request {
     double submit_s, admit_s = 0;
 };
 
-struct Cohort {
-    std::vector<Request> reqs;
-    std::vector<BrainSpec> specs;
-    DevicePool pool;
-    DeviceArena io;
-    float *d_input = nullptr, *d_out = nullptr, *d_task = nullptr;
-    uint32_t *d_rng = nullptr;
-    hipStream_t stream = nullptr;
-    int steps_done = 0;
-    int64_t neurons = 0, synapses = 0;
-    ~Cohort() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        pool.arena.release();
-        io.release();
-    }
-};
-
 // Seeded stable task: rows of A scaled so |A|_inf = 0.9 (spectral radius
 // < 1); B entries in [-0.5, 0.5]; x0 in [-1, 1].
 void make_task(uint32_t seed, float *t) {
@@ -128,106 +119,192 @@ void make_task(uint32_t seed, float *t) {
     for (int k = 0; k < 8; ++k) t[T_B + k] = next() - 0.5f;
 }
 
+// An admitted episode in slot `pb`.
+struct Episode {
+    Request req;
+    BrainSpec spec;
+    int pb = 0;
+    int steps_done = 0;
+};
+
 struct Server {
     ns_serve_config cfg;
     std::chrono::steady_clock::time_point t0;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Request> queue;
-    std::vector<std::unique_ptr<Cohort>> live;
     std::vector<ns_serve_result> done;
-    std::vector<hipStream_t> streams;
     std::thread worker;
     bool stop = false;
     std::string failure;
-    int64_t next_id = 0, used_n = 0, used_s = 0, used_e = 0;
+    int64_t next_id = 0;
     ns_serve_stats stats{};
-    size_t stream_rr = 0;
+    // the slot pool (serving thread only)
+    DevicePool pool;
+    DeviceArena io;
+    float *d_input = nullptr, *d_out = nullptr, *d_task = nullptr;
+    uint32_t *d_rng = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<Episode>> live;          // by slot
+    std::set<std::tuple<int, int, int>> freeset;         // (neurons, synapses, slot)
+    int max_n = 0, max_s = 0;                            // largest slot shape
+    Stager stage;
+    std::vector<uint8_t> h_run;                          // run flags uploaded last
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    std::unique_ptr<Cohort> admit_locked() {
-        std::unique_ptr<Cohort> c;
-        while (!queue.empty()) {
-            Request &r = queue.front();
-            const int64_t n = int64_t(r.brain.genes.size()), s = int64_t(r.brain.edges.size());
-            if (used_n + n > cfg.neuron_capacity || used_s + s > cfg.synapse_capacity ||
-                used_e + 1 > cfg.episode_capacity)
-                break;
-            if (!c) c.reset(new Cohort);
-            used_n += n; used_s += s; used_e += 1;
-            c->neurons += n; c->synapses += s;
-            r.admit_s = now();
-            c->reqs.push_back(std::move(r));
-            queue.pop_front();
+    // Slots: classes ceil(16 * sqrt2^k) up to the first class that holds
+    // max_episode_neurons; an equal number of slots per class, at least one.
+    void build_pool() {
+        std::vector<int> cls;
+        const double r = std::sqrt(2.0);
+        const int top = cfg.max_episode_neurons > 0 ? cfg.max_episode_neurons : 4096;
+        for (double c = 16.0;; c *= r) {
+            cls.push_back(int(std::ceil(c)));
+            if (cls.back() >= top) break;
         }
-        return c;
+        const double syn_ratio = std::min(32.0, std::max(1.0, double(cfg.synapse_capacity) /
+                                                                 cfg.neuron_capacity));
+        int64_t per_class_n = 0;
+        for (int c : cls) per_class_n += c;
+        int per = std::max(1, std::min(int(cfg.episode_capacity / int(cls.size())),
+                                       int(cfg.neuron_capacity / std::max<int64_t>(per_class_n, 1))));
+        std::vector<BrainSpec> specs;
+        for (int c : cls)
+            for (int j = 0; j < per; ++j)
+                specs.push_back(BrainSpec{nullptr, -1, 0, 0, false, c, int(std::ceil(c * syn_ratio))});
+        max_n = cls.back();
+        max_s = int(std::ceil(cls.back() * syn_ratio));
+        pool.force_plastic = true;
+        pool.build(specs, kOutputs, Tile{1024, 1, 0, 1, 3584}, stream);
+        pool.nbatch = pool.nb;                 // rows are slots
+        pool.pool_of.assign(pool.nb, -1);
+        const int rows = std::max(pool.nb, 1);
+        d_input = io.alloc<float>(int64_t(rows) * kChannels);
+        d_out = io.alloc<float>(int64_t(rows) * kOutputs);
+        d_task = io.alloc<float>(int64_t(rows) * kTaskFloats);
+        d_rng = io.alloc<uint32_t>(rows);
+        NS_HIP(hipMemset(d_task, 0, sizeof(float) * size_t(rows) * kTaskFloats));
+        NS_HIP(hipMemset(d_input, 0, sizeof(float) * size_t(rows) * kChannels));
+        live.clear();
+        live.resize(pool.nb);
+        for (int pb = 0; pb < pool.nb; ++pb)
+            if (pool.slot_kind[pb] != 0) freeset.insert({pool.bnc[pb], pool.bsc[pb], pb});
+        h_run.assign(pool.nb, 0);
     }
 
-    void start_cohort(Cohort &c) {
-        const int count = int(c.reqs.size());
-        for (int i = 0; i < count; ++i)
-            c.specs.push_back(BrainSpec{&c.reqs[i].brain, i, i * kChannels, i * kOutputs, false});
-        c.stream = streams[stream_rr++ % streams.size()];
-        // LDS-resident tick kernel (the fastest measured shape for pools of
-        // this kind); episodes carry no plasticity, so packs take up to 5632
-        // synapses each.
-        c.pool.build(c.specs, kOutputs, Tile{1024, 1, 0, 1, 3584}, c.stream);
-        c.d_input = c.io.alloc<float>(int64_t(count) * kChannels);
-        c.d_out = c.io.alloc<float>(int64_t(count) * kOutputs);
-        c.d_task = c.io.alloc<float>(int64_t(count) * kTaskFloats);
-        c.d_rng = c.io.alloc<uint32_t>(count);
-        std::vector<float> task(size_t(count) * kTaskFloats);
-        std::vector<uint32_t> rng(count), bseed(count);
-        std::vector<uint8_t> run(count);
-        for (int i = 0; i < count; ++i) {
-            make_task(c.reqs[i].task_seed, &task[size_t(i) * kTaskFloats]);
-            rng[i] = c.reqs[i].task_seed * 2654435761u + 12345u;
-            if (!rng[i]) rng[i] = 1;
-        }
-        for (int pb = 0; pb < count; ++pb) {
-            bseed[pb] = c.reqs[c.pool.batch_of[pb]].seed;
-            run[pb] = uint8_t(1 | (c.pool.fallback[pb] ? 2 : 0));
-        }
-        NS_HIP(hipMemcpy(c.d_task, task.data(), sizeof(float) * task.size(), hipMemcpyHostToDevice));
-        NS_HIP(hipMemcpy(c.d_rng, rng.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
-        NS_HIP(hipMemcpy(c.pool.d_bseed, bseed.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
-        NS_HIP(hipMemcpy(c.pool.view.brun, run.data(), count, hipMemcpyHostToDevice));
-        launch_reset(c.pool.view, c.pool.nn, c.pool.ns, count, c.pool.d_bseed, c.pool.d_norig,
-                     c.pool.d_sorig, c.stream);
-        NS_HIP(hipGetLastError());
+    // Best free slot for (n, s), or -1.
+    int fit(int n, int s) {
+        auto it = freeset.lower_bound({n, s, -1});
+        while (it != freeset.end() && std::get<1>(*it) < s) ++it;
+        if (it == freeset.end()) return -1;
+        const int pb = std::get<2>(*it);
+        freeset.erase(it);
+        return pb;
     }
 
-    void enqueue_steps(Cohort &c, int steps) {
-        const int count = int(c.reqs.size());
-        const unsigned blocks = unsigned((count + 127) / 128);
-        for (int k = 0; k < steps; ++k) {
-            hipLaunchKernelGGL(k_encode, dim3(blocks), dim3(128), 0, c.stream, c.d_task, count,
-                               c.d_input);
-            run_pool(c.pool, NS_FORM_RESIDENT, cfg.ticks_per_step, c.d_input, cfg.input_gain,
-                     c.d_out, int64_t(count) * kOutputs, 1, c.stream);
-            hipLaunchKernelGGL(k_env, dim3(blocks), dim3(128), 0, c.stream, c.d_task, c.d_rng,
-                               c.d_out, count, float(cfg.ticks_per_step), cfg.noise_sigma);
-        }
-        NS_HIP(hipGetLastError());
-    }
-
-    void finish_cohort(Cohort &c) {
-        const int count = int(c.reqs.size());
-        std::vector<float> task(size_t(count) * kTaskFloats);
-        std::vector<int32_t> status(count);
-        NS_HIP(hipStreamSynchronize(c.stream));
-        NS_HIP(hipMemcpy(task.data(), c.d_task, sizeof(float) * task.size(), hipMemcpyDeviceToHost));
-        const double t = now();
+    // Admit queued requests into free slots (queue order; a request that
+    // does not fit yet waits without blocking smaller ones behind it).
+    std::vector<Episode *> admit() {
+        std::vector<Episode *> fresh;
         std::lock_guard<std::mutex> lock(mu);
-        for (int i = 0; i < count; ++i) {
-            const float *tk = &task[size_t(i) * kTaskFloats];
+        for (auto it = queue.begin(); it != queue.end();) {
+            const int n = int(it->brain.genes.size()), s = int(it->brain.edges.size());
+            const int pb = fit(n, s);
+            if (pb < 0) { ++it; continue; }
+            auto ep = std::make_unique<Episode>();
+            ep->req = std::move(*it);
+            ep->req.admit_s = now();
+            ep->pb = pb;
+            ep->spec = BrainSpec{&ep->req.brain, pb, pb * kChannels, pb * kOutputs, false};
+            fresh.push_back(ep.get());
+            live[pb] = std::move(ep);
+            it = queue.erase(it);
+        }
+        stats.admitted += int64_t(fresh.size());
+        return fresh;
+    }
+
+    // Write the newcomers' images, fresh state, tasks and the run flags of
+    // every occupied slot, in one staged upload.
+    void start(const std::vector<Episode *> &fresh, bool flags_changed) {
+        if (fresh.empty() && !flags_changed) return;
+        stage.clear();
+        std::vector<std::pair<int, const BrainSpec *>> moves;
+        for (Episode *e : fresh) moves.push_back({e->pb, &e->spec});
+        pool.place(moves, stage);
+        std::vector<uint8_t> run(pool.nb, 0), next(pool.nb, 0);
+        std::vector<uint32_t> bseed(pool.nb, 0);
+        std::vector<float> task(size_t(std::max<size_t>(fresh.size(), 1)) * kTaskFloats);
+        std::vector<uint32_t> rng(std::max<size_t>(fresh.size(), 1));
+        for (size_t j = 0; j < fresh.size(); ++j) {
+            const Episode &e = *fresh[j];
+            run[e.pb] = 1;
+            bseed[e.pb] = e.req.seed;
+            make_task(e.req.task_seed, &task[j * kTaskFloats]);
+            rng[j] = e.req.task_seed * 2654435761u + 12345u;
+            if (!rng[j]) rng[j] = 1;
+        }
+        for (int pb = 0; pb < pool.nb; ++pb) next[pb] = live[pb] ? 1 : 0;
+        stage.add(pool.view.brun, run.data(), size_t(pool.nb));
+        stage.add(pool.d_bseed, bseed.data(), 4u * pool.nb);
+        for (size_t j = 0; j < fresh.size(); ++j) {
+            const int pb = fresh[j]->pb;
+            stage.add(d_task + size_t(pb) * kTaskFloats, &task[j * kTaskFloats], 4u * kTaskFloats);
+            stage.add(d_rng + pb, &rng[j], 4);
+        }
+        const int first = int(stage.entries.size());
+        stage.add(pool.view.brun, next.data(), size_t(pool.nb));
+        pool.upload_stage(stage, stream);
+        pool.scatter_range(0, first, stream);
+        if (!fresh.empty())
+            launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig,
+                         pool.d_sorig, stream);
+        pool.scatter_range(first, int(stage.entries.size()), stream);
+        NS_HIP(hipGetLastError());
+        h_run = next;
+        // the staged copies of run/next/task/rng must outlive the upload
+        NS_HIP(hipStreamSynchronize(stream));
+    }
+
+    void steps(int n) {
+        const unsigned blocks = unsigned((pool.nb + 127) / 128);
+        for (int k = 0; k < n; ++k) {
+            hipLaunchKernelGGL(k_encode, dim3(blocks), dim3(128), 0, stream, d_task, pool.view.brun,
+                               pool.nb, d_input);
+            run_pool(pool, NS_FORM_RESIDENT, cfg.ticks_per_step, d_input, cfg.input_gain, d_out,
+                     int64_t(pool.nb) * kOutputs, 1, stream);
+            hipLaunchKernelGGL(k_env, dim3(blocks), dim3(128), 0, stream, d_task, d_rng, d_out,
+                               pool.view.brun, pool.nb, float(cfg.ticks_per_step), cfg.noise_sigma,
+                               float(cfg.steps_per_episode));
+        }
+        NS_HIP(hipGetLastError());
+    }
+
+    // Collect finished episodes and free their slots.  Returns whether any
+    // slot was freed (its run flag must be cleared).
+    bool finish() {
+        std::vector<int> ended;
+        for (int pb = 0; pb < pool.nb; ++pb)
+            if (live[pb] && live[pb]->steps_done >= cfg.steps_per_episode) ended.push_back(pb);
+        if (ended.empty()) return false;
+        std::vector<float> task(size_t(pool.nb) * kTaskFloats);
+        NS_HIP(hipMemcpy(task.data(), d_task, sizeof(float) * task.size(), hipMemcpyDeviceToHost));
+        const double t = now();
+        std::vector<std::pair<int, const BrainSpec *>> vacate;
+        for (int pb : ended) vacate.push_back({pb, nullptr});
+        stage.clear();
+        pool.place(vacate, stage);             // slot tables only
+        std::lock_guard<std::mutex> lock(mu);
+        for (int pb : ended) {
+            const Episode &e = *live[pb];
+            const float *tk = &task[size_t(pb) * kTaskFloats];
             ns_serve_result r{};
-            r.id = c.reqs[i].id;
-            r.submit_s = c.reqs[i].submit_s;
-            r.admit_s = c.reqs[i].admit_s;
+            r.id = e.req.id;
+            r.submit_s = e.req.submit_s;
+            r.admit_s = e.req.admit_s;
             r.finish_s = t;
             r.episode_return = tk[T_RET];
             for (int d = 0; d < kDims; ++d) r.final_state[d] = tk[T_X + d];
@@ -235,46 +312,42 @@ struct Server {
             r.motor_spikes[1] = int32_t(tk[T_M1]);
             r.status = 0;
             done.push_back(r);
+            freeset.insert({pool.bnc[pb], pool.bsc[pb], pb});
+            live[pb].reset();
         }
-        used_n -= c.neurons; used_s -= c.synapses; used_e -= count;
-        stats.finished += count;
+        stats.finished += int64_t(ended.size());
         cv.notify_all();
+        return true;
     }
 
     void loop() {
         try {
+            bool freed = false;
             while (true) {
-                std::unique_ptr<Cohort> fresh;
                 {
                     std::unique_lock<std::mutex> lock(mu);
-                    if (live.empty())
+                    bool any_live = false;
+                    for (const auto &e : live) any_live = any_live || bool(e);
+                    if (!any_live)
                         cv.wait(lock, [&] { return stop || !queue.empty(); });
-                    if (stop && queue.empty() && live.empty()) break;
-                    fresh = admit_locked();
-                    if (fresh) stats.admitted += int64_t(fresh->reqs.size());
+                    if (stop) break;
                 }
                 const auto busy0 = std::chrono::steady_clock::now();
-                if (fresh) {
-                    start_cohort(*fresh);
-                    live.push_back(std::move(fresh));
+                std::vector<Episode *> fresh = admit();
+                start(fresh, freed);
+                int resident = 0;
+                for (const auto &e : live) resident += bool(e);
+                if (resident > stats.resident_peak) stats.resident_peak = resident;
+                if (resident) {
+                    const int n = cfg.steps_per_launch;
+                    steps(n);
+                    NS_HIP(hipStreamSynchronize(stream));
+                    for (auto &e : live)
+                        if (e) e->steps_done += n;
+                    stats.launches += 1;
                 }
-                if (int64_t(live.size()) > stats.resident_peak) stats.resident_peak = int64_t(live.size());
-                for (auto &c : live) {
-                    const int steps = std::min(cfg.steps_per_launch, cfg.steps_per_episode - c->steps_done);
-                    enqueue_steps(*c, steps);
-                    c->steps_done += steps;
-                }
-                stats.launches += 1;
-                for (auto &c : live) NS_HIP(hipStreamSynchronize(c->stream));
                 stats.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy0).count();
-                for (size_t k = 0; k < live.size();) {
-                    if (live[k]->steps_done >= cfg.steps_per_episode) {
-                        finish_cohort(*live[k]);
-                        live.erase(live.begin() + k);
-                    } else {
-                        ++k;
-                    }
-                }
+                freed = finish();
             }
         } catch (const std::exception &e) {
             std::lock_guard<std::mutex> lock(mu);
@@ -282,6 +355,7 @@ struct Server {
             stop = true;
             cv.notify_all();
         }
+        if (stream) (void)hipStreamSynchronize(stream);
     }
 };
 }  // namespace
@@ -297,20 +371,18 @@ void *ns_serve_create(const ns_serve_config *config) {
         if (!config || config->neuron_capacity < 1 || config->synapse_capacity < 0 ||
             config->episode_capacity < 1 || config->steps_per_episode < 1 ||
             config->ticks_per_step < 1 || config->ticks_per_step > 10000 ||
-            config->steps_per_launch < 1 || !(config->noise_sigma >= 0) || !(config->input_gain >= 0))
+            config->steps_per_launch < 1 || !(config->noise_sigma >= 0) || !(config->input_gain >= 0) ||
+            config->max_episode_neurons < 0 || config->max_episode_neurons > (1 << 20))
             throw std::runtime_error("Invalid serving configuration");
         auto *s = new Server;
         s->cfg = *config;
         s->t0 = std::chrono::steady_clock::now();
         try {
-            for (int k = 0; k < 4; ++k) {
-                hipStream_t st;
-                NS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-                s->streams.push_back(st);
-            }
+            NS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+            s->build_pool();
             s->worker = std::thread([s] { s->loop(); });
         } catch (...) {
-            for (auto st : s->streams) (void)hipStreamDestroy(st);
+            if (s->stream) (void)hipStreamDestroy(s->stream);
             delete s;
             throw;
         }
@@ -331,8 +403,9 @@ void ns_serve_destroy(void *p) {
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
     s->live.clear();
-    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    hipStream_t st = s->stream;
     delete s;
+    if (st) (void)hipStreamDestroy(st);
 }
 
 int64_t ns_serve_submit(void *p, int count, int edges, const float *ng, const float *sg,
@@ -341,8 +414,11 @@ int64_t ns_serve_submit(void *p, int count, int edges, const float *ng, const fl
         auto &s = *static_cast<Server *>(p);
         std::unique_ptr<HostBrain> b(parse_gym_brain(count, edges, kChannels, kOutputs, ng, sg,
                                                      motors, 8));
-        if (count > s.cfg.neuron_capacity || edges > s.cfg.synapse_capacity)
-            throw std::runtime_error("Episode exceeds the pool capacity");
+        if (count > s.max_n || edges > s.max_s)
+            throw std::runtime_error("Episode exceeds the largest slot");
+        for (const auto &g : b->genes)
+            if (g.type == 2 || g.type == 3)
+                throw std::runtime_error("LP/Prinz cells are not served (per-tick cells)");
         std::lock_guard<std::mutex> lock(s.mu);
         if (!s.failure.empty()) throw std::runtime_error("Server failed: " + s.failure);
         Request r;

@@ -47,7 +47,8 @@ class ServeConfig(ct.Structure):
     _fields_ = [('neuron_capacity', ct.c_int), ('synapse_capacity', ct.c_int),
                 ('episode_capacity', ct.c_int), ('steps_per_episode', ct.c_int),
                 ('ticks_per_step', ct.c_int), ('steps_per_launch', ct.c_int),
-                ('noise_sigma', ct.c_float), ('input_gain', ct.c_float)]
+                ('noise_sigma', ct.c_float), ('input_gain', ct.c_float),
+                ('max_episode_neurons', ct.c_int)]
 
 
 class ServeResult(ct.Structure):
