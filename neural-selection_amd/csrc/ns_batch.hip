@@ -265,15 +265,12 @@ struct SlotImage {
 // Type-sorted slot order of one pack (cortical AdEx (type 0), receptor cells
 // (5), other AdEx (1, 4), LFS (6), then inactive cells): waves of one type
 // take the same branches.
+// Host form of k_pack_order (used when a pool is built).
 static void pack_order(const Pack &pk, const uint32_t *nflags, uint16_t *perm) {
-    auto cls = [](uint32_t f) {
-        const uint32_t ty = f & NF_TYPE;
-        return !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
-    };
-    int start[6] = {0, 0, 0, 0, 0, 0};     // stable counting sort by class
-    for (int j = 0; j < pk.nn; ++j) start[cls(nflags[j]) + 1]++;
-    for (int c = 0; c < 5; ++c) start[c + 1] += start[c];
-    for (int j = 0; j < pk.nn; ++j) perm[start[cls(nflags[j])]++] = uint16_t(j);
+    int start[kSlotClasses + 1] = {};     // stable counting sort by class
+    for (int j = 0; j < pk.nn; ++j) start[slot_class(nflags[j]) + 1]++;
+    for (int c = 0; c < kSlotClasses; ++c) start[c + 1] += start[c];
+    for (int j = 0; j < pk.nn; ++j) perm[start[slot_class(nflags[j])]++] = uint16_t(j);
 }
 
 void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile &t,
@@ -494,7 +491,7 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     for (int k = 0; k < nbig; ++k) big_list[k] = fallback_brains + k;
     d_big = arena.alloc<int32_t>(std::max(nbig, 1));
     up(d_big, big_list);
-    h_perm.assign(N, 0);
+    std::vector<uint16_t> h_perm(N, 0);
     packs.clear();
     int pb = fallback_brains + nbig;
     for (const Bin &bin : bins) {
@@ -513,6 +510,8 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     }
     d_perm = arena.alloc<uint16_t>(N);
     up(d_perm, h_perm);
+    d_touched = arena.alloc<int32_t>(std::max<size_t>(packs.size(), 1));
+    touched.clear();
     d_packs = arena.alloc<Pack>(std::max<size_t>(packs.size(), 1));
     if (!packs.empty()) up(d_packs, packs);
     const int64_t fb_n = fallback_brains ? int64_t(bn0[fallback_brains - 1]) + bnc[fallback_brains - 1] : 0;
@@ -634,7 +633,7 @@ void DevicePool::scatter_range(int lo, int hi, hipStream_t st) {
 void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &moves, Stager &sg) {
     if (moves.empty()) return;
     const int64_t N = view.ncap, S = view.scap;
-    std::vector<int> touched;
+    touched.clear();
     std::vector<PlaceChunks> chunks;
     auto off = [&](const void *dst, size_t bytes) { return sg.alloc(dst, bytes); };
     // 1. Slot tables and chunk reservations, in order.
@@ -696,15 +695,18 @@ void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &mov
         chunks.push_back(c);
     }
     fill_list.insert(fill_list.end(), chunks.begin(), chunks.end());
-    // 2. Slot orders of the packs whose neuron types changed (the images
-    // themselves are written by upload_stage, straight into pinned memory).
+    // 2. The packs whose slot orders must be recomputed (by order_touched,
+    // on the device, once the images are in place).  The images themselves
+    // are written by upload_stage, straight into pinned memory.
     std::sort(touched.begin(), touched.end());
     touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
-    for (int k : touched) {
-        const Pack &pk = packs[k];
-        pack_order(pk, &h_nflags[pk.n0], &h_perm[pk.n0]);
-        sg.add(d_perm + pk.n0, &h_perm[pk.n0], 2u * pk.nn);
-    }
+    if (!touched.empty()) sg.add(d_touched, touched.data(), 4u * touched.size());
+}
+
+void DevicePool::order_touched(hipStream_t st) {
+    launch_pack_order(view.nflags, d_packs, d_touched, int(touched.size()), d_perm, st);
+    NS_HIP(hipGetLastError());
+    touched.clear();
 }
 
 // Neuron/synapse state in batch order, genome order within each brain.
@@ -890,7 +892,8 @@ void Batch::create(std::vector<HostBrain> &&owned, int) {
     specs.clear();
     for (int i = 0; i < count; ++i)
         specs.push_back(BrainSpec{&brains[i], i, i * channels, i * outputs, world});
-    pool.build(slot_specs(), outputs, default_tile(brains), stream);
+    const Tile t0 = default_tile(brains);
+    pool.build(slot_specs(t0), outputs, t0, stream);
     d_input = io.alloc<float>(int64_t(count) * channels);
     d_output = io.alloc<float>(int64_t(count) * outputs);
     last_mask.assign(count, 1);
@@ -906,11 +909,24 @@ static int ladder(int n, float slack) {
     return int(std::ceil(c));
 }
 
-std::vector<BrainSpec> Batch::slot_specs() const {
+std::vector<BrainSpec> Batch::slot_specs(const Tile &t) const {
+    // A brain that fits a pack keeps a slot that fits one: its rung is cut
+    // at the tile's pack limits (the smaller LDS limit, as the pool's
+    // plasticity is decided at build), so slack never sends it to the slower
+    // one-workgroup-per-brain path.
+    const TileShape *shape = find_tile(t);
+    const int nmax = t.nmax();
+    const int smax = t.ks > 0 ? t.threads * t.ks
+                              : shape ? std::min(shape->lds_plastic, shape->lds_static) : 0;
     std::vector<BrainSpec> out = specs;
     for (auto &sp : out) {
-        sp.cap_n = ladder(int(sp.brain->genes.size()), slack);
-        sp.cap_s = ladder(int(sp.brain->edges.size()), slack);
+        const int n = int(sp.brain->genes.size()), e = int(sp.brain->edges.size());
+        sp.cap_n = ladder(n, slack);
+        sp.cap_s = ladder(e, slack);
+        if (n <= nmax && e <= smax) {
+            sp.cap_n = std::min(sp.cap_n, nmax);
+            sp.cap_s = std::min(sp.cap_s, smax);
+        }
     }
     if (spare > 0 && count > 0) {
         // spare slots at evenly spaced quantiles of the slot sizes, plus two
@@ -1004,7 +1020,7 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
                 nst.insert(nst.end(), ost.begin() + noff[i] * 6, ost.begin() + noff[i + 1] * 6);
             }
         }
-        pool.build(slot_specs(), outputs, pool.tile, stream);
+        pool.build(slot_specs(pool.tile), outputs, pool.tile, stream);
         pool.upload_state(nn_, ns_, nh, ot, &nst);
         NS_HIP(hipStreamSynchronize(stream));
         ++relayouts;
@@ -1041,6 +1057,7 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
     stage.add(pool.view.brun, next.data(), size_t(pool.nb));
     pool.upload_stage(stage, st);
     pool.scatter_range(0, first_pass, st);
+    pool.order_touched(st);
     launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig, pool.d_sorig, st);
     NS_HIP(hipGetLastError());
     pool.scatter_range(first_pass, int(stage.entries.size()), st);
@@ -1055,7 +1072,7 @@ void Batch::relayout(const Tile &tile) {
     std::vector<int64_t> ticks;
     std::vector<uint64_t> stamps;
     pool.download_state(n, s, h, ticks, &stamps);
-    pool.build(slot_specs(), outputs, tile, stream);
+    pool.build(slot_specs(tile), outputs, tile, stream);
     pool.upload_state(n, s, h, ticks, &stamps);
     brun_valid = false;
 }
@@ -1408,7 +1425,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
         for (int k = 0; k < kNumTiles; ++k) {
             Tile t{kTiles[k].threads, kTiles[k].neurons_per_thread, kTiles[k].synapses_per_thread,
                    kTiles[k].min_waves, kTiles[k].synapses_per_thread > 0 ? 0 : kTiles[k].lds_plastic};
-            b.pool.build(b.slot_specs(), b.outputs, t, b.stream);
+            b.pool.build(b.slot_specs(t), b.outputs, t, b.stream);
             b.pool.upload_state(n, s, h, tk, &stamps);
             b.brun_valid = false;
             b.set_run(ones.data());
@@ -1426,7 +1443,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        b.pool.build(b.slot_specs(), b.outputs, best_tile, b.stream);
+        b.pool.build(b.slot_specs(best_tile), b.outputs, best_tile, b.stream);
         b.pool.upload_state(n, s, h, tk, &stamps);
         b.brun_valid = false;
         NS_HIP(hipMemcpy(b.pool.view.events, events0, sizeof events0, hipMemcpyHostToDevice));

@@ -40,6 +40,15 @@ constexpr uint32_t NF_ACTIVE = 1u << 8;
 constexpr uint32_t NF_RECEPTOR = 1u << 9;   // sensory input through receptor events
 constexpr uint32_t NF_WORLD = 1u << 10;     // inactive neurons keep their ring slot
 
+// Class of a neuron slot inside a pack: cortical AdEx (0), receptor-driven
+// (1), other AdEx (2), LFS (3), inactive (4).  Packs order their slots by
+// class (stable), so a wavefront mostly runs one integrator.
+constexpr int kSlotClasses = 5;
+__host__ __device__ inline int slot_class(uint32_t f) {
+    const uint32_t ty = f & NF_TYPE;
+    return !(f & NF_ACTIVE) ? 4 : ty == 0 ? 0 : ty == 5 ? 1 : ty == 6 ? 3 : 2;
+}
+
 // ---- synapse gene fields (float) ---------------------------------------------
 enum SynapseGeneField { SG_W0, SG_U, SG_REC, SG_FAC, SG_SREC, SG_SDEP, SG_FIELDS };
 // ---- synapse state fields (float) --------------------------------------------
@@ -485,15 +494,13 @@ __device__ inline float rng_uniform24(uint32_t &x) {    // physiology.cuh:380-39
 
 // synapse_release, physiology.cuh:425-454.  STP state in global memory,
 // touched only on arrival.  `weight` is the current synaptic weight.
-__device__ inline float release_at(const Pool &p, int64_t q, uint32_t flags, float weight,
-                                   int64_t tick, const uint64_t *etab) {
-    const int64_t S = p.scap;
-    const float U = p.sg[SG_U * S + q], rec = p.sg[SG_REC * S + q];
-    const float fac = p.sg[SG_FAC * S + q], srec = p.sg[SG_SREC * S + q];
-    float util = p.ss[SS_UTIL * S + q], slow = p.ss[SS_SLOW * S + q];
-    int64_t last = p.last[q];
-    uint64_t ready = p.ready[q];
-    uint32_t rng = p.srng[q];
+// synapse_release (physiology.cuh:425-454) on one synapse's release state,
+// wherever it lives (pool arrays or a workgroup's LDS copy).  sdep is read only
+// when the slow pool is on, as the reference does.
+__device__ inline float stp_release(uint32_t flags, float weight, int64_t tick, float U,
+                                    float rec, float fac, float srec, const float *sdep,
+                                    float &util, float &slow, int64_t &last, uint64_t &ready,
+                                    uint32_t &rng, const uint64_t *etab) {
     float p_recover = 0;
     if (last > -1000000) {
         float dt = (tick - last) * 1.0f;
@@ -513,10 +520,24 @@ __device__ inline float release_at(const Pool &p, int64_t q, uint32_t flags, flo
         }
     }
     float y = weight * out / n * slow;
-    if (srec > 0) slow *= 1.0f - p.sg[SG_SDEP * S + q];
+    if (srec > 0) slow *= 1.0f - *sdep;
+    last = tick;
+    return y;
+}
+
+__device__ inline float release_at(const Pool &p, int64_t q, uint32_t flags, float weight,
+                                   int64_t tick, const uint64_t *etab) {
+    const int64_t S = p.scap;
+    float util = p.ss[SS_UTIL * S + q], slow = p.ss[SS_SLOW * S + q];
+    int64_t last = p.last[q];
+    uint64_t ready = p.ready[q];
+    uint32_t rng = p.srng[q];
+    const float y = stp_release(flags, weight, tick, p.sg[SG_U * S + q], p.sg[SG_REC * S + q],
+                                p.sg[SG_FAC * S + q], p.sg[SG_SREC * S + q], &p.sg[SG_SDEP * S + q],
+                                util, slow, last, ready, rng, etab);
     p.ss[SS_UTIL * S + q] = util;
     p.ss[SS_SLOW * S + q] = slow;
-    p.last[q] = tick;
+    p.last[q] = last;
     p.ready[q] = ready;
     p.srng[q] = rng;
     return y;
@@ -529,15 +550,22 @@ struct PlastRegs {
     float weight, x, y, pre_age, post_age;
 };
 
+// State fields behind a profile's x and y registers (any profile value maps
+// to a valid field).
+__device__ inline int plast_x_field(int profile) {
+    return profile == 3 ? SS_ELTP : profile == 4 ? SS_DA_AGE : profile == 5 ? SS_SAGE : SS_PTRACE;
+}
+__device__ inline int plast_y_field(int profile) {
+    return profile == 3 ? SS_ELTD : profile == 5 ? SS_SPINE : SS_QTRACE;
+}
+
 __device__ inline void load_plast(const Pool &p, int64_t q, int profile, PlastRegs &r) {
     const int64_t S = p.scap;
     r.weight = p.ss[SS_WEIGHT * S + q];
     r.pre_age = p.ss[SS_PRE_AGE * S + q];
     r.post_age = p.ss[SS_POST_AGE * S + q];
-    int fx = profile == 3 ? SS_ELTP : profile == 4 ? SS_DA_AGE : profile == 5 ? SS_SAGE : SS_PTRACE;
-    int fy = profile == 3 ? SS_ELTD : profile == 5 ? SS_SPINE : SS_QTRACE;
-    r.x = p.ss[fx * S + q];
-    r.y = p.ss[fy * S + q];
+    r.x = p.ss[plast_x_field(profile) * S + q];
+    r.y = p.ss[plast_y_field(profile) * S + q];
 }
 
 __device__ inline void store_plast(const Pool &p, int64_t q, int profile, const PlastRegs &r) {
@@ -545,10 +573,8 @@ __device__ inline void store_plast(const Pool &p, int64_t q, int profile, const 
     p.ss[SS_WEIGHT * S + q] = r.weight;
     p.ss[SS_PRE_AGE * S + q] = r.pre_age;
     p.ss[SS_POST_AGE * S + q] = r.post_age;
-    int fx = profile == 3 ? SS_ELTP : profile == 4 ? SS_DA_AGE : profile == 5 ? SS_SAGE : SS_PTRACE;
-    int fy = profile == 3 ? SS_ELTD : profile == 5 ? SS_SPINE : SS_QTRACE;
-    p.ss[fx * S + q] = r.x;
-    if (profile != 4) p.ss[fy * S + q] = r.y;
+    p.ss[plast_x_field(profile) * S + q] = r.x;
+    if (profile != 4) p.ss[plast_y_field(profile) * S + q] = r.y;
 }
 
 __device__ inline float da_envelope(float age, const uint64_t *et) {   // physiology.cuh:408-412

@@ -195,7 +195,8 @@ public:
     std::vector<int32_t> occ_n, occ_s, pack_of, h_bout0;
     std::vector<uint8_t> slot_kind;  // 0 per-tick class, 1 own workgroup, 2 pack
     std::vector<uint32_t> h_nflags;  // host copies behind pack slot orders
-    std::vector<uint16_t> h_perm;
+    std::vector<int32_t> touched;   // packs place() changed, awaiting order_touched
+    int32_t *d_touched = nullptr;
     PinnedStaging staging;
     DeviceArena stage_arena;         // device copy of a staged placement
     unsigned char *d_stage = nullptr;
@@ -205,6 +206,9 @@ public:
     void upload_stage(const struct Stager &sg, hipStream_t st);
     std::vector<PlaceChunks> fill_list;   // images place() reserved, filled by upload_stage
     void scatter_range(int lo, int hi, hipStream_t st);
+    // Recompute on the device the slot orders of the packs the last place()
+    // filled; call after the staged images have been scattered.
+    void order_touched(hipStream_t st);
     // Stage the images of `moves` (slot, occupant or nullptr = vacate) into
     // `sg`: genes, flags, routes, endpoints, motors and the affected packs'
     // slot orders, and update the host-side slot tables.  State is left
@@ -291,7 +295,7 @@ struct Batch {
     void replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const uint32_t *seeds,
                  hipStream_t st);
     // Pool slots for the current brains under the reserve policy.
-    std::vector<BrainSpec> slot_specs() const;
+    std::vector<BrainSpec> slot_specs(const Tile &tile) const;
     void run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
              hipStream_t st);
     ~Batch();

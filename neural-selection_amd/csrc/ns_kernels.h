@@ -54,6 +54,10 @@ struct ScatterEntry {
     uint64_t dst, src, bytes;
 };
 void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st);
+// Stable class order of the neuron slots of packs[list[0..count)] into perm
+// (pack-relative indices at perm[pack.n0 ...]), as the pool build computes it.
+void launch_pack_order(const uint32_t *nflags, const Pack *packs, const int32_t *list, int count,
+                       uint16_t *perm, hipStream_t st);
 void launch_fill(float *x, int64_t n, float v, hipStream_t st);
 void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st);
 void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,

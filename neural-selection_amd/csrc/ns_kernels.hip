@@ -112,6 +112,52 @@ __global__ void k_scatter(const ScatterEntry *entries, int n, const unsigned cha
     }
 }
 
+// Stable counting sort of one pack's neuron slots by slot_class: per chunk of
+// 1024 slots, each class's ballot gives a slot its rank within its wavefront
+// and the wavefront's count; earlier wavefronts and chunks add their counts.
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_pack_order(const uint32_t *nflags,
+                                                              const Pack *packs,
+                                                              const int32_t *list,
+                                                              uint16_t *perm) {
+    constexpr int W = kOrderThreads / 64;
+    __shared__ int total[kSlotClasses], base[kSlotClasses], wcnt[W][kSlotClasses];
+    const Pack pk = packs[list[blockIdx.x]];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < kSlotClasses) total[tid] = 0;
+    __syncthreads();
+    for (int j = tid; j < pk.nn; j += kOrderThreads) atomicAdd(&total[slot_class(nflags[pk.n0 + j])], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int c = 0; c < kSlotClasses; ++c) { base[c] = s; s += total[c]; }
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < pk.nn; j0 += kOrderThreads) {   // uniform trip count
+        const int j = j0 + tid;
+        const int c = j < pk.nn ? slot_class(nflags[pk.n0 + j]) : -1;
+        int rank = 0;
+        for (int k = 0; k < kSlotClasses; ++k) {
+            const uint64_t m = __ballot(c == k);
+            if (lane == 0) wcnt[wave][k] = __popcll(m);
+            if (c == k) rank = __popcll(m & ((uint64_t(1) << lane) - 1));
+        }
+        __syncthreads();
+        if (c >= 0) {
+            int pos = base[c] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][c];
+            perm[pk.n0 + pos] = uint16_t(j);
+        }
+        __syncthreads();
+        if (tid < kSlotClasses) {
+            int s = 0;
+            for (int w = 0; w < W; ++w) s += wcnt[w][tid];
+            base[tid] += s;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_set_ticks(Pool p, int nb, int64_t add, int64_t set, uint8_t need) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !need_ok(p, b, need)) return;
@@ -746,9 +792,16 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     }
     const Pack pk = packs[blockIdx.x - nbig];
     const int tid = threadIdx.x;
-    {   // a pack whose brains all sit out this call (free slots, masks) does nothing
+    {   // the pack's brains; a pack whose brains all sit out this call (free
+        // slots, masks) does nothing
         int any = 0;
-        for (int b = tid; b < pk.nb; b += WG) any |= p.brun[pk.b0 + b] & 1;
+        for (int b = tid; b < pk.nb; b += WG) {
+            const uint8_t run = p.brun[pk.b0 + b] & 1;
+            btick_l[b] = p.btick[pk.b0 + b];
+            brun_l[b] = run;
+            bfrozen_l[b] = p.bfrozen[pk.b0 + b];
+            any |= run;
+        }
         if (!__syncthreads_or(any)) return;
     }
     const int64_t N = p.ncap, S = p.scap;
@@ -769,12 +822,17 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     uint8_t *sbr_l = reinterpret_cast<uint8_t *>(queue_l + smax);
     uint8_t *arr_l = sbr_l + smax;
 
-    if (tid < 32) etab[tid] = g_exp_tab[tid];
-    for (int b = tid; b < pk.nb; b += WG) {
-        btick_l[b] = p.btick[pk.b0 + b];
-        brun_l[b] = p.brun[pk.b0 + b] & 1;
-        bfrozen_l[b] = p.bfrozen[pk.b0 + b];
+    // Setup issues its loads in as few dependent rounds as it can, with no
+    // barrier between them (the run flags are already in LDS): slot order,
+    // neuron and synapse fields, then neuron registers with the flag-
+    // dependent synapse traces, then the input level.
+    int nid[KN];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+        const int slot = k * WG + tid;
+        nid[k] = slot < pk.nn ? int(perm[n0 + slot]) : pk.nn;
     }
+    if (tid < 32) etab[tid] = g_exp_tab[tid];
     for (int n = tid; n < pk.nn; n += WG) {
         hist_l[n] = p.hist[n0 + n];
         in_l[n] = uint16_t(p.nin[n0 + n] - s0);
@@ -792,24 +850,43 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         mot_l[m] = mn >= 0 ? uint16_t(mn - n0) : kNoMotor;
         cnt_l[m] = 0;
     }
-    __syncthreads();
-    for (int q = tid; q < ns; q += WG) {
-        const int64_t gq = s0 + q;
-        const uint8_t b = uint8_t(p.sbrain[gq] - pk.b0);
-        uint32_t f = p.sflags[gq];
-        if (!brun_l[b]) f &= ~(R_VALID << SF_ROUTE_SHIFT);
-        fl_l[q] = f;
-        sbr_l[q] = b;
-        pp_l[q] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << 16);
-        w0_l[q] = p.sg[SG_W0 * S + gq];
-        rel_l[q] = 0.0f;
-        arr_l[q] = 0;
-        if (PLASTIC) {
-            PlastRegs r;
-            load_plast(p, gq, (f >> SF_PLAST_SHIFT) & 7, r);
-            w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
-        } else {
-            w_l[q] = p.ss[SS_WEIGHT * S + gq];
+    // Synapse fields that do not depend on the flag word, SU synapses per
+    // thread per round (a slot past the pack reloads the round's first
+    // synapse; stores are guarded).  A synapse of a brain that sits out loses
+    // its valid bit.
+    constexpr int SU = 4;
+    for (int q0 = tid; q0 < ns; q0 += SU * WG) {
+        uint32_t f[SU], pp[SU];
+        int b[SU];
+        float w0[SU], w[SU], pa[SU], qa[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int64_t gq = s0 + (q0 + u * WG < ns ? q0 + u * WG : q0);
+            b[u] = p.sbrain[gq] - pk.b0;
+            f[u] = p.sflags[gq];
+            pp[u] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << 16);
+            w0[u] = p.sg[SG_W0 * S + gq];
+            w[u] = p.ss[SS_WEIGHT * S + gq];
+            if (PLASTIC) {
+                pa[u] = p.ss[SS_PRE_AGE * S + gq];
+                qa[u] = p.ss[SS_POST_AGE * S + gq];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int q = q0 + u * WG;
+            if (q >= ns) break;
+            fl_l[q] = brun_l[b[u]] ? f[u] : f[u] & ~(R_VALID << SF_ROUTE_SHIFT);
+            sbr_l[q] = uint8_t(b[u]);
+            pp_l[q] = pp[u];
+            w0_l[q] = w0[u];
+            w_l[q] = w[u];
+            rel_l[q] = 0.0f;
+            arr_l[q] = 0;
+            if (PLASTIC) {
+                pa_l[q] = pa[u];
+                qa_l[q] = qa[u];
+            }
         }
     }
 
@@ -817,26 +894,52 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     NeuronRegs nr[KN];
     NeuronConst nc[KN];
     float nscale[KN], ncur[KN];
-    int nbr[KN], nid[KN];
+    int nbr[KN], nidx[KN];
     bool nlive[KN];
 #pragma unroll
     for (int k = 0; k < KN; ++k) {
-        const int slot = k * WG + tid;
+        const int n = nid[k];
         nlive[k] = false;
         nbr[k] = 0;
         nscale[k] = 0;
-        ncur[k] = 0;
-        nid[k] = slot < pk.nn ? int(perm[n0 + slot]) : pk.nn;
-        const int n = nid[k];
-        if (slot < pk.nn) {
+        nidx[k] = -1;
+        if (n < pk.nn) {
             const int64_t i = n0 + n;
             nbr[k] = p.nbrain[i] - pk.b0;
             load_neuron(p, i, nr[k], nc[k], g_exp_tab);
+            nscale[k] = p.ns[NS_LAS * N + i];
+            nidx[k] = p.ninput[i];
+        }
+    }
+    // The profile-selected traces of each plastic synapse (the flag word
+    // names the fields), thread-private LDS entries written above.
+    if (PLASTIC) {
+        for (int q0 = tid; q0 < ns; q0 += SU * WG) {
+            float x[SU], y[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int q = q0 + u * WG < ns ? q0 + u * WG : q0;
+                const int profile = (fl_l[q] >> SF_PLAST_SHIFT) & 7;
+                x[u] = p.ss[plast_x_field(profile) * S + s0 + q];
+                y[u] = p.ss[plast_y_field(profile) * S + s0 + q];
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int q = q0 + u * WG;
+                if (q >= ns) break;
+                x_l[q] = x[u];
+                y_l[q] = y[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+        ncur[k] = 0;
+        if (nid[k] < pk.nn) {
             nlive[k] = brun_l[nbr[k]] != 0;
-            nscale[k] = ns_expf(p.ns[NS_LAS * N + i], g_exp_tab);
-            int32_t idx = p.ninput[i];
-            ncur[k] = idx >= 0 ? input[idx] : -1.0f;
-            if (idx < 0) nc[k].flags |= 1u << 30;
+            nscale[k] = ns_expf(nscale[k], g_exp_tab);
+            ncur[k] = nidx[k] >= 0 ? input[nidx[k]] : -1.0f;
+            if (nidx[k] < 0) nc[k].flags |= 1u << 30;
         }
     }
     __syncthreads();
@@ -1040,6 +1143,12 @@ void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t 
 
 void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_scatter, dim3(n), dim3(256), 0, st, entries, n, base);
+}
+
+void launch_pack_order(const uint32_t *nflags, const Pack *packs, const int32_t *list, int count,
+                       uint16_t *perm, hipStream_t st) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_pack_order, dim3(count), dim3(kOrderThreads), 0, st, nflags, packs, list, perm);
 }
 
 void launch_fill(float *x, int64_t n, float v, hipStream_t st) {

@@ -259,6 +259,7 @@ struct Server {
         stage.add(pool.view.brun, next.data(), size_t(pool.nb));
         pool.upload_stage(stage, stream);
         pool.scatter_range(0, first, stream);
+        pool.order_touched(stream);
         if (!fresh.empty())
             launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig,
                          pool.d_sorig, stream);
