@@ -512,6 +512,7 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     up(d_perm, h_perm);
     d_touched = arena.alloc<int32_t>(std::max<size_t>(packs.size(), 1));
     touched.clear();
+    d_reset = arena.alloc<int32_t>(2 * int64_t(std::max(nb, 1)));
     d_packs = arena.alloc<Pack>(std::max<size_t>(packs.size(), 1));
     if (!packs.empty()) up(d_packs, packs);
     const int64_t fb_n = fallback_brains ? int64_t(bn0[fallback_brains - 1]) + bnc[fallback_brains - 1] : 0;
@@ -614,20 +615,49 @@ void DevicePool::upload_stage(const Stager &sg, hipStream_t st) {
         for (auto &th : workers) th.join();
     }
     fill_list.clear();
-    if (d_stage_cap < bytes) {
-        NS_HIP(hipStreamSynchronize(st));
-        stage_arena.release();
-        d_stage_cap = std::max(bytes + bytes / 2, size_t(1) << 20);
-        d_stage = stage_arena.alloc<unsigned char>(int64_t(d_stage_cap));
+    if (!copy_stream) {
+        NS_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+        for (auto &d : dstage) {
+            NS_HIP(hipEventCreateWithFlags(&d.uploaded, hipEventDisableTiming));
+            NS_HIP(hipEventCreateWithFlags(&d.read, hipEventDisableTiming));
+            NS_HIP(hipEventRecord(d.read, copy_stream));   // a first wait finds it complete
+        }
     }
-    NS_HIP(hipMemcpyAsync(d_stage, buf.p, bytes, hipMemcpyHostToDevice, st));
-    NS_HIP(hipEventRecord(buf.done, st));
+    stage_cur = stage_next;
+    stage_next ^= 1;
+    DeviceStage &d = dstage[stage_cur];
+    if (d.cap < bytes) {
+        NS_HIP(hipEventSynchronize(d.read));   // no queued scatter reads the old buffer
+        d.arena.release();
+        d.cap = std::max(2 * bytes, kStageMinBytes);
+        d.p = d.arena.alloc<unsigned char>(int64_t(d.cap));
+    }
+    NS_HIP(hipStreamWaitEvent(copy_stream, d.read, 0));
+    NS_HIP(hipMemcpyAsync(d.p, buf.p, bytes, hipMemcpyHostToDevice, copy_stream));
+    NS_HIP(hipEventRecord(buf.done, copy_stream));
+    NS_HIP(hipEventRecord(d.uploaded, copy_stream));
+    NS_HIP(hipStreamWaitEvent(st, d.uploaded, 0));
 }
 
 void DevicePool::scatter_range(int lo, int hi, hipStream_t st) {
     if (hi <= lo) return;
-    launch_scatter(reinterpret_cast<const ScatterEntry *>(d_stage) + lo, hi - lo, d_stage, st);
+    DeviceStage &d = dstage[stage_cur];
+    launch_scatter(reinterpret_cast<const ScatterEntry *>(d.p) + lo, hi - lo, d.p, st);
     NS_HIP(hipGetLastError());
+    NS_HIP(hipEventRecord(d.read, st));
+}
+
+DevicePool::~DevicePool() {
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    for (auto &d : dstage) {
+        if (d.read) {
+            (void)hipEventSynchronize(d.read);
+            (void)hipEventDestroy(d.read);
+        }
+        if (d.uploaded) (void)hipEventDestroy(d.uploaded);
+    }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    arena.release();
 }
 
 void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &moves, Stager &sg) {
@@ -1037,30 +1067,26 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
         pool.place(moves, stage);
         placements += k;
     }
-    // Fresh state for the newborns (initial states seeded as ns_gpu_reset),
-    // then the run flags of the last mask under the new slot mapping; one
-    // upload carries the images and both flag sets.
-    std::vector<uint8_t> run(pool.nb, 0), next(pool.nb, 0);
-    std::vector<uint32_t> bseed(pool.nb, 0);
+    // Fresh state for the newborns' slots (initial states seeded as
+    // ns_gpu_reset) and the run flags of the last mask under the new slot
+    // mapping, in the same upload as the images.
+    std::vector<int32_t> born(2 * size_t(k));
+    std::vector<uint8_t> next(pool.nb, 0);
     for (int j = 0; j < k; ++j) {
-        const int pb = pool.pool_of[idx[j]];
-        run[pb] = uint8_t(1 | (pool.fallback[pb] ? 2 : 0));
-        bseed[pb] = seeds ? seeds[j] : 0u;
+        born[j] = pool.pool_of[idx[j]];
+        born[k + j] = int32_t(seeds ? seeds[j] : 0u);
     }
     for (int pb = 0; pb < pool.nb; ++pb) {
         const int i = pool.batch_of[pb];
         next[pb] = uint8_t((i >= 0 && last_mask[i] ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
     }
-    stage.add(pool.view.brun, run.data(), size_t(pool.nb));
-    stage.add(pool.d_bseed, bseed.data(), 4u * pool.nb);
-    const int first_pass = int(stage.entries.size());
+    stage.add(pool.d_reset, born.data(), 4u * born.size());
     stage.add(pool.view.brun, next.data(), size_t(pool.nb));
     pool.upload_stage(stage, st);
-    pool.scatter_range(0, first_pass, st);
+    pool.scatter_range(0, int(stage.entries.size()), st);
     pool.order_touched(st);
-    launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig, pool.d_sorig, st);
+    launch_reset_slots(pool.view, pool.d_reset, k, pool.d_norig, pool.d_sorig, st);
     NS_HIP(hipGetLastError());
-    pool.scatter_range(first_pass, int(stage.entries.size()), st);
     h_brun = next;
     brun_valid = true;
 }

@@ -116,6 +116,10 @@ struct PlaceChunks {
     int ninx = 0;
 };
 
+// Staging buffers start at this size and double when outgrown (a regrowth
+// waits for the buffer's last use).
+constexpr size_t kStageMinBytes = size_t(16) << 20;
+
 // Pinned host staging for slot placements, double buffered so one buffer can
 // be filled while the other's copy is still in flight.
 struct PinnedStaging {
@@ -130,8 +134,9 @@ struct PinnedStaging {
             if (b.p) (void)hipHostFree(b.p);
             b.p = nullptr;
             b.cap = 0;
-            NS_HIP(hipHostMalloc(&b.p, bytes + bytes / 2));
-            b.cap = bytes + bytes / 2;
+            const size_t cap = std::max(2 * bytes, kStageMinBytes);
+            NS_HIP(hipHostMalloc(&b.p, cap));
+            b.cap = cap;
         }
         if (!b.done) NS_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
         return b;
@@ -156,7 +161,7 @@ const TileShape *find_tile(const Tile &t);
 // Structure-of-arrays device pool over a set of brains, laid out in packs.
 class DevicePool {
 public:
-    ~DevicePool() { arena.release(); }
+    ~DevicePool();
     void build(const std::vector<BrainSpec> &specs, int outputs, const Tile &tile,
                hipStream_t stream);
 
@@ -198,11 +203,22 @@ public:
     std::vector<int32_t> touched;   // packs place() changed, awaiting order_touched
     int32_t *d_touched = nullptr;
     PinnedStaging staging;
-    DeviceArena stage_arena;         // device copy of a staged placement
-    unsigned char *d_stage = nullptr;
-    size_t d_stage_cap = 0;
-    // Copy the staged buffer to the device on `st` (one upload); entries
-    // [lo, hi) are then scattered by scatter_range.
+    // Device copies of staged placements, double buffered: one is uploaded
+    // on copy_stream (overlapping work already queued on the caller's
+    // stream) while scatters of the other may still be queued.
+    struct DeviceStage {
+        DeviceArena arena;
+        unsigned char *p = nullptr;
+        size_t cap = 0;
+        hipEvent_t uploaded = nullptr;   // on copy_stream, after its upload
+        hipEvent_t read = nullptr;       // after the last scatter that read it
+    };
+    DeviceStage dstage[2];
+    int stage_cur = 0, stage_next = 0;
+    hipStream_t copy_stream = nullptr;   // default priority
+    // Copy the staged buffer to the device (one upload on copy_stream, which
+    // `st` then waits for); entries [lo, hi) are then scattered on `st` by
+    // scatter_range.
     void upload_stage(const struct Stager &sg, hipStream_t st);
     std::vector<PlaceChunks> fill_list;   // images place() reserved, filled by upload_stage
     void scatter_range(int lo, int hi, hipStream_t st);
@@ -215,6 +231,7 @@ public:
     // alone (the caller resets newborns).
     void place(const std::vector<std::pair<int, const BrainSpec *>> &moves, struct Stager &sg);
     uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
+    int32_t *d_reset = nullptr;      // 2 x nb: slots to reset, then their seeds
     hipStream_t stream = nullptr;
 };
 

@@ -48,6 +48,10 @@ extern const int kNumTiles;
 
 void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t *bseed,
                   const uint32_t *norig, const uint32_t *sorig, hipStream_t st);
+// Reset of the slots list[0..count) with seeds list[count..2 count) (device
+// memory), as launch_reset resets a run-flagged brain.
+void launch_reset_slots(const Pool &p, const int32_t *list, int count, const uint32_t *norig,
+                        const uint32_t *sorig, hipStream_t st);
 // One staged copy of a slot placement: `bytes` bytes from offset `src` of the
 // staged buffer to device address `dst`.
 struct ScatterEntry {

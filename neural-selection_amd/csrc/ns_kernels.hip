@@ -38,11 +38,10 @@ __device__ inline uint32_t mix_seed(uint32_t seed) {   // physiology.cuh:183-186
 // Reset: initial_neuron_state / initial_synapse_state (physiology.cuh:179-202,
 // 393-404) seeded as gym_adapter/gpu.cu:18-23.  need = 1: brains with brun&1.
 // ---------------------------------------------------------------------------
-__global__ void k_reset_neurons(Pool p, int64_t n, const uint32_t *bseed, const uint32_t *orig) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t b = p.nbrain[i];
-    if (!(p.brun[b] & 1)) return;
+// Initial state of neuron i of a brain reset with `seed` (initial_neuron_state,
+// physiology.cuh:179-202, and the cell initial values of lp_cell.cuh /
+// prinz_cell.cuh); `orig` is its original genome index.
+__device__ inline void reset_neuron(const Pool &p, int64_t i, uint32_t seed, uint32_t orig) {
     const int64_t N = p.ncap;
     uint32_t type = p.nflags[i] & NF_TYPE;
     float rest = p.ng[NG_REST * N + i];
@@ -64,7 +63,7 @@ __global__ void k_reset_neurons(Pool p, int64_t n, const uint32_t *bseed, const 
     }
     for (int k = 0; k < kCellFields; ++k) p.cell[k * N + i] = cell[k];
     p.ns[NS_V * N + i] = v;
-    uint32_t x = mix_seed(bseed[b] + orig[i] * 7919u);
+    uint32_t x = mix_seed(seed + orig * 7919u);
     p.rrng[i] = x ? x : 1;
     p.total[i] = 0;
     p.spike[i] = 0;
@@ -73,13 +72,10 @@ __global__ void k_reset_neurons(Pool p, int64_t n, const uint32_t *bseed, const 
     for (int k = 0; k < 6; ++k) p.stamps[k * N + i] = 0;
 }
 
-__global__ void k_reset_synapses(Pool p, int64_t n, const uint32_t *bseed, const uint32_t *orig) {
-    int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    int32_t b = p.sbrain[q];
-    if (!(p.brun[b] & 1)) return;
+// initial_synapse_state (physiology.cuh:393-404) of synapse q.
+__device__ inline void reset_synapse(const Pool &p, int64_t q, uint32_t seed, uint32_t orig) {
     const int64_t S = p.scap;
-    uint32_t x = mix_seed(bseed[b] + orig[q] * 104729u);
+    uint32_t x = mix_seed(seed + orig * 104729u);
     if (!x) x = 1;
     float init[SS_FIELDS] = {};
     init[SS_WEIGHT] = p.sg[SG_W0 * S + q];
@@ -92,6 +88,37 @@ __global__ void k_reset_synapses(Pool p, int64_t n, const uint32_t *bseed, const
     p.srng[q] = x;
     p.arrived[q] = 0;
     p.released[q] = 0;
+}
+
+// Reset of every brain whose run flag is set (seeds per pool brain).
+__global__ void k_reset_neurons(Pool p, int64_t n, const uint32_t *bseed, const uint32_t *orig) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t b = p.nbrain[i];
+    if (!(p.brun[b] & 1)) return;
+    reset_neuron(p, i, bseed[b], orig[i]);
+}
+
+__global__ void k_reset_synapses(Pool p, int64_t n, const uint32_t *bseed, const uint32_t *orig) {
+    int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int32_t b = p.sbrain[q];
+    if (!(p.brun[b] & 1)) return;
+    reset_synapse(p, q, bseed[b], orig[q]);
+}
+
+// Reset of the listed slots only: list[0..count) slot indices, list[count..2
+// count) their seeds.  One workgroup per slot, over its whole capacity, and
+// its tick counter back to zero.
+__global__ void k_reset_slots(Pool p, const int32_t *list, int count, const uint32_t *norig,
+                              const uint32_t *sorig) {
+    const int b = list[blockIdx.x];
+    const uint32_t seed = uint32_t(list[count + blockIdx.x]);
+    const int64_t n0 = p.bn0[b], n1 = n0 + p.bnc[b];
+    const int64_t s0 = p.bs0[b], s1 = s0 + p.bsc[b];
+    for (int64_t i = n0 + threadIdx.x; i < n1; i += blockDim.x) reset_neuron(p, i, seed, norig[i]);
+    for (int64_t q = s0 + threadIdx.x; q < s1; q += blockDim.x) reset_synapse(p, q, seed, sorig[q]);
+    if (threadIdx.x == 0) p.btick[b] = 0;
 }
 
 // Slot placement: each entry copies `bytes` from the staged image (device
@@ -1139,6 +1166,12 @@ void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t 
     if (nn) hipLaunchKernelGGL(k_reset_neurons, dim3(blocks_for(nn, 256)), dim3(256), 0, st, p, nn, bseed, norig);
     if (ns) hipLaunchKernelGGL(k_reset_synapses, dim3(blocks_for(ns, 256)), dim3(256), 0, st, p, ns, bseed, sorig);
     hipLaunchKernelGGL(k_set_ticks, dim3(blocks_for(nb, 256)), dim3(256), 0, st, p, nb, int64_t(0), int64_t(0), uint8_t(1));
+}
+
+void launch_reset_slots(const Pool &p, const int32_t *list, int count, const uint32_t *norig,
+                        const uint32_t *sorig, hipStream_t st) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_reset_slots, dim3(count), dim3(256), 0, st, p, list, count, norig, sorig);
 }
 
 void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st) {

@@ -235,39 +235,34 @@ struct Server {
         std::vector<std::pair<int, const BrainSpec *>> moves;
         for (Episode *e : fresh) moves.push_back({e->pb, &e->spec});
         pool.place(moves, stage);
-        std::vector<uint8_t> run(pool.nb, 0), next(pool.nb, 0);
-        std::vector<uint32_t> bseed(pool.nb, 0);
-        std::vector<float> task(size_t(std::max<size_t>(fresh.size(), 1)) * kTaskFloats);
-        std::vector<uint32_t> rng(std::max<size_t>(fresh.size(), 1));
-        for (size_t j = 0; j < fresh.size(); ++j) {
+        const int k = int(fresh.size());
+        std::vector<uint8_t> next(pool.nb, 0);
+        std::vector<int32_t> born(2 * size_t(k));   // slots to reset, then their seeds
+        std::vector<float> task(size_t(std::max(k, 1)) * kTaskFloats);
+        std::vector<uint32_t> rng(std::max(k, 1));
+        for (int j = 0; j < k; ++j) {
             const Episode &e = *fresh[j];
-            run[e.pb] = 1;
-            bseed[e.pb] = e.req.seed;
+            born[j] = e.pb;
+            born[k + j] = int32_t(e.req.seed);
             make_task(e.req.task_seed, &task[j * kTaskFloats]);
             rng[j] = e.req.task_seed * 2654435761u + 12345u;
             if (!rng[j]) rng[j] = 1;
         }
         for (int pb = 0; pb < pool.nb; ++pb) next[pb] = live[pb] ? 1 : 0;
-        stage.add(pool.view.brun, run.data(), size_t(pool.nb));
-        stage.add(pool.d_bseed, bseed.data(), 4u * pool.nb);
-        for (size_t j = 0; j < fresh.size(); ++j) {
+        if (k) stage.add(pool.d_reset, born.data(), 4u * born.size());
+        for (int j = 0; j < k; ++j) {
             const int pb = fresh[j]->pb;
             stage.add(d_task + size_t(pb) * kTaskFloats, &task[j * kTaskFloats], 4u * kTaskFloats);
             stage.add(d_rng + pb, &rng[j], 4);
         }
-        const int first = int(stage.entries.size());
         stage.add(pool.view.brun, next.data(), size_t(pool.nb));
+        // the staged vectors are copied into pinned memory by upload_stage
         pool.upload_stage(stage, stream);
-        pool.scatter_range(0, first, stream);
+        pool.scatter_range(0, int(stage.entries.size()), stream);
         pool.order_touched(stream);
-        if (!fresh.empty())
-            launch_reset(pool.view, pool.nn, pool.ns, pool.nb, pool.d_bseed, pool.d_norig,
-                         pool.d_sorig, stream);
-        pool.scatter_range(first, int(stage.entries.size()), stream);
+        launch_reset_slots(pool.view, pool.d_reset, k, pool.d_norig, pool.d_sorig, stream);
         NS_HIP(hipGetLastError());
         h_run = next;
-        // the staged copies of run/next/task/rng must outlive the upload
-        NS_HIP(hipStreamSynchronize(stream));
     }
 
     void steps(int n) {
