@@ -204,6 +204,82 @@ def bench_serving(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_mazes(args):
+    """C5: many concurrent maze requests (sizes log-uniform 64..2048 neurons),
+    each `--mazes` depth-first mazes of at most `--maze-horizon` steps with the
+    brain's state kept; all submitted at once.  Throughput counts the steps
+    each request took until its last maze ended."""
+    import ctypes as ct
+    L = ns_amd.lib()
+    rng = np.random.default_rng(23)
+    reqs = []
+    for i in range(args.maze_requests):
+        size = int(np.exp(rng.uniform(np.log(64), np.log(2048))))
+        nr, sr, mo = W.maze_genome(rng, size)
+        reqs.append((nr, sr, mo, 3000 + i, 9000 + i))
+    classes = []
+    c = 16.0
+    while True:                            # the server's slot classes
+        classes.append(int(np.ceil(c)))
+        if classes[-1] >= 2048:
+            break
+        c *= np.sqrt(2.0)
+    per = int(np.ceil(args.maze_requests / 10)) + 64   # sizes fall in about 10 classes
+    cfg = ns_amd.ServeConfig(neuron_capacity=per * sum(classes), synapse_capacity=4 * per * sum(classes),
+                             episode_capacity=per * len(classes), steps_per_episode=1,
+                             ticks_per_step=16, steps_per_launch=25, noise_sigma=0.0,
+                             input_gain=230.0, max_episode_neurons=2048, task=ns_amd.TASK_MAZE,
+                             maze_episodes=args.mazes, maze_horizon=args.maze_horizon)
+    t_build = time.perf_counter()
+    srv = L.ns_serve_create(ct.byref(cfg))
+    if not srv:
+        raise SystemExit(ns_amd.error())
+    log(f'mazes: slot pool for {per} requests per class built in {time.perf_counter() - t_build:.1f} s; '
+        f'submitting {len(reqs)} requests x {args.mazes} mazes')
+    f = ct.POINTER(ct.c_float)
+    t0 = time.perf_counter()
+    for nr, sr, mo, seed, tseed in reqs:
+        if L.ns_serve_submit(srv, len(nr), len(sr), nr.ctypes.data_as(f), sr.ctypes.data_as(f),
+                             mo.ctypes.data_as(ct.POINTER(ct.c_int)), seed, tseed) < 0:
+            raise SystemExit(ns_amd.error())
+    if L.ns_serve_drain(srv):
+        raise SystemExit(ns_amd.error())
+    wall = time.perf_counter() - t0
+    res = (ns_amd.ServeResult * len(reqs))()
+    n = L.ns_serve_poll(srv, res, len(reqs))
+    stats = ns_amd.ServeStats()
+    L.ns_serve_stats_get(srv, ct.byref(stats))
+    L.ns_serve_destroy(srv)
+    order = [r for r in res[:n]]
+    steps = np.array([r.final_state[1] for r in order])
+    nu = float(sum(len(reqs[r.id][0]) * r.final_state[1] * 16 for r in order))
+    lat = np.array([r.finish_s - r.submit_s for r in order]) * 1e3
+    solved = np.array([r.final_state[0] for r in order])
+    log(f'  {n} requests in {wall:.2f} s: {nu / wall:.3e} neuron-updates/s, '
+        f'latency p50 {np.percentile(lat, 50):.0f} ms p99 {np.percentile(lat, 99):.0f} ms')
+    line = {
+        'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
+        'value': nu / wall, 'unit': 'neuron-updates/s', 'n_gpus': 1,
+        'steps': int(steps.max()) if n else 0, 'warmup': 0,
+        'ms_per_step': wall * 1e3 / max(float(steps.max()) if n else 1.0, 1.0),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic (seeded maze genomes and mazes)',
+        'config': {'workload': 'c5', 'desc': f'{len(reqs)} concurrent maze requests, sizes log-uniform '
+                   f'64..2048 neurons, {args.mazes} depth-first 31x31 mazes each (state kept), '
+                   f'horizon {args.maze_horizon} steps x 16 ticks, reward as a dopamine event',
+                   'requests': len(reqs), 'neurons_total': int(sum(len(r[0]) for r in reqs))},
+        'serving': {'requests': n, 'wall_s': wall, 'requests_per_s': n / wall,
+                    'latency_ms': {'p50': float(np.percentile(lat, 50)),
+                                   'p99': float(np.percentile(lat, 99)), 'max': float(lat.max())},
+                    'env_steps': {'mean': float(steps.mean()), 'max': float(steps.max())},
+                    'mazes_solved': {'total': int(solved.sum()), 'requests_solving_any': int((solved > 0).sum())},
+                    'launch_rounds': stats.launches, 'resident_peak': stats.resident_peak,
+                    'device_busy_s': stats.busy_s},
+        'roofline': None, 'cpu_baseline': None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -219,6 +295,9 @@ def main():
     ap.add_argument('--rates', default='100,1000,5000', help='c4: Poisson arrival rates (1/s)')
     ap.add_argument('--episodes', type=int, default=300, help='c4: episodes per rate')
     ap.add_argument('--episode-steps', type=int, default=500, help='c4: env steps per episode')
+    ap.add_argument('--maze-requests', type=int, default=8192, help='c5: concurrent maze requests')
+    ap.add_argument('--mazes', type=int, default=10, help='c5: mazes per request')
+    ap.add_argument('--maze-horizon', type=int, default=100, help='c5: steps per maze at most')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy MIB MiB device to device (HBM counter calibration)')
     args = ap.parse_args()
@@ -234,10 +313,13 @@ def main():
         return
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (asked for {args.gpus}); brains are independent, '
-              'so the update shards by brain with no exchange (see DESIGN.md)', file=sys.stderr)
+              'so the update shards by brain with no exchange', file=sys.stderr)
         sys.exit(2)
     if args.workload == 'c4':
         bench_serving(args)
+        return
+    if args.workload == 'c5':
+        bench_mazes(args)
         return
     import torch
     torch.cuda.init()

@@ -320,9 +320,20 @@ int ns_world_homeostasis(void *pool, const int *age, const uint8_t *frozen);
 /* 3. Serving: continuous batching of genome-evaluation episodes            */
 /* ------------------------------------------------------------------------ */
 
-/* Episode task: a seeded 4-d linear system x' = A x + B u + N(0, sigma^2)
- * observed through an 8-sensor opponent rate code; two motor neurons drive
- * u.  One device pool of slots holds every episode: size classes
+/* Episode tasks (config.task):
+ *  0  a seeded 4-d linear system x' = A x + B u + N(0, sigma^2) observed
+ *     through an 8-sensor opponent rate code; two motor neurons drive u;
+ *     steps_per_episode steps.  Result: return = -sum |x|^2, final_state = x,
+ *     motor_spikes = the two motors' totals.
+ *  1  maze_episodes (0: 10) seeded 31 x 31 depth-first mazes in a row with
+ *     the brain's state kept throughout; 9 channels (walls N/E/S/W, goal
+ *     E/W/S/N of the agent, reward) and 4 motors (move N/E/S/W: most spikes,
+ *     ties to the lower motor, none = stay).  A maze ends at the goal (reward
+ *     1 - 0.9 steps / horizon, gym_adapter/maze_gpu.cu:46, and the reward
+ *     channel on for the next step) or after maze_horizon (0: 100) steps.
+ *     Result: return = sum of rewards, final_state = {mazes solved, steps,
+ *     mazes played, 0}, motor_spikes = totals of motors 0 and 1.
+ * One device pool of slots holds every episode: size classes
  * ceil(16 * sqrt2^k) up to max_episode_neurons, the same number of slots per
  * class (at least one) within neuron_capacity and episode_capacity, synapse
  * capacity per slot in the ratio synapse_capacity / neuron_capacity (clamped
@@ -337,14 +348,17 @@ typedef struct ns_serve_config {
     float noise_sigma;
     float input_gain;
     int max_episode_neurons;    /* largest slot class (0: 4096) */
+    int task;                   /* 0 linear system, 1 mazes */
+    int maze_episodes;          /* mazes per request (0: 10) */
+    int maze_horizon;           /* steps per maze (0: 100) */
 } ns_serve_config;
 
 void *ns_serve_create(const ns_serve_config *config);
 void ns_serve_destroy(void *server);
 
-/* Queue one episode (genome in ns_create encoding with channels = 8 and
- * outputs = 2).  Returns a request id >= 0.  Thread-safe with respect to
- * the serving thread. */
+/* Queue one episode (genome in ns_create encoding with the task's channels
+ * and outputs: 8 and 2 for the linear system, 9 and 4 for mazes).  Returns a
+ * request id >= 0.  Thread-safe with respect to the serving thread. */
 int64_t ns_serve_submit(void *server, int count, int edges,
                         const float *neuron_genes, const float *synapse_genes,
                         const int *motors, uint32_t seed, uint32_t task_seed);
@@ -370,6 +384,11 @@ typedef struct ns_serve_stats {
     double busy_s;
 } ns_serve_stats;
 int ns_serve_stats_get(void *server, ns_serve_stats *stats);
+
+/* Maze `index` of a maze request seeded `seed`, as the serving loop carves
+ * it: rows[31] (bit x of rows[y] = cell (x, y) open) and goal[2] = (x, y).
+ * Host only; for callers that replay or display episodes. */
+void ns_maze_layout(uint32_t seed, uint32_t index, uint32_t *rows, uint32_t *goal);
 
 #ifdef __cplusplus
 }

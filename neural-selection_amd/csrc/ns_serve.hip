@@ -2,11 +2,21 @@
 // episodes (no reference counterpart; gym_adapter/batch_evaluate.py runs
 // lock-step batches with masks instead).
 //
-// A request is one episode: a brain genome (ns_create encoding, 8 sensory
-// channels, 2 motors) driving a seeded 4-d linear system
-//     x' = A x + B u + sigma * n,   u_j = motor_j spikes / ticks per step,
-// observed through an opponent rate code on 8 channels (2 per dimension).
-// Return = -sum over steps of |x|^2.
+// A request is one evaluation of a brain genome (ns_create encoding) on the
+// server's task:
+//  * linear (task 0): a seeded 4-d linear system
+//        x' = A x + B u + sigma * n,   u_j = motor_j spikes / ticks per step,
+//    observed through an opponent rate code on 8 channels (2 per dimension),
+//    2 motors.  Return = -sum over steps of |x|^2.
+//  * mazes (task 1): `maze_episodes` seeded 31 x 31 depth-first mazes in a
+//    row, the brain's state kept from one to the next.  9 channels: walls
+//    north/east/south/west of the agent, goal east/west/south/north of it,
+//    and reward; 4 motors move north/east/south/west (most spikes wins, ties
+//    to the lower motor, none = stay).  A maze ends at the goal or after
+//    `maze_horizon` steps; reaching the goal after k steps pays
+//    1 - 0.9 k / horizon (gym_adapter/maze_gpu.cu:46) and sets the reward
+//    channel for the next step, so a dopamine modulator neuron listening on
+//    it delivers the reward as a modulatory event.
 //
 // One device pool of slots serves every episode.  Slot capacities follow a
 // ladder of size classes (ratio sqrt 2) with the same number of slots per
@@ -38,19 +48,144 @@
 namespace nsa {
 
 namespace {
-constexpr int kChannels = 8, kOutputs = 2, kDims = 4;
-// task record per slot (floats): x[4], A[16], B[8], return, motor totals[2], steps
-constexpr int kTaskFloats = 32;
+constexpr int kChannels = 8, kOutputs = 2, kDims = 4;            // linear task
+constexpr int kMazeChannels = 9, kMazeOutputs = 4;               // maze task
+constexpr int kTaskWords = 64;   // task record per slot (4-byte words)
+// linear record (floats): x[4], A[16], B[8], return, motor totals[2], steps
 enum { T_X = 0, T_A = 4, T_B = 20, T_RET = 28, T_M0 = 29, T_M1 = 30, T_STEPS = 31 };
+// maze record (u32 words; return and motor totals hold float bits): open
+// cells per row, agent, goal, maze index, steps in this maze, steps in all,
+// mazes solved, reward pending, return, motor totals[4], request seed
+enum {
+    M_ROWS = 0, M_AX = 31, M_AY, M_GX, M_GY, M_INDEX, M_STEPS, M_TOTAL, M_SOLVED, M_PENDING,
+    M_RET, M_MOT, M_SEED = M_MOT + 4
+};
+constexpr int kMazeSide = 31, kMazeRooms = 15;
 
 __global__ void k_encode(const float *task, const uint8_t *run, int slots, float *input) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= slots || !(run[e] & 1)) return;
-    const float *x = task + size_t(e) * kTaskFloats + T_X;
+    const float *x = task + size_t(e) * kTaskWords + T_X;
     for (int d = 0; d < kDims; ++d) {
         input[e * kChannels + 2 * d] = fminf(1.0f, fmaxf(0.0f, x[d]));
         input[e * kChannels + 2 * d + 1] = fminf(1.0f, fmaxf(0.0f, -x[d]));
     }
+}
+
+__host__ __device__ inline uint64_t splitmix64(uint64_t &z) {
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+    r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+    return r ^ (r >> 31);
+}
+
+// Maze `index` of a request seeded `seed`: a depth-first (recursive
+// backtracker) carve of the 15 x 15 rooms of a 31 x 31 grid, from room 0 at
+// (1, 1), neighbours tried in the order north, east, south, west and one
+// drawn uniformly (SplitMix64 mod count); then the goal room, drawn from the
+// other 224.  rows[y] bit x = cell (x, y) open.
+__host__ __device__ inline void make_maze(uint32_t seed, uint32_t index, uint32_t *rows,
+                                          uint32_t &gx, uint32_t &gy) {
+    uint64_t z = (0x9e3779b97f4a7c15ull * (uint64_t(seed) + 1)) ^ (uint64_t(index) * 0xd1b54a32d192ed03ull);
+    for (int y = 0; y < kMazeSide; ++y) rows[y] = 0;
+    uint32_t visited[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint8_t stack[kMazeRooms * kMazeRooms];
+    int top = 0;
+    rows[1] |= 1u << 1;
+    visited[0] = 1;
+    stack[top++] = 0;
+    while (top > 0) {
+        const int r = stack[top - 1], rx = r % kMazeRooms, ry = r / kMazeRooms;
+        int cand[4], nc = 0;
+        auto fresh = [&](int q) { return !((visited[q >> 5] >> (q & 31)) & 1u); };
+        if (ry > 0 && fresh(r - kMazeRooms)) cand[nc++] = r - kMazeRooms;
+        if (rx < kMazeRooms - 1 && fresh(r + 1)) cand[nc++] = r + 1;
+        if (ry < kMazeRooms - 1 && fresh(r + kMazeRooms)) cand[nc++] = r + kMazeRooms;
+        if (rx > 0 && fresh(r - 1)) cand[nc++] = r - 1;
+        if (!nc) { --top; continue; }
+        const int q = cand[splitmix64(z) % uint64_t(nc)];
+        const int ax = 1 + 2 * rx, ay = 1 + 2 * ry;
+        const int bx = 1 + 2 * (q % kMazeRooms), by = 1 + 2 * (q / kMazeRooms);
+        rows[(ay + by) / 2] |= 1u << ((ax + bx) / 2);
+        rows[by] |= 1u << bx;
+        visited[q >> 5] |= 1u << (q & 31);
+        stack[top++] = uint8_t(q);
+    }
+    const int g = 1 + int(splitmix64(z) % uint64_t(kMazeRooms * kMazeRooms - 1));
+    gx = uint32_t(1 + 2 * (g % kMazeRooms));
+    gy = uint32_t(1 + 2 * (g / kMazeRooms));
+}
+
+__host__ __device__ inline bool maze_open(const uint32_t *rows, int x, int y) {
+    return x >= 0 && x < kMazeSide && y >= 0 && y < kMazeSide && ((rows[y] >> x) & 1u);
+}
+
+__global__ void k_encode_maze(const uint32_t *task, const uint8_t *run, int slots, float *input) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= slots || !(run[e] & 1)) return;
+    const uint32_t *t = task + size_t(e) * kTaskWords;
+    const int x = int(t[M_AX]), y = int(t[M_AY]), gx = int(t[M_GX]), gy = int(t[M_GY]);
+    float *in = input + size_t(e) * kMazeChannels;
+    in[0] = maze_open(t + M_ROWS, x, y - 1) ? 0.0f : 1.0f;
+    in[1] = maze_open(t + M_ROWS, x + 1, y) ? 0.0f : 1.0f;
+    in[2] = maze_open(t + M_ROWS, x, y + 1) ? 0.0f : 1.0f;
+    in[3] = maze_open(t + M_ROWS, x - 1, y) ? 0.0f : 1.0f;
+    in[4] = gx > x ? 1.0f : 0.0f;
+    in[5] = gx < x ? 1.0f : 0.0f;
+    in[6] = gy > y ? 1.0f : 0.0f;
+    in[7] = gy < y ? 1.0f : 0.0f;
+    in[8] = t[M_PENDING] ? 1.0f : 0.0f;
+}
+
+// One maze step per live slot (see the header for the rules); a request
+// whose last maze ended sets its done flag and stops changing.
+__global__ void k_env_maze(uint32_t *task, const float *out, const uint8_t *run, int slots,
+                           uint32_t mazes, uint32_t horizon, uint8_t *done) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= slots || !(run[e] & 1)) return;
+    uint32_t *t = task + size_t(e) * kTaskWords;
+    if (t[M_INDEX] >= mazes) return;
+    t[M_PENDING] = 0;
+    const float *o = out + size_t(e) * kMazeOutputs;
+    int best = -1;
+    float top = 0.0f;
+    for (int m = 0; m < kMazeOutputs; ++m) {
+        if (o[m] > top) { top = o[m]; best = m; }
+        t[M_MOT + m] = __float_as_uint(__uint_as_float(t[M_MOT + m]) + o[m]);
+    }
+    int x = int(t[M_AX]), y = int(t[M_AY]);
+    if (best >= 0) {
+        const int nx = x + (best == 1) - (best == 3), ny = y + (best == 2) - (best == 0);
+        if (maze_open(t + M_ROWS, nx, ny)) { x = nx; y = ny; }
+    }
+    t[M_AX] = uint32_t(x);
+    t[M_AY] = uint32_t(y);
+    t[M_STEPS] += 1;
+    t[M_TOTAL] += 1;
+    bool next = false;
+    if (x == int(t[M_GX]) && y == int(t[M_GY])) {
+        const float reward = float(1.0 - 0.9 * (double(t[M_STEPS]) / double(horizon)));
+        t[M_RET] = __float_as_uint(__uint_as_float(t[M_RET]) + reward);
+        t[M_SOLVED] += 1;
+        t[M_PENDING] = 1;
+        next = true;
+    } else if (t[M_STEPS] >= horizon) {
+        next = true;
+    }
+    if (!next) return;
+    t[M_INDEX] += 1;
+    if (t[M_INDEX] >= mazes) {
+        done[e] = 1;
+        return;
+    }
+    uint32_t gx, gy;
+    make_maze(t[M_SEED], t[M_INDEX], t + M_ROWS, gx, gy);
+    t[M_GX] = gx;
+    t[M_GY] = gy;
+    t[M_AX] = 1;
+    t[M_AY] = 1;
+    t[M_STEPS] = 0;
 }
 
 __device__ inline float uniform01(uint32_t &s) {
@@ -59,12 +194,12 @@ __device__ inline float uniform01(uint32_t &s) {
 }
 
 // x' = A x + B u + sigma * (sum of 12 uniforms - 6); fixed evaluation order.
-// A slot stops updating once its episode has `steps` steps.
+// A slot stops updating once its episode has `steps` steps (done flag set).
 __global__ void k_env(float *task, uint32_t *rng, const float *out, const uint8_t *run, int slots,
-                      float ticks, float sigma, float steps) {
+                      float ticks, float sigma, float steps, uint8_t *done) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= slots || !(run[e] & 1)) return;
-    float *t = task + size_t(e) * kTaskFloats;
+    float *t = task + size_t(e) * kTaskWords;
     if (!(t[T_STEPS] < steps)) return;
     const float u0 = out[e * kOutputs] / ticks, u1 = out[e * kOutputs + 1] / ticks;
     uint32_t s = rng[e];
@@ -88,6 +223,7 @@ __global__ void k_env(float *task, uint32_t *rng, const float *out, const uint8_
     t[T_M1] += out[e * kOutputs + 1];
     t[T_STEPS] += 1.0f;
     rng[e] = s;
+    if (!(t[T_STEPS] < steps)) done[e] = 1;
 }
 
 struct Request {
@@ -109,7 +245,7 @@ void make_task(uint32_t seed, float *t) {
         r ^= r >> 31;
         return float(double(r >> 11) * (1.0 / 9007199254740992.0));
     };
-    std::memset(t, 0, sizeof(float) * kTaskFloats);
+    std::memset(t, 0, sizeof(float) * kTaskWords);
     for (int i = 0; i < kDims; ++i) t[T_X + i] = 2.0f * next() - 1.0f;
     for (int i = 0; i < kDims; ++i) {
         float row[kDims], norm = 0;
@@ -119,12 +255,23 @@ void make_task(uint32_t seed, float *t) {
     for (int k = 0; k < 8; ++k) t[T_B + k] = next() - 0.5f;
 }
 
+// First maze of a maze request (later ones are carved on the device).
+void make_maze_task(uint32_t seed, uint32_t *t) {
+    std::memset(t, 0, sizeof(uint32_t) * kTaskWords);
+    uint32_t gx, gy;
+    make_maze(seed, 0, t + M_ROWS, gx, gy);
+    t[M_GX] = gx;
+    t[M_GY] = gy;
+    t[M_AX] = 1;
+    t[M_AY] = 1;
+    t[M_SEED] = seed;
+}
+
 // An admitted episode in slot `pb`.
 struct Episode {
     Request req;
     BrainSpec spec;
     int pb = 0;
-    int steps_done = 0;
 };
 
 struct Server {
@@ -144,6 +291,10 @@ struct Server {
     DeviceArena io;
     float *d_input = nullptr, *d_out = nullptr, *d_task = nullptr;
     uint32_t *d_rng = nullptr;
+    uint8_t *d_done = nullptr;                           // per slot: request finished
+    std::vector<uint8_t> h_done;
+    int channels = kChannels, outputs = kOutputs;        // of the task
+    bool maze = false;
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<Episode>> live;          // by slot
     std::set<std::tuple<int, int, int>> freeset;         // (neurons, synapses, slot)
@@ -178,16 +329,19 @@ struct Server {
         max_n = cls.back();
         max_s = int(std::ceil(cls.back() * syn_ratio));
         pool.force_plastic = true;
-        pool.build(specs, kOutputs, Tile{1024, 1, 0, 1, 3584}, stream);
+        pool.build(specs, outputs, Tile{1024, 1, 0, 1, 3584}, stream);
         pool.nbatch = pool.nb;                 // rows are slots
         pool.pool_of.assign(pool.nb, -1);
         const int rows = std::max(pool.nb, 1);
-        d_input = io.alloc<float>(int64_t(rows) * kChannels);
-        d_out = io.alloc<float>(int64_t(rows) * kOutputs);
-        d_task = io.alloc<float>(int64_t(rows) * kTaskFloats);
+        d_input = io.alloc<float>(int64_t(rows) * channels);
+        d_out = io.alloc<float>(int64_t(rows) * outputs);
+        d_task = io.alloc<float>(int64_t(rows) * kTaskWords);
         d_rng = io.alloc<uint32_t>(rows);
-        NS_HIP(hipMemset(d_task, 0, sizeof(float) * size_t(rows) * kTaskFloats));
-        NS_HIP(hipMemset(d_input, 0, sizeof(float) * size_t(rows) * kChannels));
+        d_done = io.alloc<uint8_t>(rows);
+        NS_HIP(hipMemset(d_task, 0, sizeof(float) * size_t(rows) * kTaskWords));
+        NS_HIP(hipMemset(d_input, 0, sizeof(float) * size_t(rows) * channels));
+        NS_HIP(hipMemset(d_done, 0, size_t(rows)));
+        h_done.assign(pool.nb, 0);
         live.clear();
         live.resize(pool.nb);
         for (int pb = 0; pb < pool.nb; ++pb)
@@ -218,7 +372,7 @@ struct Server {
             ep->req = std::move(*it);
             ep->req.admit_s = now();
             ep->pb = pb;
-            ep->spec = BrainSpec{&ep->req.brain, pb, pb * kChannels, pb * kOutputs, false};
+            ep->spec = BrainSpec{&ep->req.brain, pb, pb * channels, pb * outputs, false};
             fresh.push_back(ep.get());
             live[pb] = std::move(ep);
             it = queue.erase(it);
@@ -238,22 +392,25 @@ struct Server {
         const int k = int(fresh.size());
         std::vector<uint8_t> next(pool.nb, 0);
         std::vector<int32_t> born(2 * size_t(k));   // slots to reset, then their seeds
-        std::vector<float> task(size_t(std::max(k, 1)) * kTaskFloats);
+        std::vector<float> task(size_t(std::max(k, 1)) * kTaskWords);
         std::vector<uint32_t> rng(std::max(k, 1));
         for (int j = 0; j < k; ++j) {
             const Episode &e = *fresh[j];
             born[j] = e.pb;
             born[k + j] = int32_t(e.req.seed);
-            make_task(e.req.task_seed, &task[j * kTaskFloats]);
+            if (maze) make_maze_task(e.req.task_seed, reinterpret_cast<uint32_t *>(&task[j * kTaskWords]));
+            else make_task(e.req.task_seed, &task[j * kTaskWords]);
             rng[j] = e.req.task_seed * 2654435761u + 12345u;
             if (!rng[j]) rng[j] = 1;
         }
         for (int pb = 0; pb < pool.nb; ++pb) next[pb] = live[pb] ? 1 : 0;
         if (k) stage.add(pool.d_reset, born.data(), 4u * born.size());
+        static const uint8_t kNotDone = 0;
         for (int j = 0; j < k; ++j) {
             const int pb = fresh[j]->pb;
-            stage.add(d_task + size_t(pb) * kTaskFloats, &task[j * kTaskFloats], 4u * kTaskFloats);
+            stage.add(d_task + size_t(pb) * kTaskWords, &task[j * kTaskWords], 4u * kTaskWords);
             stage.add(d_rng + pb, &rng[j], 4);
+            stage.add(d_done + pb, &kNotDone, 1);
         }
         stage.add(pool.view.brun, next.data(), size_t(pool.nb));
         // the staged vectors are copied into pinned memory by upload_stage
@@ -267,26 +424,38 @@ struct Server {
 
     void steps(int n) {
         const unsigned blocks = unsigned((pool.nb + 127) / 128);
+        uint32_t *task_words = reinterpret_cast<uint32_t *>(d_task);
+        const uint32_t mazes = uint32_t(cfg.maze_episodes > 0 ? cfg.maze_episodes : 10);
+        const uint32_t horizon = uint32_t(cfg.maze_horizon > 0 ? cfg.maze_horizon : 100);
         for (int k = 0; k < n; ++k) {
-            hipLaunchKernelGGL(k_encode, dim3(blocks), dim3(128), 0, stream, d_task, pool.view.brun,
-                               pool.nb, d_input);
+            if (maze)
+                hipLaunchKernelGGL(k_encode_maze, dim3(blocks), dim3(128), 0, stream, task_words,
+                                   pool.view.brun, pool.nb, d_input);
+            else
+                hipLaunchKernelGGL(k_encode, dim3(blocks), dim3(128), 0, stream, d_task,
+                                   pool.view.brun, pool.nb, d_input);
             run_pool(pool, NS_FORM_RESIDENT, cfg.ticks_per_step, d_input, cfg.input_gain, d_out,
-                     int64_t(pool.nb) * kOutputs, 1, stream);
-            hipLaunchKernelGGL(k_env, dim3(blocks), dim3(128), 0, stream, d_task, d_rng, d_out,
-                               pool.view.brun, pool.nb, float(cfg.ticks_per_step), cfg.noise_sigma,
-                               float(cfg.steps_per_episode));
+                     int64_t(pool.nb) * outputs, 1, stream);
+            if (maze)
+                hipLaunchKernelGGL(k_env_maze, dim3(blocks), dim3(128), 0, stream, task_words, d_out,
+                                   pool.view.brun, pool.nb, mazes, horizon, d_done);
+            else
+                hipLaunchKernelGGL(k_env, dim3(blocks), dim3(128), 0, stream, d_task, d_rng, d_out,
+                                   pool.view.brun, pool.nb, float(cfg.ticks_per_step),
+                                   cfg.noise_sigma, float(cfg.steps_per_episode), d_done);
         }
         NS_HIP(hipGetLastError());
     }
 
-    // Collect finished episodes and free their slots.  Returns whether any
-    // slot was freed (its run flag must be cleared).
+    // Collect finished episodes (done flags read after the last launch
+    // round) and free their slots.  Returns whether any slot was freed (its
+    // run flag must be cleared).
     bool finish() {
         std::vector<int> ended;
         for (int pb = 0; pb < pool.nb; ++pb)
-            if (live[pb] && live[pb]->steps_done >= cfg.steps_per_episode) ended.push_back(pb);
+            if (live[pb] && h_done[pb]) ended.push_back(pb);
         if (ended.empty()) return false;
-        std::vector<float> task(size_t(pool.nb) * kTaskFloats);
+        std::vector<float> task(size_t(pool.nb) * kTaskWords);
         NS_HIP(hipMemcpy(task.data(), d_task, sizeof(float) * task.size(), hipMemcpyDeviceToHost));
         const double t = now();
         std::vector<std::pair<int, const BrainSpec *>> vacate;
@@ -296,16 +465,29 @@ struct Server {
         std::lock_guard<std::mutex> lock(mu);
         for (int pb : ended) {
             const Episode &e = *live[pb];
-            const float *tk = &task[size_t(pb) * kTaskFloats];
+            const float *tk = &task[size_t(pb) * kTaskWords];
             ns_serve_result r{};
             r.id = e.req.id;
             r.submit_s = e.req.submit_s;
             r.admit_s = e.req.admit_s;
             r.finish_s = t;
-            r.episode_return = tk[T_RET];
-            for (int d = 0; d < kDims; ++d) r.final_state[d] = tk[T_X + d];
-            r.motor_spikes[0] = int32_t(tk[T_M0]);
-            r.motor_spikes[1] = int32_t(tk[T_M1]);
+            if (maze) {
+                const uint32_t *w = reinterpret_cast<const uint32_t *>(tk);
+                float mot[2];
+                std::memcpy(&r.episode_return, &w[M_RET], 4);
+                std::memcpy(mot, &w[M_MOT], 8);
+                r.final_state[0] = float(w[M_SOLVED]);
+                r.final_state[1] = float(w[M_TOTAL]);
+                r.final_state[2] = float(w[M_INDEX]);
+                r.final_state[3] = 0.0f;
+                r.motor_spikes[0] = int32_t(mot[0]);
+                r.motor_spikes[1] = int32_t(mot[1]);
+            } else {
+                r.episode_return = tk[T_RET];
+                for (int d = 0; d < kDims; ++d) r.final_state[d] = tk[T_X + d];
+                r.motor_spikes[0] = int32_t(tk[T_M0]);
+                r.motor_spikes[1] = int32_t(tk[T_M1]);
+            }
             r.status = 0;
             done.push_back(r);
             freeset.insert({pool.bnc[pb], pool.bsc[pb], pb});
@@ -337,9 +519,9 @@ struct Server {
                 if (resident) {
                     const int n = cfg.steps_per_launch;
                     steps(n);
+                    NS_HIP(hipMemcpyAsync(h_done.data(), d_done, size_t(pool.nb),
+                                          hipMemcpyDeviceToHost, stream));
                     NS_HIP(hipStreamSynchronize(stream));
-                    for (auto &e : live)
-                        if (e) e->steps_done += n;
                     stats.launches += 1;
                 }
                 stats.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy0).count();
@@ -368,10 +550,15 @@ void *ns_serve_create(const ns_serve_config *config) {
             config->episode_capacity < 1 || config->steps_per_episode < 1 ||
             config->ticks_per_step < 1 || config->ticks_per_step > 10000 ||
             config->steps_per_launch < 1 || !(config->noise_sigma >= 0) || !(config->input_gain >= 0) ||
-            config->max_episode_neurons < 0 || config->max_episode_neurons > (1 << 20))
+            config->max_episode_neurons < 0 || config->max_episode_neurons > (1 << 20) ||
+            config->task < 0 || config->task > 1 || config->maze_episodes < 0 ||
+            config->maze_horizon < 0)
             throw std::runtime_error("Invalid serving configuration");
         auto *s = new Server;
         s->cfg = *config;
+        s->maze = config->task == 1;
+        s->channels = s->maze ? kMazeChannels : kChannels;
+        s->outputs = s->maze ? kMazeOutputs : kOutputs;
         s->t0 = std::chrono::steady_clock::now();
         try {
             NS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -408,7 +595,7 @@ int64_t ns_serve_submit(void *p, int count, int edges, const float *ng, const fl
                         const int *motors, uint32_t seed, uint32_t task_seed) {
     try {
         auto &s = *static_cast<Server *>(p);
-        std::unique_ptr<HostBrain> b(parse_gym_brain(count, edges, kChannels, kOutputs, ng, sg,
+        std::unique_ptr<HostBrain> b(parse_gym_brain(count, edges, s.channels, s.outputs, ng, sg,
                                                      motors, 8));
         if (count > s.max_n || edges > s.max_s)
             throw std::runtime_error("Episode exceeds the largest slot");
@@ -455,6 +642,10 @@ int ns_serve_drain(void *p) {
         return -1;
     }
     return 0;
+}
+
+void ns_maze_layout(uint32_t seed, uint32_t index, uint32_t *rows, uint32_t *goal) {
+    make_maze(seed, index, rows, goal[0], goal[1]);
 }
 
 int ns_serve_stats_get(void *p, ns_serve_stats *stats) {

@@ -48,7 +48,11 @@ class ServeConfig(ct.Structure):
                 ('episode_capacity', ct.c_int), ('steps_per_episode', ct.c_int),
                 ('ticks_per_step', ct.c_int), ('steps_per_launch', ct.c_int),
                 ('noise_sigma', ct.c_float), ('input_gain', ct.c_float),
-                ('max_episode_neurons', ct.c_int)]
+                ('max_episode_neurons', ct.c_int), ('task', ct.c_int),
+                ('maze_episodes', ct.c_int), ('maze_horizon', ct.c_int)]
+
+
+TASK_LINEAR, TASK_MAZE = 0, 1
 
 
 class ServeResult(ct.Structure):
@@ -102,6 +106,8 @@ SIGNATURES = {
     'ns_serve_poll': ([_vp, ct.POINTER(ServeResult), ct.c_int], ct.c_int),
     'ns_serve_drain': ([_vp], ct.c_int),
     'ns_serve_stats_get': ([_vp, ct.POINTER(ServeStats)], ct.c_int),
+    'ns_maze_layout': ([ct.c_uint32, ct.c_uint32, ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_uint32)],
+                       None),
 }
 
 _lib = None

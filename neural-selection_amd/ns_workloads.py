@@ -212,6 +212,24 @@ def small_mix(seed=11, brains=8, calls=40, plasticity=True, max_delay=8, types=(
                      0.4, 230.0, 16, calls, active_p)
 
 
+def maze_genome(rng, neurons, fanout=4.0):
+    """One maze-task genome: 9 sensor neurons on channels 0-8 (walls, goal
+    direction, reward), the reward sensor a dopamine modulator, 4 motor
+    neurons last; synapses with pair STDP (profiles 1/2) and D1 dopamine
+    plasticity (profile 4)."""
+    neurons = max(neurons, 13)
+    types = _uniform_types([0, 1, 4])(rng, neurons)
+    nrows = _neuron_rows(rng, neurons, types, 0.2, 9, 9)
+    nrows[8, 3] = 0
+    nrows[8, 4] = 3
+    m = int(round(neurons * fanout))
+    u = rng.random(m)
+    plast = np.where(u < 0.3, 4, np.where(u < 0.55, 2, np.where(u < 0.8, 1, 0))).astype(np.float32)
+    srows = _synapse_rows(rng, neurons, m, 8, -1.0, 0.5, plast)
+    motors = np.arange(neurons - 4, neurons, dtype=np.int32)
+    return nrows, srows, motors
+
+
 def episode_genome(rng, neurons, fanout=4.0, channels=8, outputs=2):
     """One serving-episode genome (log-uniform size chosen by the caller)."""
     types = _uniform_types([0, 1, 4])(rng, neurons)
