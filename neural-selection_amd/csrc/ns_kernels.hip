@@ -300,11 +300,12 @@ __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, co
         return;
     }
     int64_t tick = p.btick[b] + t;
-    float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
-    unsigned mod = gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab);
+    // the state and gene loads go out before the input gather's chain
     NeuronRegs s;
     NeuronConst g;
     load_neuron(p, i, s, g, g_exp_tab);
+    float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
+    unsigned mod = gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab);
     float current = sensory_current(p, i, s, g, input, gain);
     int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, g_exp_tab);
     store_neuron(p, i, s);
@@ -465,18 +466,25 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
     const int64_t s0 = p.bs0[b], s1 = s0 + p.bsc[b];
     if (tid == 0) big_events = 0;
     unsigned arrivals = 0;
+    // Per tick: plasticity of tick t-1 and delivery of tick t in one synapse
+    // pass (a synapse's plasticity precedes its next delivery in its own
+    // thread, the reference's order), then the neurons of tick t.
     for (int t = 0; t < ticks; ++t) {
-        for (int64_t q = s0 + tid; q < s1; q += WG) arrivals += deliver_one(p, q, b, t);
+        for (int64_t q = s0 + tid; q < s1; q += WG) {
+            if (t > 0) plasticity_one(p, q, b);
+            arrivals += deliver_one(p, q, b, t);
+        }
         __syncthreads();
         for (int64_t i = n0 + tid; i < n1; i += WG) neuron_one(p, i, b, t, input, gain);
         __syncthreads();
-        for (int64_t q = s0 + tid; q < s1; q += WG) plasticity_one(p, q, b);
         for (int m = tid; m < p.outputs; m += WG) {
             const int32_t mn = p.motor[int64_t(b) * p.outputs + m];
             if (mn >= 0) out[p.bout0[b] + m] += float(p.spike[mn]);
         }
-        __syncthreads();
     }
+    if (ticks > 0)
+        for (int64_t q = s0 + tid; q < s1; q += WG) plasticity_one(p, q, b);
+    __syncthreads();
     if (homeostasis) {
         const int64_t tick = p.btick[b] + ticks;
         for (int64_t i = n0 + tid; i < n1; i += WG) {
