@@ -1277,6 +1277,7 @@ NS_INSTANTIATE(1024, 1, 8, 1)
 NS_INSTANTIATE(128, 2, 8, 1)
 NS_INSTANTIATE(128, 1, 4, 4)
 NS_INSTANTIATE_LDS(1024, 1, 1)
+NS_INSTANTIATE_LDS(1024, 2, 1)
 NS_INSTANTIATE_LDS(512, 1, 1)
 NS_INSTANTIATE_LDS(512, 2, 1)
 NS_INSTANTIATE_LDS(256, 1, 1)
@@ -1304,6 +1305,10 @@ const TileShape kTiles[] = {
     NS_TILE(128, 2, 8, 1),
     NS_TILE(128, 1, 4, 4),
     NS_TILE_LDS(1024, 1, 1, 3584, 5632),
+    // two neurons per thread (2,048 per pack): spills about 65 registers per
+    // lane to scratch, yet keeps brains of 1,025-2,048 neurons out of the
+    // one-workgroup-per-brain path (8% faster on the 4M-neuron Pareto mix)
+    NS_TILE_LDS(1024, 2, 1, 3328, 5632),
     NS_TILE_LDS(1024, 1, 1, 2048, 3072),
     NS_TILE_LDS(512, 1, 1, 1792, 2944),
     NS_TILE_LDS(512, 2, 1, 3584, 5632),

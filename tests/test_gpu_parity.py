@@ -54,10 +54,20 @@ def test_masks_and_resets_match():
 
 @pytest.mark.parametrize('tiles', [(256, 1, 4), (512, 2, 4), (1024, 1, 8), (128, 2, 8),
                                    (256, 1, 4, 3), (1024, 1, 0, 1, 3584), (256, 1, 0, 1, 896),
-                                   (512, 2, 0, 1, 3584)])
+                                   (512, 2, 0, 1, 3584), (1024, 2, 0, 1, 3328)])
 def test_tile_shapes_agree(tiles):
     w = W.small_mix(seed=31, brains=10, calls=12, types=(0, 1, 4, 5, 6), lo=20, hi=120)
     _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=tiles)
+
+
+def test_two_neurons_per_thread_packs():
+    """The 1024-thread tile with two neuron slots per thread: packs of more
+    than 1,024 neurons, so every thread also runs a second neuron."""
+    w = W.small_mix(seed=37, brains=24, calls=12, types=(0, 1, 4, 5, 6), lo=60, hi=120,
+                    fanout=1.5)
+    assert w.total_neurons > 1400
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=(1024, 2, 0, 1, 3328))
+    assert info['resident_packs'] <= 2 and info['big_brains'] == 0
 
 
 def test_big_brains_bit_exact():
