@@ -15,6 +15,8 @@
 //    wavefront ballots, then short-term plasticity runs densely over the
 //    queue.  Brains are independent, so no cross-workgroup synchronisation
 //    is needed inside a call.
+#include <atomic>
+
 #include "ns_kernels.h"
 
 namespace nsa {
@@ -1237,13 +1239,21 @@ static hipError_t launch_lds_t(const Pool &p, const Pack *packs, int npacks, con
                                int nbig, const uint16_t *perm, int ticks, const float *input,
                                float gain, float *out, int homeostasis, int smax, hipStream_t st) {
     const size_t bytes = size_t(smax) * lds_synapse_bytes<PLASTIC>() + 16;
-    static size_t allowed = 0;   // dynamic LDS limit raised so far for this instance
-    if (bytes > allowed) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&k_resident_lds<WG, KN, MINW, PLASTIC>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    // dynamic LDS limit raised so far for this instance, per device (threads
+    // that launch on one device at the same time raise it the same way)
+    constexpr int kMaxDevices = 64;
+    static std::atomic<size_t> allowed[kMaxDevices];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices || bytes > allowed[dev].load(std::memory_order_relaxed)) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_resident_lds<WG, KN, MINW, PLASTIC>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
         if (e != hipSuccess) return e;
-        allowed = bytes;
+        if (dev >= 0 && dev < kMaxDevices) {
+            size_t cur = allowed[dev].load(std::memory_order_relaxed);
+            while (cur < bytes && !allowed[dev].compare_exchange_weak(cur, bytes)) {}
+        }
     }
     hipLaunchKernelGGL((k_resident_lds<WG, KN, MINW, PLASTIC>), dim3(nbig + npacks), dim3(WG), bytes,
                        st, p, packs, big, nbig, perm, ticks, input, gain, out, homeostasis, smax);
