@@ -109,6 +109,36 @@ def cpu_baseline(w, seconds=15.0, max_neurons=65536):
                       f'x {w.ticks} ticks, {dt:.1f} s wall, oracle/liboracle.so threaded over brains'}
 
 
+def serving_cpu_baseline(reqs, run_one, work_of, budget=1.5e9):
+    """The same request stream on the host cores: the first requests whose
+    work (neuron-updates) fits `budget`, all submitted at once to a thread pool
+    that runs each through the CPU oracle brain and the task restatement.
+    run_one returns the neuron-updates the request took."""
+    import concurrent.futures as cf
+    sample, est = [], 0.0
+    for r in reqs:
+        w = work_of(r)
+        if sample and est + w > budget:
+            break
+        sample.append(r)
+        est += w
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    done, lat = 0.0, []
+    with cf.ThreadPoolExecutor(threads) as ex:
+        futs = [ex.submit(run_one, r) for r in sample]
+        for f in cf.as_completed(futs):
+            done += f.result()
+            lat.append(time.perf_counter() - t0)
+    wall = time.perf_counter() - t0
+    lat = np.array(lat) * 1e3
+    return {'value': done / wall, 'unit': 'neuron-updates/s', 'cores': threads, 'kind': 'port',
+            'sample': f'first {len(sample)} requests of the same stream ({done:.3g} neuron-updates), '
+                      f'all submitted at once to a {threads}-thread pool, each run whole by the '
+                      f'oracle library (oracle/liboracle.so: brain and task); {wall:.1f} s wall',
+            'latency_ms': {'p50': float(np.percentile(lat, 50)), 'p99': float(np.percentile(lat, 99))}}
+
+
 def load_traffic(kernel, workload, tile):
     """Measured HBM bytes per launch of `kernel` from its committed rocprofv3
     summary, if that summary was taken on this workload and tile."""
@@ -189,6 +219,17 @@ def bench_serving(args):
         log(f'  p50 {runs[-1]["latency_ms"]["p50"]:.1f} ms, p99 {runs[-1]["latency_ms"]["p99"]:.1f} ms, '
             f'{runs[-1]["neuron_updates_per_s"]:.3e} neuron-updates/s')
     top = runs[-1]
+    cpu = None
+    if not args.no_cpu:
+        log('timing the CPU oracle on a bounded sample of the same episodes')
+        import tasks
+        steps = args.episode_steps
+
+        def run_one(e):
+            _, ran = tasks.linear_episode_native(e[0], e[1], e[2], e[3], e[4], steps, 0.01)
+            return len(e[0]) * ran * 16
+
+        cpu = serving_cpu_baseline(eps, run_one, lambda e: len(e[0]) * steps * 16)
     line = {
         'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
         'value': top['neuron_updates_per_s'], 'unit': 'neuron-updates/s', 'n_gpus': 1,
@@ -199,7 +240,7 @@ def bench_serving(args):
                    'episodes, sizes log-uniform 16..4096 neurons, 4-d linear task, 8 sensors, '
                    f'2 motors, {args.episode_steps} steps x 16 ticks', 'episodes': len(eps),
                    'neurons_total': total_neurons, 'rates_per_s': rates},
-        'serving': runs, 'roofline': None, 'cpu_baseline': None,
+        'serving': runs, 'roofline': None, 'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -257,6 +298,17 @@ def bench_mazes(args):
     solved = np.array([r.final_state[0] for r in order])
     log(f'  {n} requests in {wall:.2f} s: {nu / wall:.3e} neuron-updates/s, '
         f'latency p50 {np.percentile(lat, 50):.0f} ms p99 {np.percentile(lat, 99):.0f} ms')
+    cpu = None
+    if not args.no_cpu:
+        log('timing the CPU oracle on a bounded sample of the same requests')
+        import tasks
+        mz, hz = args.mazes, args.maze_horizon
+
+        def run_one(r):
+            _, ran = tasks.maze_request_native(r[0], r[1], r[2], r[3], r[4], mz, hz)
+            return len(r[0]) * ran * 16
+
+        cpu = serving_cpu_baseline(reqs, run_one, lambda r: len(r[0]) * mz * hz * 16)
     line = {
         'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
         'value': nu / wall, 'unit': 'neuron-updates/s', 'n_gpus': 1,
@@ -275,7 +327,7 @@ def bench_mazes(args):
                     'mazes_solved': {'total': int(solved.sum()), 'requests_solving_any': int((solved > 0).sum())},
                     'launch_rounds': stats.launches, 'resident_peak': stats.resident_peak,
                     'device_busy_s': stats.busy_s},
-        'roofline': None, 'cpu_baseline': None,
+        'roofline': None, 'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
 

@@ -926,4 +926,168 @@ uint64_t ora_branch_io_nonzero(uint32_t lo, uint32_t hi, uint32_t step) {
 }
 float ora_fn_logf(float x) { return logf(x); }
 
+// ---------------------------------------------------------------------------
+// Serving tasks on the CPU (the serving loop's linear and maze tasks, for the
+// serving CPU baseline and the tests).  Whole requests run here, one oracle
+// brain each, so a thread pool runs them without the caller's interpreter.
+// ---------------------------------------------------------------------------
+static uint64_t task_splitmix(uint64_t &z) {
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+    r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+    return r ^ (r >> 31);
+}
+
+// Linear request: x' = A x + B u + sigma * (sum of 12 uniforms - 6), u the two
+// motor counts per tick; return = -sum |x|^2.  Returns the steps run.
+int ora_linear_episode(void *p, uint32_t task_seed, int steps, float sigma, int ticks, float gain,
+                       float *ret, float *x_out, int32_t *motors) {
+    auto &b = *static_cast<Brain *>(p);
+    uint64_t z = 0x9e3779b97f4a7c15ull * (uint64_t(task_seed) + 1);
+    auto uni53 = [&]() { return float(double(task_splitmix(z) >> 11) * (1.0 / 9007199254740992.0)); };
+    float x[4], A[16], B[8];
+    for (int i = 0; i < 4; ++i) x[i] = 2.0f * uni53() - 1.0f;
+    for (int i = 0; i < 4; ++i) {
+        float row[4], norm = 0.0f;
+        for (int j = 0; j < 4; ++j) { row[j] = 2.0f * uni53() - 1.0f; norm += std::fabs(row[j]); }
+        for (int j = 0; j < 4; ++j) A[i * 4 + j] = 0.9f * row[j] / norm;
+    }
+    for (int k = 0; k < 8; ++k) B[k] = uni53() - 0.5f;
+    uint32_t s = task_seed * 2654435761u + 12345u;
+    if (!s) s = 1;
+    auto uni24 = [&]() {
+        s ^= s << 13; s ^= s >> 17; s ^= s << 5;
+        return (s >> 8) * (1.0f / 16777216.0f);
+    };
+    float r = 0.0f, m0 = 0.0f, m1 = 0.0f, in[8], out[2];
+    for (int k = 0; k < steps; ++k) {
+        for (int d = 0; d < 4; ++d) {
+            in[2 * d] = std::fmin(1.0f, std::fmax(0.0f, x[d]));
+            in[2 * d + 1] = std::fmin(1.0f, std::fmax(0.0f, -x[d]));
+        }
+        step_brain(b, in, ticks, gain, out, nullptr);
+        const float u0 = out[0] / float(ticks), u1 = out[1] / float(ticks);
+        float nx[4];
+        for (int i = 0; i < 4; ++i) {
+            float acc = 0.0f;
+            for (int j = 0; j < 4; ++j) acc += A[i * 4 + j] * x[j];
+            acc += B[i * 2] * u0;
+            acc += B[i * 2 + 1] * u1;
+            float n = -6.0f;
+            for (int q = 0; q < 12; ++q) n += uni24();
+            nx[i] = acc + sigma * n;
+        }
+        float cost = 0.0f;
+        for (int i = 0; i < 4; ++i) { x[i] = nx[i]; cost += nx[i] * nx[i]; }
+        r -= cost;
+        m0 += out[0];
+        m1 += out[1];
+    }
+    *ret = r;
+    for (int i = 0; i < 4; ++i) x_out[i] = x[i];
+    motors[0] = int32_t(m0);
+    motors[1] = int32_t(m1);
+    return steps;
+}
+
+// Depth-first 31 x 31 maze `index` of a request seeded `seed` (rooms at odd
+// coordinates, carve from (1, 1), neighbours north/east/south/west, one drawn
+// as SplitMix64 mod count; goal room drawn from the other 224).
+static void task_maze(uint32_t seed, uint32_t index, uint32_t rows[31], int &gx, int &gy) {
+    uint64_t z = (0x9e3779b97f4a7c15ull * (uint64_t(seed) + 1)) ^ (uint64_t(index) * 0xd1b54a32d192ed03ull);
+    for (int y = 0; y < 31; ++y) rows[y] = 0;
+    std::vector<uint8_t> seen(225, 0);
+    std::vector<int> stack{0};
+    rows[1] |= 1u << 1;
+    seen[0] = 1;
+    while (!stack.empty()) {
+        const int r = stack.back(), rx = r % 15, ry = r / 15;
+        int cand[4], nc = 0;
+        if (ry > 0 && !seen[r - 15]) cand[nc++] = r - 15;
+        if (rx < 14 && !seen[r + 1]) cand[nc++] = r + 1;
+        if (ry < 14 && !seen[r + 15]) cand[nc++] = r + 15;
+        if (rx > 0 && !seen[r - 1]) cand[nc++] = r - 1;
+        if (!nc) { stack.pop_back(); continue; }
+        const int q = cand[task_splitmix(z) % uint64_t(nc)];
+        const int ax = 1 + 2 * rx, ay = 1 + 2 * ry, bx = 1 + 2 * (q % 15), by = 1 + 2 * (q / 15);
+        rows[(ay + by) / 2] |= 1u << ((ax + bx) / 2);
+        rows[by] |= 1u << bx;
+        seen[q] = 1;
+        stack.push_back(q);
+    }
+    const int g = 1 + int(task_splitmix(z) % 224);
+    gx = 1 + 2 * (g % 15);
+    gy = 1 + 2 * (g / 15);
+}
+
+void ora_maze_layout(uint32_t seed, uint32_t index, uint32_t *rows, uint32_t *goal) {
+    int gx, gy;
+    task_maze(seed, index, rows, gx, gy);
+    goal[0] = uint32_t(gx);
+    goal[1] = uint32_t(gy);
+}
+
+// Maze request: `mazes` mazes in a row, the brain kept; observation walls
+// N/E/S/W, goal E/W/S/N, reward; most motor spikes moves N/E/S/W (ties to
+// the lower motor, none = stay); the goal pays 1 - 0.9 steps / horizon.
+// state4 = {solved, steps, mazes played, 0}.  Returns the steps run.
+int ora_maze_request(void *p, uint32_t task_seed, int mazes, int horizon, int ticks, float gain,
+                     float *ret, float *state4, int32_t *motors) {
+    auto &b = *static_cast<Brain *>(p);
+    uint32_t rows[31];
+    int gx, gy, x = 1, y = 1, steps = 0, total = 0, solved = 0, index = 0;
+    bool pending = false;
+    task_maze(task_seed, 0, rows, gx, gy);
+    auto open = [&](int cx, int cy) { return cx >= 0 && cx < 31 && cy >= 0 && cy < 31 && ((rows[cy] >> cx) & 1u); };
+    float r = 0.0f, mot[4] = {0, 0, 0, 0}, in[9], out[4];
+    while (index < mazes) {
+        in[0] = open(x, y - 1) ? 0.0f : 1.0f;
+        in[1] = open(x + 1, y) ? 0.0f : 1.0f;
+        in[2] = open(x, y + 1) ? 0.0f : 1.0f;
+        in[3] = open(x - 1, y) ? 0.0f : 1.0f;
+        in[4] = gx > x ? 1.0f : 0.0f;
+        in[5] = gx < x ? 1.0f : 0.0f;
+        in[6] = gy > y ? 1.0f : 0.0f;
+        in[7] = gy < y ? 1.0f : 0.0f;
+        in[8] = pending ? 1.0f : 0.0f;
+        step_brain(b, in, ticks, gain, out, nullptr);
+        pending = false;
+        int best = -1;
+        float top = 0.0f;
+        for (int m = 0; m < 4; ++m) {
+            if (out[m] > top) { top = out[m]; best = m; }
+            mot[m] += out[m];
+        }
+        if (best >= 0) {
+            const int nx = x + (best == 1) - (best == 3), ny = y + (best == 2) - (best == 0);
+            if (open(nx, ny)) { x = nx; y = ny; }
+        }
+        ++steps;
+        ++total;
+        bool next = false;
+        if (x == gx && y == gy) {
+            r += float(1.0 - 0.9 * (double(steps) / double(horizon)));
+            ++solved;
+            pending = true;
+            next = true;
+        } else if (steps >= horizon) {
+            next = true;
+        }
+        if (!next) continue;
+        if (++index >= mazes) break;
+        task_maze(task_seed, uint32_t(index), rows, gx, gy);
+        x = y = 1;
+        steps = 0;
+    }
+    *ret = r;
+    state4[0] = float(solved);
+    state4[1] = float(total);
+    state4[2] = float(index);
+    state4[3] = 0.0f;
+    motors[0] = int32_t(mot[0]);
+    motors[1] = int32_t(mot[1]);
+    return total;
+}
+
 }  // extern "C"

@@ -94,6 +94,13 @@ def oracle_lib():
         lib.ora_batch_step.argtypes = [ct.POINTER(_vp), ct.c_int, _f, ct.c_int, _u8, ct.c_int,
                                        ct.c_float, _f, ct.c_int, _i32, ct.c_int]
         lib.ora_batch_step.restype = ct.c_int
+        lib.ora_linear_episode.argtypes = [_vp, ct.c_uint32, ct.c_int, ct.c_float, ct.c_int, ct.c_float,
+                                           _f, _f, _i32]
+        lib.ora_linear_episode.restype = ct.c_int
+        lib.ora_maze_request.argtypes = [_vp, ct.c_uint32, ct.c_int, ct.c_int, ct.c_int, ct.c_float,
+                                         _f, _f, _i32]
+        lib.ora_maze_request.restype = ct.c_int
+        lib.ora_maze_layout.argtypes = [ct.c_uint32, ct.c_uint32, _u32, _u32]
         for name in ('ora_fn_expf', 'ora_fn_logf', 'ora_fn_branch_io'):
             getattr(lib, name).argtypes = [ct.c_float]
             getattr(lib, name).restype = ct.c_float

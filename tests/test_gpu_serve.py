@@ -1,6 +1,7 @@
 """Serving loop: episodes admitted by free pool capacity run concurrently;
 each episode's result is bit-identical alone, in the mix, and to a CPU
-replay through the oracle (brain) plus the same task arithmetic in float32."""
+replay through the oracle (brain) plus the task restated in float32
+(oracle/tasks.py)."""
 import ctypes as ct
 import time
 
@@ -9,7 +10,7 @@ import pytest
 
 import ns_amd
 import ns_workloads as W
-from oracle import OracleBrain
+from tasks import replay_linear
 
 pytestmark = pytest.mark.gpu
 
@@ -65,76 +66,6 @@ def run(eps, cfg, gap=0.0):
         L.ns_serve_destroy(srv)
 
 
-def task(seed):
-    """make_task of the serving loop, restated (SplitMix64 -> float32)."""
-    mask = (1 << 64) - 1
-    z = (0x9e3779b97f4a7c15 * (seed + 1)) & mask
-
-    def nxt():
-        nonlocal z
-        z = (z + 0x9e3779b97f4a7c15) & mask
-        r = z
-        r = ((r ^ (r >> 30)) * 0xbf58476d1ce4e5b9) & mask
-        r = ((r ^ (r >> 27)) * 0x94d049bb133111eb) & mask
-        r ^= r >> 31
-        return np.float32((r >> 11) * (1.0 / 9007199254740992.0))
-    f = np.float32
-    x = np.array([f(2) * nxt() - f(1) for _ in range(4)], np.float32)
-    A = np.zeros((4, 4), np.float32)
-    for i in range(4):
-        row = [f(2) * nxt() - f(1) for _ in range(4)]
-        norm = f(0)
-        for v in row:
-            norm = f(norm + abs(v))
-        for j in range(4):
-            A[i, j] = f(f(0.9) * row[j]) / norm
-    B = np.array([nxt() - f(0.5) for _ in range(8)], np.float32).reshape(4, 2)
-    return x, A, B
-
-
-def cpu_episode(nr, sr, mo, seed, tseed):
-    f = np.float32
-    b = OracleBrain(nr, sr, mo, 8)
-    b.reset(seed)
-    x, A, B = task(tseed)
-    s = (tseed * 2654435761 + 12345) & 0xffffffff
-    s = s or 1
-    ret = f(0)
-    m = [f(0), f(0)]
-
-    def uni():
-        nonlocal s
-        s ^= (s << 13) & 0xffffffff
-        s ^= s >> 17
-        s ^= (s << 5) & 0xffffffff
-        return f((s >> 8) * (1.0 / 16777216.0))
-    for _ in range(STEPS):
-        inp = np.zeros(8, np.float32)
-        for d in range(4):
-            inp[2 * d] = min(f(1), max(f(0), x[d]))
-            inp[2 * d + 1] = min(f(1), max(f(0), -x[d]))
-        out, rc = b.step(inp, 16, 230.0)
-        u0, u1 = f(out[0]) / f(16), f(out[1]) / f(16)
-        nx = np.zeros(4, np.float32)
-        for i in range(4):
-            acc = f(0)
-            for j in range(4):
-                acc = f(acc + f(A[i, j] * x[j]))
-            acc = f(acc + f(B[i, 0] * u0))
-            acc = f(acc + f(B[i, 1] * u1))
-            n = f(-6)
-            for _ in range(12):
-                n = f(n + uni())
-            nx[i] = f(acc + f(f(0.01) * n))
-        cost = f(0)
-        for i in range(4):
-            cost = f(cost + f(nx[i] * nx[i]))
-        x = nx
-        ret = f(ret - cost)
-        m = [f(m[0] + out[0]), f(m[1] + out[1])]
-    return ret, x, (int(m[0]), int(m[1]))
-
-
 def test_serving_mix_equals_alone_equals_cpu():
     eps = genomes(24)
     print(f'[serve] {len(eps)} episodes x {STEPS} steps', flush=True)
@@ -145,7 +76,8 @@ def test_serving_mix_equals_alone_equals_cpu():
         assert alone[0][0] == mixed[k][0] and np.array_equal(alone[0][1], mixed[k][1])
         assert alone[0][2] == mixed[k][2]
     for k in (0, 9):
-        ret, x, m = cpu_episode(*eps[k])
+        nr, sr, mo, seed, tseed = eps[k]
+        ret, x, m = replay_linear(nr, sr, mo, seed, tseed, STEPS, 0.01)
         assert ret == mixed[k][0], (ret, mixed[k][0])
         assert np.array_equal(x, mixed[k][1]) and m == mixed[k][2]
 
