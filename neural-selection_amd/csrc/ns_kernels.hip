@@ -288,8 +288,11 @@ __device__ inline float sensory_current(const Pool &p, int64_t i, NeuronRegs &s,
 
 // neuron_step (src/main.cu:710-746) / the neuron loop of core.cuh:49-58, for
 // the register types (0, 1, 4, 5, 6) and inactive cells of any type.
+// `arrived` = false: no incoming synapse of neuron i arrived this tick, so
+// every release it would sum is +0 and its sums are +0 without reading them
+// (sums start at +0; +0 + +0 = +0) and no modulator bit is set.
 __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, const float *input,
-                                  float gain) {
+                                  float gain, bool arrived = true) {
     uint32_t flags = p.nflags[i];
     if (!(flags & NF_ACTIVE)) {
         uint32_t h = p.hist[i];
@@ -307,7 +310,7 @@ __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, co
     NeuronConst g;
     load_neuron(p, i, s, g, g_exp_tab);
     float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
-    unsigned mod = gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab);
+    const unsigned mod = arrived ? gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab) : 0u;
     float current = sensory_current(p, i, s, g, input, gain);
     int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, g_exp_tab);
     store_neuron(p, i, s);
@@ -459,14 +462,22 @@ __global__ void k_homeostasis_world(Pool p, int64_t n, const int32_t *age) {
 // per-tick phases separated by workgroup barriers instead of kernel launches
 // and its state kept in global memory (L2-resident for the call).
 template <int WG>
+// `hit` (LDS, hit_cap bytes; may be null) holds one arrival flag per neuron
+// of the brain when the brain fits: the delivery pass sets the flags of the
+// posts it reaches, the neuron pass reads and clears them and skips the
+// input gather of neurons without one.
 __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float *input,
-                                float gain, float *out, int homeostasis) {
+                                float gain, float *out, int homeostasis, uint8_t *hit, int64_t hit_cap) {
     __shared__ unsigned big_events;
     if (!(p.brun[b] & 1)) return;
     const int tid = threadIdx.x;
     const int64_t n0 = p.bn0[b], n1 = n0 + p.bnc[b];
     const int64_t s0 = p.bs0[b], s1 = s0 + p.bsc[b];
+    if (!hit || n1 - n0 > hit_cap) hit = nullptr;
+    if (hit)
+        for (int64_t i = tid; i < n1 - n0; i += WG) hit[i] = 0;
     if (tid == 0) big_events = 0;
+    __syncthreads();
     unsigned arrivals = 0;
     // Per tick: plasticity of tick t-1 and delivery of tick t in one synapse
     // pass (a synapse's plasticity precedes its next delivery in its own
@@ -474,10 +485,19 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
     for (int t = 0; t < ticks; ++t) {
         for (int64_t q = s0 + tid; q < s1; q += WG) {
             if (t > 0) plasticity_one(p, q, b);
-            arrivals += deliver_one(p, q, b, t);
+            const unsigned arr = deliver_one(p, q, b, t);
+            if (arr && hit) hit[p.spost[q] - n0] = 1;
+            arrivals += arr;
         }
         __syncthreads();
-        for (int64_t i = n0 + tid; i < n1; i += WG) neuron_one(p, i, b, t, input, gain);
+        for (int64_t i = n0 + tid; i < n1; i += WG) {
+            bool arrived = true;
+            if (hit) {
+                arrived = hit[i - n0] != 0;
+                hit[i - n0] = 0;
+            }
+            neuron_one(p, i, b, t, input, gain, arrived);
+        }
         __syncthreads();
         for (int m = tid; m < p.outputs; m += WG) {
             const int32_t mn = p.motor[int64_t(b) * p.outputs + m];
@@ -527,7 +547,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
     __shared__ unsigned events_l;
 
     if (int(blockIdx.x) < nbig) {
-        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis);
+        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis, nullptr, 0);
         return;
     }
     const Pack pk = packs[blockIdx.x - nbig];
@@ -824,7 +844,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __shared__ unsigned events_l;
 
     if (int(blockIdx.x) < nbig) {
-        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis);
+        // the pack's synapse arrays are free in a big brain's workgroup
+        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis, lds_syn,
+                            int64_t(smax) * lds_synapse_bytes<PLASTIC>());
         return;
     }
     const Pack pk = packs[blockIdx.x - nbig];
