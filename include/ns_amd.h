@@ -316,6 +316,37 @@ int ns_world_neural_step(void *pool, int64_t first_tick, const float *senses,
 /* homeostasis_step (main.cu:1287-1300): age[a] in world steps. */
 int ns_world_homeostasis(void *pool, const int *age, const uint8_t *frozen);
 
+/* Births and deaths between world steps (main.cu:1302-1338 reclaim_dead_
+ * genomes releases a dead animal's neuron and synapse slots; main.cu:364-454
+ * claim_slot and the genome lists give a newborn its slots).  Animals
+ * animals[0..animal_count) lose all their records; the given records (ascending
+ * slot order per array, each owned by a listed animal, checked like
+ * ns_pool_import_records, slots free once the listed animals' are released)
+ * become their new brains, state and rings as the records hold them.  A
+ * listed animal with no records just dies.  Each newborn goes to the best-
+ * fitting free pool slot; the pool is rebuilt (state kept) only when none
+ * fits or for LP/Prinz cells.  The result equals exporting the records,
+ * applying the same change and importing them.  Synchronous. */
+int ns_pool_replace_records(void *pool, int animal_count, const int *animals,
+                            const ns_checkpoint_neuron *neurons,
+                            int neuron_count,
+                            const ns_checkpoint_synapse *synapses,
+                            int synapse_count);
+
+/* Slot reserve for ns_pool_replace_records, as ns_gpu_reserve: rebuilds the
+ * pool (state kept) with capacities rounded up a ladder of ratio 1 + slack and
+ * spare slots for a fraction `spare` of the animals.  It stays in force for
+ * later imports.  No counterpart in the reference (its slots are single
+ * neurons and synapses in one bitmap). */
+int ns_pool_reserve(void *pool, float slack, float spare);
+
+typedef struct ns_pool_stats {
+    int64_t animals, neuron_records, synapse_records;
+    int64_t slots, free_slots;          /* pool brain slots; without a brain */
+    int64_t placements, relayouts;      /* newborns placed in free slots; rebuilds */
+} ns_pool_stats;
+int ns_pool_info(void *pool, ns_pool_stats *info);
+
 /* ------------------------------------------------------------------------ */
 /* 3. Serving: continuous batching of genome-evaluation episodes            */
 /* ------------------------------------------------------------------------ */

@@ -164,7 +164,7 @@ const TileShape *find_tile(const Tile &t) {
 
 // A brain plastic pair exists: a valid excitatory-to-excitatory synapse with a
 // plasticity profile (the pairs the plastic kernels keep traces for).
-static bool has_plastic_pair(const HostBrain &b) {
+bool has_plastic_pair(const HostBrain &b) {
     for (const auto &g : b.edges) {
         const ns_neuron_gene &a = b.genes[g.pre], &z = b.genes[g.post];
         if (g.plasticity && g.active && a.active && z.active && !a.inhibitory_transmitter &&
@@ -175,7 +175,7 @@ static bool has_plastic_pair(const HostBrain &b) {
     return false;
 }
 
-static bool has_slow_cell(const HostBrain &b) {
+bool has_slow_cell(const HostBrain &b) {
     for (const auto &g : b.genes)
         if (g.type == 2 || g.type == 3) return true;
     return false;
@@ -739,6 +739,103 @@ void DevicePool::order_touched(hipStream_t st) {
     touched.clear();
 }
 
+void DevicePool::stage_state(int pb, const HostBrain &brain, const ns_neuron_state *in_n,
+                             const uint32_t *hist, const uint64_t *stamps,
+                             const ns_synapse_state *in_s, int64_t tick, Stager &sg,
+                             std::vector<std::vector<unsigned char>> &keep) {
+    const int64_t N = view.ncap, S = view.scap, n0 = bn0[pb], s0 = bs0[pb];
+    const int nc = occ_n[pb], sc = occ_s[pb];
+    if (nc != int(brain.genes.size()) || sc != int(brain.edges.size()))
+        throw std::runtime_error("Slot occupant differs");
+    // the slot's synapse order, as fill_slot lays it out: by (post, edge index)
+    std::vector<int> start(nc + 1, 0), order(sc);
+    for (int j = 0; j < sc; ++j) start[brain.edges[j].post + 1]++;
+    for (int i = 0; i < nc; ++i) start[i + 1] += start[i];
+    for (int j = 0; j < sc; ++j) order[start[brain.edges[j].post]++] = j;
+    // one host buffer per field, staged as one range of the slot
+    auto field = [&](void *dst_base, int64_t index, size_t elem, int count, auto fill) {
+        if (!count) return;
+        keep.emplace_back(elem * size_t(count));
+        unsigned char *buf = keep.back().data();
+        fill(buf);
+        sg.add(static_cast<unsigned char *>(dst_base) + elem * size_t(index), buf, elem * size_t(count));
+    };
+    auto nsf = [&](int k, auto get) {
+        field(view.ns, k * N + n0, 4, nc, [&](unsigned char *b) {
+            float *f = reinterpret_cast<float *>(b);
+            for (int i = 0; i < nc; ++i) f[i] = get(in_n[i]);
+        });
+    };
+    nsf(NS_V, [](const ns_neuron_state &r) { return r.voltage_mv; });
+    nsf(NS_W, [](const ns_neuron_state &r) { return r.adaptation_pa; });
+    nsf(NS_ADP, [](const ns_neuron_state &r) { return r.afterdepol_pa; });
+    nsf(NS_INH, [](const ns_neuron_state &r) { return r.inh_ns; });
+    nsf(NS_DEPOL, [](const ns_neuron_state &r) { return r.recent_depol_mv; });
+    nsf(NS_LAS, [](const ns_neuron_state &r) { return r.log_ampa_scale; });
+    for (int d = 0; d < 4; ++d)
+        nsf(NS_D0 + d, [d](const ns_neuron_state &r) { return r.dendrite_exc_ns[d]; });
+    nsf(NS_RNS, [](const ns_neuron_state &r) { return r.receptor_ns; });
+    for (int c = 0; c < kCellFields; ++c)
+        field(view.cell, c * N + n0, 4, nc, [&](unsigned char *b) {
+            float *f = reinterpret_cast<float *>(b);
+            for (int i = 0; i < nc; ++i) f[i] = in_n[i].cell[c];
+        });
+    field(view.total, n0, 4, nc, [&](unsigned char *b) {
+        int32_t *f = reinterpret_cast<int32_t *>(b);
+        for (int i = 0; i < nc; ++i) f[i] = in_n[i].total_spikes;
+    });
+    field(view.spike, n0, 1, nc, [&](unsigned char *b) {
+        for (int i = 0; i < nc; ++i) b[i] = in_n[i].spike;
+    });
+    field(view.rrng, n0, 4, nc, [&](unsigned char *b) {
+        uint32_t *f = reinterpret_cast<uint32_t *>(b);
+        for (int i = 0; i < nc; ++i) f[i] = in_n[i].receptor_rng;
+    });
+    field(view.hist, n0, 4, nc, [&](unsigned char *b) { std::memcpy(b, hist, 4u * nc); });
+    field(view.modbits, n0, 1, nc, [&](unsigned char *b) { std::memset(b, 0, size_t(nc)); });
+    for (int e = 0; e < 6; ++e)
+        field(view.stamps, e * N + n0, 8, nc, [&](unsigned char *b) {
+            uint64_t *f = reinterpret_cast<uint64_t *>(b);
+            for (int i = 0; i < nc; ++i) f[i] = stamps ? stamps[i * 6 + e] : 0;
+        });
+    // synapses: pool position q holds genome edge order[q]
+    auto ssf = [&](int k, auto get) {
+        field(view.ss, k * S + s0, 4, sc, [&](unsigned char *b) {
+            float *f = reinterpret_cast<float *>(b);
+            for (int q = 0; q < sc; ++q) f[q] = get(in_s[order[q]]);
+        });
+    };
+    ssf(SS_WEIGHT, [](const ns_synapse_state &r) { return r.weight_ns; });
+    ssf(SS_UTIL, [](const ns_synapse_state &r) { return r.running_utilization; });
+    ssf(SS_PTRACE, [](const ns_synapse_state &r) { return r.pre_trace; });
+    ssf(SS_QTRACE, [](const ns_synapse_state &r) { return r.post_trace; });
+    ssf(SS_ELTP, [](const ns_synapse_state &r) { return r.eligible_ltp; });
+    ssf(SS_ELTD, [](const ns_synapse_state &r) { return r.eligible_ltd; });
+    ssf(SS_DA_AGE, [](const ns_synapse_state &r) { return r.da_age_ms; });
+    ssf(SS_SPINE, [](const ns_synapse_state &r) { return r.spine_volume; });
+    ssf(SS_SAGE, [](const ns_synapse_state &r) { return r.structural_age_ms; });
+    ssf(SS_PRE_AGE, [](const ns_synapse_state &r) { return r.pre_age_ms; });
+    ssf(SS_POST_AGE, [](const ns_synapse_state &r) { return r.post_age_ms; });
+    ssf(SS_SLOW, [](const ns_synapse_state &r) { return r.slow_resource; });
+    field(view.last, s0, 8, sc, [&](unsigned char *b) {
+        int64_t *f = reinterpret_cast<int64_t *>(b);
+        for (int q = 0; q < sc; ++q) f[q] = in_s[order[q]].last_arrival_tick;
+    });
+    field(view.ready, s0, 8, sc, [&](unsigned char *b) {
+        uint64_t *f = reinterpret_cast<uint64_t *>(b);
+        for (int q = 0; q < sc; ++q) f[q] = in_s[order[q]].ready_mask;
+    });
+    field(view.srng, s0, 4, sc, [&](unsigned char *b) {
+        uint32_t *f = reinterpret_cast<uint32_t *>(b);
+        for (int q = 0; q < sc; ++q) f[q] = in_s[order[q]].rng_state;
+    });
+    field(view.arrived, s0, 1, sc, [&](unsigned char *b) {
+        for (int q = 0; q < sc; ++q) b[q] = in_s[order[q]].arrived;
+    });
+    field(view.released, s0, 4, sc, [&](unsigned char *b) { std::memset(b, 0, 4u * size_t(sc)); });
+    field(view.btick, pb, 8, 1, [&](unsigned char *b) { std::memcpy(b, &tick, 8); });
+}
+
 // Neuron/synapse state in batch order, genome order within each brain.
 void DevicePool::download_state(std::vector<ns_neuron_state> &out_n,
                                 std::vector<ns_synapse_state> &out_s, std::vector<uint32_t> &hist,
@@ -761,9 +858,9 @@ void DevicePool::download_state(std::vector<ns_neuron_state> &out_n,
     if (stamps) down(st, view.stamps);
     // batch-order offsets
     std::vector<int64_t> noff(nbatch + 1, 0), soff(nbatch + 1, 0);
-    for (int i = 0; i < nbatch; ++i) {
-        noff[i + 1] = noff[i] + occ_n[pool_of[i]];
-        soff[i + 1] = soff[i] + occ_s[pool_of[i]];
+    for (int i = 0; i < nbatch; ++i) {   // a batch index without a slot holds no brain
+        noff[i + 1] = noff[i] + (pool_of[i] >= 0 ? occ_n[pool_of[i]] : 0);
+        soff[i + 1] = soff[i] + (pool_of[i] >= 0 ? occ_s[pool_of[i]] : 0);
     }
     out_n.assign(noff[nbatch], ns_neuron_state{});
     out_s.assign(soff[nbatch], ns_synapse_state{});
@@ -772,6 +869,7 @@ void DevicePool::download_state(std::vector<ns_neuron_state> &out_n,
     if (stamps) stamps->assign(noff[nbatch] * 6, 0);
     for (int i = 0; i < nbatch; ++i) {
         const int pb = pool_of[i];
+        if (pb < 0) continue;
         ticks[i] = btick[pb];
         for (int k = 0; k < occ_n[pb]; ++k) {
             const int64_t n = int64_t(bn0[pb]) + k;
@@ -829,15 +927,16 @@ void DevicePool::upload_state(const std::vector<ns_neuron_state> &in_n,
     std::vector<int64_t> last(S, 0), btick(std::max(nb, 1), 0);
     std::vector<uint64_t> ready(S, 0), st(6 * N, 0);
     std::vector<int64_t> noff(nbatch + 1, 0), soff(nbatch + 1, 0);
-    for (int i = 0; i < nbatch; ++i) {
-        noff[i + 1] = noff[i] + occ_n[pool_of[i]];
-        soff[i + 1] = soff[i] + occ_s[pool_of[i]];
+    for (int i = 0; i < nbatch; ++i) {   // a batch index without a slot holds no brain
+        noff[i + 1] = noff[i] + (pool_of[i] >= 0 ? occ_n[pool_of[i]] : 0);
+        soff[i + 1] = soff[i] + (pool_of[i] >= 0 ? occ_s[pool_of[i]] : 0);
     }
     if (int64_t(in_n.size()) != noff[nbatch] || int64_t(in_s.size()) != soff[nbatch] ||
         int64_t(hist.size()) != noff[nbatch] || int(ticks.size()) != nbatch)
         throw std::runtime_error("State dimensions differ");
     for (int i = 0; i < nbatch; ++i) {
         const int pb = pool_of[i];
+        if (pb < 0) continue;
         btick[pb] = ticks[i];
         for (int k = 0; k < occ_n[pb]; ++k) {
             const int64_t n = int64_t(bn0[pb]) + k;
@@ -939,11 +1038,13 @@ static int ladder(int n, float slack) {
     return int(std::ceil(c));
 }
 
-std::vector<BrainSpec> Batch::slot_specs(const Tile &t) const {
+std::vector<BrainSpec> reserve_slots(const std::vector<BrainSpec> &specs, float slack, float spare,
+                                     const Tile &t, bool world) {
     // A brain that fits a pack keeps a slot that fits one: its rung is cut
     // at the tile's pack limits (the smaller LDS limit, as the pool's
     // plasticity is decided at build), so slack never sends it to the slower
     // one-workgroup-per-brain path.
+    const int count = int(specs.size());
     const TileShape *shape = find_tile(t);
     const int nmax = t.nmax();
     const int smax = t.ks > 0 ? t.threads * t.ks
@@ -982,6 +1083,10 @@ std::vector<BrainSpec> Batch::slot_specs(const Tile &t) const {
         out.insert(out.end(), extra.begin(), extra.end());
     }
     return out;
+}
+
+std::vector<BrainSpec> Batch::slot_specs(const Tile &t) const {
+    return reserve_slots(specs, slack, spare, t, world);
 }
 
 void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const uint32_t *seeds,

@@ -37,6 +37,10 @@ struct HostBrain {
 };
 
 ns_neuron_gene profile_gene(int type);
+// A valid excitatory-to-excitatory synapse with a plasticity profile.
+bool has_plastic_pair(const HostBrain &b);
+// LP or Prinz cells (type 2 or 3), which run on the per-tick kernels.
+bool has_slow_cell(const HostBrain &b);
 HostBrain *parse_gym_brain(int count, int edges, int channels, int outputs, const float *ng,
                            const float *sg, const int *motors, int max_delay);
 
@@ -158,6 +162,14 @@ struct Tile {
 };
 const TileShape *find_tile(const Tile &t);
 
+// Pool slots for `specs` under a slot reserve: capacities rounded up a
+// geometric ladder of ratio 1 + slack (cut at the tile's pack limits for
+// brains that fit a pack), plus empty slots for a fraction `spare` of the
+// brains at quantiles of the slot sizes and a few "jumbo" slots as large as
+// the largest.
+std::vector<BrainSpec> reserve_slots(const std::vector<BrainSpec> &specs, float slack, float spare,
+                                     const Tile &tile, bool world);
+
 // Structure-of-arrays device pool over a set of brains, laid out in packs.
 class DevicePool {
 public:
@@ -225,6 +237,15 @@ public:
     // Recompute on the device the slot orders of the packs the last place()
     // filled; call after the staged images have been scattered.
     void order_touched(hipStream_t st);
+    // Stage the state of the occupant of slot pb (placed by place() in the
+    // same staging): neurons in genome order with their shift registers and
+    // modulator stamps (6 per neuron), synapses in genome edge order, and the
+    // slot's tick.  The staged field buffers are kept in `keep` until the
+    // upload.
+    void stage_state(int pb, const HostBrain &brain, const ns_neuron_state *neurons,
+                     const uint32_t *hist, const uint64_t *stamps,
+                     const ns_synapse_state *synapses, int64_t tick, struct Stager &sg,
+                     std::vector<std::vector<unsigned char>> &keep);
     // Stage the images of `moves` (slot, occupant or nullptr = vacate) into
     // `sg`: genes, flags, routes, endpoints, motors and the affected packs'
     // slot orders, and update the host-side slot tables.  State is left

@@ -67,6 +67,12 @@ class ServeStats(ct.Structure):
                 ('launches', ct.c_int64), ('resident_peak', ct.c_int64), ('busy_s', ct.c_double)]
 
 
+class PoolStats(ct.Structure):
+    _fields_ = [('animals', ct.c_int64), ('neuron_records', ct.c_int64), ('synapse_records', ct.c_int64),
+                ('slots', ct.c_int64), ('free_slots', ct.c_int64), ('placements', ct.c_int64),
+                ('relayouts', ct.c_int64)]
+
+
 # Exported symbols of include/ns_amd.h with their ctypes signatures.
 SIGNATURES = {
     'ns_error': ([], ct.c_char_p),
@@ -99,6 +105,9 @@ SIGNATURES = {
     'ns_pool_export_records': ([_vp, _vp, ct.c_int, _vp, ct.c_int, _i32, _i32], ct.c_int),
     'ns_world_neural_step': ([_vp, ct.c_int64, _f, _i32, _u8], ct.c_int),
     'ns_world_homeostasis': ([_vp, _i32, _u8], ct.c_int),
+    'ns_pool_replace_records': ([_vp, ct.c_int, _i32, _vp, ct.c_int, _vp, ct.c_int], ct.c_int),
+    'ns_pool_reserve': ([_vp, ct.c_float, ct.c_float], ct.c_int),
+    'ns_pool_info': ([_vp, ct.POINTER(PoolStats)], ct.c_int),
     'ns_serve_create': ([ct.POINTER(ServeConfig)], _vp),
     'ns_serve_destroy': ([_vp], None),
     'ns_serve_submit': ([_vp, ct.c_int, ct.c_int, _f, _f, _i32, ct.c_uint32, ct.c_uint32],

@@ -23,11 +23,11 @@ SYNAPSE_RECORD = np.dtype({
     'offsets': [0, 4, 8, 12, 56], 'itemsize': 136})
 
 
-def build_world(seed=7, animals=10):
+def build_world(seed=7, animals=10, lo=30, hi=60):
     """Records for `animals` animals with interleaved slots; genes from the
     small mixed workload, ports 0..18 sensory, 19..23 motors."""
     lib = O.oracle_lib()
-    w = W.small_mix(seed=seed, brains=animals, types=(0, 1, 4, 5, 6), lo=30, hi=60,
+    w = W.small_mix(seed=seed, brains=animals, types=(0, 1, 4, 5, 6), lo=lo, hi=hi,
                     modulator_p=0.15)
     rng = np.random.default_rng(seed)
     neurons, synapses = [], []
@@ -188,3 +188,153 @@ def test_world_steps_match_oracle_and_round_trip_records():
         L.ns_pool_destroy(pool2)
     finally:
         L.ns_pool_destroy(pool)
+
+
+def _field_bytes(dtype, base=0, out=None):
+    """Byte mask of the bytes a (nested) structured dtype's fields cover:
+    numpy leaves the holes between fields unset when it copies records
+    field by field (boolean indexing), so record comparisons skip them."""
+    out = np.zeros(dtype.itemsize, bool) if out is None else out
+    for name in dtype.names:
+        sub, off = dtype.fields[name][:2]
+        if sub.names:
+            _field_bytes(sub, base + off, out)
+        else:
+            out[base + off:base + off + sub.itemsize] = True
+    return out
+
+
+def _same_records(x, y):
+    if len(x) != len(y):
+        return False
+    m = _field_bytes(x.dtype)
+    rx = np.ascontiguousarray(x).view(np.uint8).reshape(len(x), -1)[:, m]
+    ry = np.ascontiguousarray(y).view(np.uint8).reshape(len(y), -1)[:, m]
+    return np.array_equal(rx, ry)
+
+
+def _export(L, pool, ncap, scap):
+    n = np.zeros(ncap, NEURON_RECORD)
+    s = np.zeros(scap, SYNAPSE_RECORD)
+    cn, cs = ct.c_int(), ct.c_int()
+    assert L.ns_pool_export_records(pool, n.ctypes.data, ncap, s.ctypes.data, scap,
+                                    ct.byref(cn), ct.byref(cs)) == 0, ns_amd.error()
+    return n[:cn.value].copy(), s[:cs.value].copy()
+
+
+def _newborn(dn, ds, donor, owner, nslots, sslots):
+    """Donor animal's records relabelled to `owner` on the given free slots
+    (its synapses' endpoints follow their neurons)."""
+    mine = dn[dn['owner'] == donor].copy()
+    syn = ds[ds['owner'] == donor].copy()
+    where = {int(old): int(new) for old, new in zip(mine['slot'], nslots)}
+    mine['slot'] = nslots
+    mine['owner'] = owner
+    g = syn['gene'].copy()
+    g['pre'] = [where[int(x)] for x in g['pre']]
+    g['post'] = [where[int(x)] for x in g['post']]
+    syn['gene'] = g
+    syn['slot'] = sslots
+    syn['owner'] = owner
+    return mine, syn
+
+
+def test_births_and_deaths_equal_export_merge_reimport():
+    """ns_pool_replace_records with a slot reserve (placements into free
+    slots, one forced rebuild, births into empty animal indices, deaths
+    without births, before the first step and between steps, then a skipped
+    tick) equals exporting the records, applying the same change and
+    importing them, bit for bit, after every change and every step."""
+    L = ns_amd.lib()
+    nr, sr, _ = build_world(seed=11, animals=12)
+    dn, ds, _ = build_world(seed=12, animals=8)             # donor genomes for newborns
+    bn, bs, _ = build_world(seed=13, animals=1, lo=150, hi=160)   # larger than every slot
+    P, ncap, scap = 16, 4000, 16000
+    print(f'[world-births] animals={P} neurons={len(nr)} synapses={len(sr)}', flush=True)
+    A = L.ns_pool_create(P, ncap, scap)
+    B = L.ns_pool_create(P, ncap, scap)
+    assert A and B, ns_amd.error()
+    try:
+        assert L.ns_pool_reserve(A, 0.19, 0.25) == 0, ns_amd.error()
+        for pl in (A, B):
+            assert L.ns_pool_import_records(pl, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == 0
+        # (dead, [(donor records, donor, owner)]) per change; step index -> change
+        changes = {
+            0: ([3, 12], [(dn, ds, 0, 3), (dn, ds, 1, 12)]),            # before the first step
+            2: ([0, 5, 13], [(dn, ds, 2, 0), (dn, ds, 3, 13)]),          # 5 just dies
+            4: ([7], [(bn, bs, 0, 7)]),                                  # no slot fits: rebuild
+            5: ([1, 2, 5], [(dn, ds, 4, 2), (dn, ds, 5, 5), (dn, ds, 6, 1)]),
+        }
+        rng = np.random.default_rng(5)
+        motors_a = np.zeros(P * 5, np.int32)
+        motors_b = np.zeros(P * 5, np.int32)
+        frozen = np.zeros(P, np.uint8)
+        frozen[4] = 1
+        tick = 3000
+        for step in range(8):
+            if step in changes:
+                dead, born = changes[step]
+                en, es = _export(L, B, ncap, scap)
+                keep_n = en[~np.isin(en['owner'], dead)]
+                keep_s = es[~np.isin(es['owner'], dead)]
+                free_n = np.setdiff1d(np.arange(ncap), keep_n['slot'])
+                free_s = np.setdiff1d(np.arange(scap), keep_s['slot'])
+                new_n, new_s = [], []
+                for src_n, src_s, donor, owner in born:
+                    k = int((src_n['owner'] == donor).sum())
+                    m = int((src_s['owner'] == donor).sum())
+                    a, b = _newborn(src_n, src_s, donor, owner, free_n[:k], free_s[:m])
+                    free_n, free_s = free_n[k:], free_s[m:]
+                    new_n.append(a)
+                    new_s.append(b)
+                new_n = O.cat_records(new_n, NEURON_RECORD)             # keeps the record padding
+                new_s = O.cat_records(new_s, SYNAPSE_RECORD)
+                new_n = O.cat_records([new_n[np.argsort(new_n['slot'])]], NEURON_RECORD)
+                new_s = O.cat_records([new_s[np.argsort(new_s['slot'])]], SYNAPSE_RECORD)
+                d = np.array(dead, np.int32)
+                assert L.ns_pool_replace_records(A, len(d), d.ctypes.data_as(O._i32), new_n.ctypes.data,
+                                                 len(new_n), new_s.ctypes.data, len(new_s)) == 0, ns_amd.error()
+                mn = O.cat_records([keep_n, new_n], NEURON_RECORD)
+                ms = O.cat_records([keep_s, new_s], SYNAPSE_RECORD)
+                mn = O.cat_records([mn[np.argsort(mn['slot'])]], NEURON_RECORD)
+                ms = O.cat_records([ms[np.argsort(ms['slot'])]], SYNAPSE_RECORD)
+                assert L.ns_pool_import_records(B, mn.ctypes.data, len(mn), ms.ctypes.data, len(ms)) == 0
+                xa, xb = _export(L, A, ncap, scap), _export(L, B, ncap, scap)
+                assert _same_records(xa[0], xb[0]), f'neurons after change {step}'
+                assert _same_records(xa[1], xb[1]), f'synapses after change {step}'
+            senses = (rng.random(P * 19) < 0.5).astype(np.float32)
+            tick += 48 if step == 7 else 16                          # the last step skips two
+            for pl, m in ((A, motors_a), (B, motors_b)):
+                assert L.ns_world_neural_step(pl, tick, senses.ctypes.data_as(O._f), m.ctypes.data_as(O._i32),
+                                              frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
+                age = (np.arange(P) * 2 + step).astype(np.int32)
+                assert L.ns_world_homeostasis(pl, age.ctypes.data_as(O._i32),
+                                              frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
+            assert np.array_equal(motors_a, motors_b), f'motors at step {step}'
+            xa, xb = _export(L, A, ncap, scap), _export(L, B, ncap, scap)
+            assert _same_records(xa[0], xb[0]), f'neurons at step {step}'
+            assert _same_records(xa[1], xb[1]), f'synapses at step {step}'
+        assert motors_a.any()
+        info = ns_amd.PoolStats()
+        assert L.ns_pool_info(A, ct.byref(info)) == 0
+        assert info.placements == 7 and info.relayouts == 1, (info.placements, info.relayouts)
+        assert info.animals == 14 and info.free_slots > 0
+        # rejected changes leave the pool as it was
+        before = _export(L, A, ncap, scap)
+        en, es = before
+        clash_n = en[en['owner'] == 6][:1].copy()                # a survivor's slot
+        clash_n['owner'] = 9
+        d = np.array([9], np.int32)
+        assert L.ns_pool_replace_records(A, 1, d.ctypes.data_as(O._i32), clash_n.ctypes.data, 1, None, 0) == -1
+        assert 'slot in use' in ns_amd.error()
+        d = np.array([8], np.int32)
+        assert L.ns_pool_replace_records(A, 1, d.ctypes.data_as(O._i32), clash_n.ctypes.data, 1, None, 0) == -1
+        assert 'owner not replaced' in ns_amd.error()
+        d = np.array([8, 8], np.int32)
+        assert L.ns_pool_replace_records(A, 2, d.ctypes.data_as(O._i32), None, 0, None, 0) == -1
+        assert 'Invalid animal index' in ns_amd.error()
+        after = _export(L, A, ncap, scap)
+        assert _same_records(before[0], after[0]) and _same_records(before[1], after[1])
+    finally:
+        L.ns_pool_destroy(A)
+        L.ns_pool_destroy(B)
