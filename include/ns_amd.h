@@ -421,6 +421,67 @@ int ns_serve_stats_get(void *server, ns_serve_stats *stats);
  * Host only; for callers that replay or display episodes. */
 void ns_maze_layout(uint32_t seed, uint32_t index, uint32_t *rows, uint32_t *goal);
 
+/* ------------------------------------------------------------------------ */
+/* 4. NSCHKP01 checkpoint files (src/checkpoint_impl.cuh)                    */
+/* ------------------------------------------------------------------------ */
+
+/* checkpoint_impl.cuh:6-19 CheckpointHeader (608 B). */
+typedef struct ns_checkpoint_header {
+    char magic[8];                     /* "NSCHKP01" */
+    uint32_t version, endian;          /* 1, 0x01020304 */
+    uint64_t layout;                   /* ns_checkpoint_layout() */
+    int64_t step, checkpoint_every;
+    unsigned long long seed;
+    int32_t initial, trace_animal;
+    float food_renewal;
+    uint32_t flags, neuron_count, synapse_count;
+    uint64_t csv_size, behavior_size, lives_size;
+    char output[512];
+} ns_checkpoint_header;
+
+/* checkpoint_impl.cuh:21-32 checkpoint_layout: FNV-1a over the world build's
+ * capacities and struct sizes (32768 animals, 4194304 neurons, 8388608
+ * synapses, 19 senses, 5 motors, a 128 x 128 world, ...). */
+uint64_t ns_checkpoint_layout(void);
+
+/* Read a whole checkpoint file with the reference's checks, in its order
+ * (main.cu:1986-2000 header, checkpoint_impl.cuh:201-366 state, 368-383
+ * header checks): incomplete or corrupt data, trailing data, an
+ * incompatible header or layout, history counts, animal counts against the
+ * record counts, and every record (ascending slots, owner alive, bounds)
+ * are rejected with the reference's messages.  The handle holds the header,
+ * the records, and the world sections as one block in file order: observer
+ * totals and histories, animals, food, kind, senses, motors, counters,
+ * traces, then the plasticity-cohort extension when flags & 64.  NULL on
+ * failure. */
+void *ns_checkpoint_open(const char *path);
+int ns_checkpoint_get(void *checkpoint, ns_checkpoint_header *header,
+                      const void **world, size_t *world_bytes,
+                      const ns_checkpoint_neuron **neurons,
+                      const ns_checkpoint_synapse **synapses);
+void ns_checkpoint_close(void *checkpoint);
+
+/* checkpoint_impl.cuh:135-199, 385-413 checkpoint_atomic_save: gzip level 1
+ * into path.tmp, fsync, rename onto path, fsync of the directory.  The
+ * header's record counts are set from the arguments; the world block is laid
+ * out as ns_checkpoint_open returns it; records must be in ascending slot
+ * order and owned by living animals whose counts match.  Nothing is written
+ * when a check fails. */
+int ns_checkpoint_save(const char *path, const ns_checkpoint_header *header,
+                       const void *world, size_t world_bytes,
+                       const ns_checkpoint_neuron *neurons, int neuron_count,
+                       const ns_checkpoint_synapse *synapses, int synapse_count);
+
+/* The same with a world pool's records (ns_pool_export_records /
+ * ns_pool_import_records).  Loading checks the whole file before the pool
+ * changes; world may be NULL (only *world_bytes is set). */
+int ns_pool_save_checkpoint(void *pool, const char *path,
+                            const ns_checkpoint_header *header,
+                            const void *world, size_t world_bytes);
+int ns_pool_load_checkpoint(void *pool, const char *path,
+                            ns_checkpoint_header *header, void *world,
+                            size_t world_capacity, size_t *world_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,15 @@ class PoolStats(ct.Structure):
                 ('relayouts', ct.c_int64)]
 
 
+class CheckpointHeader(ct.Structure):
+    _fields_ = [('magic', ct.c_char * 8), ('version', ct.c_uint32), ('endian', ct.c_uint32),
+                ('layout', ct.c_uint64), ('step', ct.c_int64), ('checkpoint_every', ct.c_int64),
+                ('seed', ct.c_ulonglong), ('initial', ct.c_int32), ('trace_animal', ct.c_int32),
+                ('food_renewal', ct.c_float), ('flags', ct.c_uint32), ('neuron_count', ct.c_uint32),
+                ('synapse_count', ct.c_uint32), ('csv_size', ct.c_uint64), ('behavior_size', ct.c_uint64),
+                ('lives_size', ct.c_uint64), ('output', ct.c_char * 512)]
+
+
 # Exported symbols of include/ns_amd.h with their ctypes signatures.
 SIGNATURES = {
     'ns_error': ([], ct.c_char_p),
@@ -115,6 +124,16 @@ SIGNATURES = {
     'ns_serve_poll': ([_vp, ct.POINTER(ServeResult), ct.c_int], ct.c_int),
     'ns_serve_drain': ([_vp], ct.c_int),
     'ns_serve_stats_get': ([_vp, ct.POINTER(ServeStats)], ct.c_int),
+    'ns_checkpoint_layout': ([], ct.c_uint64),
+    'ns_checkpoint_open': ([ct.c_char_p], _vp),
+    'ns_checkpoint_get': ([_vp, ct.POINTER(CheckpointHeader), ct.POINTER(_vp), ct.POINTER(ct.c_size_t),
+                           ct.POINTER(_vp), ct.POINTER(_vp)], ct.c_int),
+    'ns_checkpoint_close': ([_vp], None),
+    'ns_checkpoint_save': ([ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t, _vp, ct.c_int, _vp,
+                            ct.c_int], ct.c_int),
+    'ns_pool_save_checkpoint': ([_vp, ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t], ct.c_int),
+    'ns_pool_load_checkpoint': ([_vp, ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t,
+                                 ct.POINTER(ct.c_size_t)], ct.c_int),
     'ns_maze_layout': ([ct.c_uint32, ct.c_uint32, ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_uint32)],
                        None),
 }

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -1088,6 +1089,74 @@ int ora_maze_request(void *p, uint32_t task_seed, int mazes, int horizon, int ti
     motors[0] = int32_t(mot[0]);
     motors[1] = int32_t(mot[1]);
     return total;
+}
+
+// World structs of the checkpoint container, field lists as src/main.cu:66-104
+// and 1845-1849 declare them (S = 19, M = 5), laid out by this compiler:
+// out = {sizeof Animal, offsetof alive, neuron_count, synapse_count,
+// sizeof BehaviorTotals, LifeRecord, BiteEvent, PlasticityCohort, Behavior,
+// CheckpointHeader (checkpoint_impl.cuh:6-19)}.
+void ora_world_layout(uint64_t *out) {
+    struct Behavior {
+        int opportunities[3][2][3], accepted[3][2][3];
+        int empty_steps[3][2], empty_eat[3][2];
+        float approach_progress[3][3], approach_travel[3][3];
+        int first_b_age, offspring, death_cause;
+    };
+    struct Animal {
+        float x, y, heading, energy, health, fullness, pain;
+        float taste[4], muscle[3];
+        unsigned long long rng;
+        int id, parent_id, generation, age, alive;
+        int neuron_head, synapse_head, neuron_count, synapse_count;
+        int ports[19 + 5];
+        Behavior behavior;
+    };
+    struct LifeRecord {
+        Animal body;
+        int64_t death_step;
+        int active_neurons, active_synapses, modulatory_neurons, plastic_synapses;
+        float mean_abs_weight_change;
+    };
+    struct BiteEvent {
+        int64_t step;
+        int attacker_id, prey_id;
+        float x, y;
+    };
+    struct PlasticityCohort {
+        int id, assigned, frozen, initial_age, initial_offspring;
+        int final_age, final_offspring, alive, death_cause;
+        int neurons, synapses, generation;
+        float initial_energy, initial_health;
+        int64_t start_step, observed_step, death_step;
+    };
+    struct BehaviorTotals {
+        uint64_t opportunities[3][2][3], accepted[3][2][3];
+        uint64_t empty_steps[3][2], empty_eat[3][2];
+        double approach_progress[3][3], approach_travel[3][3];
+    };
+    struct Header {
+        char magic[8];
+        uint32_t version, endian;
+        uint64_t layout;
+        int64_t step, checkpoint_every;
+        unsigned long long seed;
+        int initial, trace_animal;
+        float food_renewal;
+        uint32_t flags, neuron_count, synapse_count;
+        uint64_t csv_size, behavior_size, lives_size;
+        char output[512];
+    };
+    out[0] = sizeof(Animal);
+    out[1] = offsetof(Animal, alive);
+    out[2] = offsetof(Animal, neuron_count);
+    out[3] = offsetof(Animal, synapse_count);
+    out[4] = sizeof(BehaviorTotals);
+    out[5] = sizeof(LifeRecord);
+    out[6] = sizeof(BiteEvent);
+    out[7] = sizeof(PlasticityCohort);
+    out[8] = sizeof(Behavior);
+    out[9] = sizeof(Header);
 }
 
 }  // extern "C"

@@ -338,3 +338,66 @@ def test_births_and_deaths_equal_export_merge_reimport():
     finally:
         L.ns_pool_destroy(A)
         L.ns_pool_destroy(B)
+
+
+def test_checkpoint_resume_equals_continuous_run(tmp_path):
+    """The protocol of the reference's tests/checkpoint_roundtrip.py on the
+    world pool: 12 world steps in one run equal 6 steps, an NSCHKP01
+    checkpoint, a resume into a fresh pool and 6 more steps (motors every
+    step, records at the end, bit for bit); a truncated checkpoint is
+    refused and leaves the pool as it was."""
+    import checkpoint as C
+    L = ns_amd.lib()
+    nr, sr, _ = build_world(seed=21, animals=12)
+    counts = {a: (int((nr['owner'] == a).sum()), int((sr['owner'] == a).sum())) for a in range(12)}
+    world = C.world_block(np.random.default_rng(21), counts)
+    head = ns_amd.CheckpointHeader.from_buffer_copy(
+        C.header(L.ns_checkpoint_layout(), step=6, initial=12, flags=1))
+    P = C.P
+    print(f'[world-checkpoint] animals=12 of {P} neurons={len(nr)} synapses={len(sr)}', flush=True)
+    A, B, R = (L.ns_pool_create(P, C.NEURON_CAPACITY, C.SYNAPSE_CAPACITY) for _ in range(3))
+    assert A and B and R, ns_amd.error()
+    try:
+        for pl in (A, B):
+            assert L.ns_pool_import_records(pl, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == 0
+        frozen = np.zeros(P, np.uint8)
+        frozen[2] = 1
+
+        def step(pl, k):
+            senses = (np.random.default_rng(100 + k).random(P * 19) < 0.5).astype(np.float32)
+            m = np.zeros(P * 5, np.int32)
+            assert L.ns_world_neural_step(pl, 9000 + 16 * k, senses.ctypes.data_as(O._f), m.ctypes.data_as(O._i32),
+                                          frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
+            age = (np.arange(P) * 3 + k).astype(np.int32)
+            assert L.ns_world_homeostasis(pl, age.ctypes.data_as(O._i32), frozen.ctypes.data_as(O._u8)) == 0
+            return m
+
+        for k in range(6):
+            assert np.array_equal(step(A, k), step(B, k))
+        path = tmp_path / 'world.checkpoint.gz'
+        assert L.ns_pool_save_checkpoint(B, str(path).encode(), ct.byref(head), world, len(world)) == 0, \
+            ns_amd.error()
+        got = ns_amd.CheckpointHeader()
+        wb = ct.c_size_t()
+        assert L.ns_pool_load_checkpoint(R, str(path).encode(), ct.byref(got), None, 0, ct.byref(wb)) == 0, \
+            ns_amd.error()
+        assert wb.value == len(world) and got.neuron_count == len(nr) and got.synapse_count == len(sr)
+        back = ct.create_string_buffer(len(world))
+        assert L.ns_pool_load_checkpoint(R, str(path).encode(), None, back, len(world), None) == 0
+        assert back.raw == world
+        for k in range(6, 12):
+            ma, mr = step(A, k), step(R, k)
+            assert np.array_equal(ma, mr), f'motors at step {k}'
+        xa, xr = _export(L, A, len(nr), len(sr)), _export(L, R, len(nr), len(sr))
+        assert _same_records(xa[0], xr[0]) and _same_records(xa[1], xr[1])
+        assert xa[0]['state']['total_spikes'].sum() > 0      # the brains were active
+        bad = tmp_path / 'corrupt.gz'
+        payload = path.read_bytes()
+        bad.write_bytes(payload[:len(payload) // 2])
+        assert L.ns_pool_load_checkpoint(R, str(bad).encode(), None, None, 0, None) == -1
+        assert 'incomplete or corrupt' in ns_amd.error()
+        after = _export(L, R, len(nr), len(sr))
+        assert _same_records(after[0], xr[0]) and _same_records(after[1], xr[1])
+    finally:
+        for pl in (A, B, R):
+            L.ns_pool_destroy(pl)
