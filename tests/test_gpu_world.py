@@ -23,11 +23,11 @@ SYNAPSE_RECORD = np.dtype({
     'offsets': [0, 4, 8, 12, 56], 'itemsize': 136})
 
 
-def build_world(seed=7, animals=10, lo=30, hi=60):
+def build_world(seed=7, animals=10, lo=30, hi=60, types=(0, 1, 4, 5, 6)):
     """Records for `animals` animals with interleaved slots; genes from the
     small mixed workload, ports 0..18 sensory, 19..23 motors."""
     lib = O.oracle_lib()
-    w = W.small_mix(seed=seed, brains=animals, types=(0, 1, 4, 5, 6), lo=lo, hi=hi,
+    w = W.small_mix(seed=seed, brains=animals, types=types, lo=lo, hi=hi,
                     modulator_p=0.15)
     rng = np.random.default_rng(seed)
     neurons, synapses = [], []
@@ -112,10 +112,11 @@ def oracle_world(nr, sr, brains):
     return handles
 
 
-def test_world_steps_match_oracle_and_round_trip_records():
+@pytest.mark.parametrize('types', [(0, 1, 4, 5, 6), (0, 1, 2, 3, 5)], ids=['adex-lfs', 'lp-prinz'])
+def test_world_steps_match_oracle_and_round_trip_records(types):
     L = ns_amd.lib()
     lib = O.oracle_lib()
-    nr, sr, brains = build_world()
+    nr, sr, brains = build_world(types=types)
     P = len(brains)
     print(f'[world] animals={P} neurons={len(nr)} synapses={len(sr)}', flush=True)
     pool = L.ns_pool_create(P, int(nr['slot'].max()) + 1, int(sr['slot'].max()) + 1)
@@ -249,6 +250,7 @@ def test_births_and_deaths_equal_export_merge_reimport():
     nr, sr, _ = build_world(seed=11, animals=12)
     dn, ds, _ = build_world(seed=12, animals=8)             # donor genomes for newborns
     bn, bs, _ = build_world(seed=13, animals=1, lo=150, hi=160)   # larger than every slot
+    ln, ls, _ = build_world(seed=14, animals=1, types=(0, 2, 3, 4))   # LP/Prinz cells: a rebuild
     P, ncap, scap = 16, 4000, 16000
     print(f'[world-births] animals={P} neurons={len(nr)} synapses={len(sr)}', flush=True)
     A = L.ns_pool_create(P, ncap, scap)
@@ -264,6 +266,7 @@ def test_births_and_deaths_equal_export_merge_reimport():
             2: ([0, 5, 13], [(dn, ds, 2, 0), (dn, ds, 3, 13)]),          # 5 just dies
             4: ([7], [(bn, bs, 0, 7)]),                                  # no slot fits: rebuild
             5: ([1, 2, 5], [(dn, ds, 4, 2), (dn, ds, 5, 5), (dn, ds, 6, 1)]),
+            6: ([9], [(ln, ls, 0, 9)]),
         }
         rng = np.random.default_rng(5)
         motors_a = np.zeros(P * 5, np.int32)
@@ -317,7 +320,7 @@ def test_births_and_deaths_equal_export_merge_reimport():
         assert motors_a.any()
         info = ns_amd.PoolStats()
         assert L.ns_pool_info(A, ct.byref(info)) == 0
-        assert info.placements == 7 and info.relayouts == 1, (info.placements, info.relayouts)
+        assert info.placements == 7 and info.relayouts == 2, (info.placements, info.relayouts)
         assert info.animals == 14 and info.free_slots > 0
         # rejected changes leave the pool as it was
         before = _export(L, A, ncap, scap)
