@@ -404,3 +404,56 @@ def test_checkpoint_resume_equals_continuous_run(tmp_path):
     finally:
         for pl in (A, B, R):
             L.ns_pool_destroy(pl)
+
+
+def test_replace_into_empty_pool_and_after_every_death():
+    """Edges of ns_pool_replace_records: births into a pool that holds no
+    records (the first build), every animal dying (steps then do nothing and
+    export nothing), and births after that; each state equals importing the
+    same records."""
+    L = ns_amd.lib()
+    nr, sr, _ = build_world(seed=31, animals=4)
+    P, ncap, scap = 6, int(nr['slot'].max()) + 1, int(sr['slot'].max()) + 1
+    A = L.ns_pool_create(P, ncap, scap)
+    B = L.ns_pool_create(P, ncap, scap)
+    assert A and B, ns_amd.error()
+    print(f'[world-edges] animals={P} neurons={len(nr)} synapses={len(sr)}', flush=True)
+    senses = (np.random.default_rng(4).random(P * 19) < 0.5).astype(np.float32)
+    frozen = np.zeros(P, np.uint8)
+
+    def step(pl, tick):
+        m = np.zeros(P * 5, np.int32)
+        assert L.ns_world_neural_step(pl, tick, senses.ctypes.data_as(O._f), m.ctypes.data_as(O._i32),
+                                      frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
+        return m
+
+    try:
+        all4 = np.arange(4, dtype=np.int32)
+        # births into an empty pool: the first build
+        assert L.ns_pool_replace_records(A, 4, all4.ctypes.data_as(O._i32), nr.ctypes.data, len(nr),
+                                         sr.ctypes.data, len(sr)) == 0, ns_amd.error()
+        assert L.ns_pool_import_records(B, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == 0
+        for k in range(3):
+            assert np.array_equal(step(A, 100 + 16 * k), step(B, 100 + 16 * k))
+        xa, xb = _export(L, A, len(nr), len(sr)), _export(L, B, len(nr), len(sr))
+        assert _same_records(xa[0], xb[0]) and _same_records(xa[1], xb[1])
+        # every animal dies: nothing runs, nothing is exported
+        assert L.ns_pool_replace_records(A, 4, all4.ctypes.data_as(O._i32), None, 0, None, 0) == 0
+        info = ns_amd.PoolStats()
+        assert L.ns_pool_info(A, ct.byref(info)) == 0
+        assert info.animals == 0 and info.neuron_records == 0 and info.free_slots == info.slots
+        assert not step(A, 148).any()
+        n0, s0 = _export(L, A, 1, 1)
+        assert len(n0) == 0 and len(s0) == 0
+        # and births again, into the freed slots, from the exported state
+        assert L.ns_pool_replace_records(A, 4, all4.ctypes.data_as(O._i32), xa[0].ctypes.data, len(xa[0]),
+                                         xa[1].ctypes.data, len(xa[1])) == 0, ns_amd.error()
+        assert L.ns_pool_import_records(B, xa[0].ctypes.data, len(xa[0]), xa[1].ctypes.data, len(xa[1])) == 0
+        for k in range(3):
+            assert np.array_equal(step(A, 148 + 16 * k), step(B, 148 + 16 * k))
+        xa, xb = _export(L, A, len(nr), len(sr)), _export(L, B, len(nr), len(sr))
+        assert _same_records(xa[0], xb[0]) and _same_records(xa[1], xb[1])
+        assert xa[0]['state']['total_spikes'].sum() > 0
+    finally:
+        L.ns_pool_destroy(A)
+        L.ns_pool_destroy(B)
