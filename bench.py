@@ -336,11 +336,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
-    ap.add_argument('--warmup', type=int, default=5)
+    # 40 warmup steps: the first ~20 steps of sustained load after the GPU
+    # idles (autotuning, pool build) hold a one-time slowdown of one step
+    ap.add_argument('--warmup', type=int, default=40)
     ap.add_argument('--workload', default='c3')
     ap.add_argument('--plain-steps', type=int, default=5)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-autotune', action='store_true')
+    ap.add_argument('--eager', action='store_true',
+                    help='time eager steps (default: each step one hipGraph replay)')
     ap.add_argument('--tiles', default='')
     ap.add_argument('--churn', type=float, default=None,
                     help='fraction of brains replaced by newborns every step (c3 default 0.01)')
@@ -403,7 +407,7 @@ def main():
         per_step = max(1, int(round(churn * w.count)))
         rng = np.random.default_rng(77)
         plans = []
-        for k in range(args.warmup + args.steps):
+        for k in range(args.warmup + args.steps + 1):
             idx = np.ascontiguousarray(rng.choice(w.count, per_step, replace=False), np.int32)
             which = rng.integers(0, newborn.count, per_step)
             arr = (ct.c_void_p * per_step)(*[births.ptrs[j] for j in which])
@@ -422,7 +426,7 @@ def main():
     # null stream, on which events would not bracket the library's launches
     stream = torch.cuda.Stream()
 
-    def step(k):
+    def births_for(k):
         if births is not None:
             import ctypes as ct
             idx, arr, seeds = plans[k]
@@ -431,9 +435,19 @@ def main():
                                           stream.cuda_stream)
             if rc:
                 raise RuntimeError(ns_amd.error())
+
+    # A world step launches its ticks, then places the births and deaths the
+    # next step starts from (host work overlapping this launch, device work
+    # queued behind it), as a world loop runs them.
+    def step(k):
         batch.step_device(inputs[k % patterns].data_ptr(), w.ticks, w.gain, out.data_ptr(),
                           stream=stream.cuda_stream)
+        births_for(k + 1)
 
+    # Timed steps: each step one replay of a captured hipGraph (the output
+    # clears and the whole 16-tick launch), replacements eager between them.
+    batch.set_graph(not args.eager)
+    births_for(0)
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
@@ -469,6 +483,7 @@ def main():
 
     # plain per-tick form on the same pools (the 5x denominator)
     batch.set_timing(False)
+    batch.set_graph(False)
     batch.set_form(ns_amd.FORM_PER_TICK)
     for k in range(2):
         step(k)
@@ -481,8 +496,30 @@ def main():
     plain_value = w.total_neurons * w.ticks * args.plain_steps / plain_wall
     plain_alg = (NEURON_UPDATE_BYTES * w.total_neurons * w.ticks
                  + syn_bytes * w.total_synapses * w.ticks + EVENT_BYTES * ev / args.steps)
+
+    def leg(steps, graph):
+        # graph: each step one replay of a captured hipGraph (ns_gpu_set_graph);
+        # the replacements stay eager work between replays
+        batch.set_graph(graph)
+        for k in range(2):
+            step(k)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(steps):
+            step(k)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t
+        batch.set_graph(False)
+        return t
+
+    plain_graph_wall = leg(args.plain_steps, True)
+    plain_graph_value = w.total_neurons * w.ticks * args.plain_steps / plain_graph_wall
     batch.set_form(ns_amd.FORM_RESIDENT)
-    log(f'resident {value:.3e} vs plain per-tick {plain_value:.3e} neuron-updates/s')
+    other_wall = leg(args.steps, args.eager)        # the resident form in the other launch mode
+    other_value = w.total_neurons * w.ticks * args.steps / other_wall
+    log(f'resident {value:.3e} ({"eager" if args.eager else "graph"}; '
+        f'{"graph" if args.eager else "eager"} {other_value:.3e}) vs plain per-tick '
+        f'{plain_value:.3e} (graph replays {plain_graph_value:.3e}) neuron-updates/s')
 
     cpu = None
     if not args.no_cpu:
@@ -520,7 +557,12 @@ def main():
         'plain_per_tick': {'value': plain_value, 'speedup': value / plain_value,
                            'steps': args.plain_steps,
                            'ms_per_step': plain_wall * 1e3 / max(args.plain_steps, 1),
-                           'effective_GBps': plain_alg / (plain_wall / max(args.plain_steps, 1)) / 1e9},
+                           'effective_GBps': plain_alg / (plain_wall / max(args.plain_steps, 1)) / 1e9,
+                           'graph': {'value': plain_graph_value, 'speedup': value / plain_graph_value,
+                                     'ms_per_step': plain_graph_wall * 1e3 / max(args.plain_steps, 1)}},
+        'step_launch': 'eager' if args.eager else 'hipGraph replay',
+        'other_launch': {'mode': 'hipGraph replay' if args.eager else 'eager', 'value': other_value,
+                         'steps': args.steps, 'ms_per_step': other_wall * 1e3 / args.steps},
         'tile': {'key': list(tile_key) if tile_key else None,
                  'threads': tile[0], 'neurons_per_thread': tile[1], 'synapses_per_thread': tile[2],
                  'min_waves_per_simd': tile[3], 'lds_synapses': tile[4],
@@ -539,6 +581,7 @@ def main():
     if births is not None:
         line['churn'] = {'fraction_per_step': churn, 'brains_per_step': len(plans[0][0]),
                          'placements': i1['placements'], 'relayouts': i1['relayouts'],
+                         'staging_grows': i1['staging_grows'],
                          'slots': i1['slots'], 'free_slots': i1['free_slots'],
                          'slot_neurons': i1['slot_neurons'], 'slot_synapses': i1['slot_synapses']}
     print(json.dumps(line), flush=True)

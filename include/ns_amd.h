@@ -245,6 +245,8 @@ typedef struct ns_batch_info {
     int64_t slots, free_slots;  /* pool brain slots; slots without a brain */
     int64_t slot_neurons, slot_synapses;   /* total slot capacity */
     int64_t placements, relayouts;         /* replacements placed in free slots / by rebuild */
+    int64_t graph_captures, graph_launches;  /* ns_gpu_set_graph */
+    int64_t staging_grows;      /* replacement staging buffers enlarged (host and device) */
 } ns_batch_info;
 int ns_gpu_info(void *batch, ns_batch_info *info);
 
@@ -272,6 +274,18 @@ int ns_gpu_replace(void *batch, int count, const int *indices, void **handles,
 /* Record HIP events around every tick-kernel launch of this batch (adds one
  * event pair per launch; totals read back by ns_gpu_info). */
 int ns_gpu_set_timing(void *batch, int on);
+
+/* Graph mode (on = 1): each step (ns_gpu_step, ns_gpu_step_device) runs as
+ * one hipGraph launch on its stream: the output clears and the whole tick
+ * loop (the resident launch, or every per-tick kernel of the plain form,
+ * plus the LP/Prinz slots' per-tick kernels) are captured once and replayed
+ * while the step's arguments (ticks, input and output buffers, gain) and the
+ * pool layout stay the same; a rebuild, a new tile or form, or other
+ * arguments capture again.  Replacements and run masks stay eager work
+ * between replays.  Results are those of eager steps.  No counterpart in the
+ * reference; the per-tick form issues dozens of launches per step, which a
+ * graph replays at once. */
+int ns_gpu_set_graph(void *batch, int on);
 
 /* Diagnostic: evaluate the device expf (fn = 0) or logf (fn = 1) on every
  * float bit pattern u in [lo, hi) and write one digest per `chunk` inputs:

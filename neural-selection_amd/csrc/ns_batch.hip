@@ -280,6 +280,7 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     tile = t;
     outputs = outs;
     nb = int(specs.size());
+    ++generation;
     const TileShape *shape = find_tile(t);
     if (!shape) throw std::runtime_error("Tile shape not compiled");
     plastic = force_plastic;
@@ -627,6 +628,7 @@ void DevicePool::upload_stage(const Stager &sg, hipStream_t st) {
     stage_next ^= 1;
     DeviceStage &d = dstage[stage_cur];
     if (d.cap < bytes) {
+        ++dstage_grows;
         NS_HIP(hipEventSynchronize(d.read));   // no queued scatter reads the old buffer
         d.arena.release();
         d.cap = std::max(2 * bytes, kStageMinBytes);
@@ -1224,10 +1226,52 @@ void Batch::set_run(const uint8_t *mask) {
     brun_valid = true;
 }
 
+void Batch::StepGraph::release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+}
+
 void Batch::run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
                 hipStream_t st) {
-    run_pool(pool, form, ticks, d_in, gain, d_out, int64_t(count) * outputs, homeostasis, st,
-             timing ? &timer : nullptr);
+    const int64_t out_count = int64_t(count) * outputs;
+    if (!graph_mode) {
+        run_pool(pool, form, ticks, d_in, gain, d_out, out_count, homeostasis, st,
+                 timing ? &timer : nullptr);
+        return;
+    }
+    StepGraph &g = step_graph;
+    if (!g.exec || g.generation != pool.generation || g.form != form || g.ticks != ticks ||
+        g.homeostasis != homeostasis || g.in != d_in || g.out != d_out || g.gain != gain ||
+        g.stream != st) {
+        g.release();
+        NS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            run_pool(pool, form, ticks, d_in, gain, d_out, out_count, homeostasis, st, nullptr);
+        } catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(st, &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        NS_HIP(hipStreamEndCapture(st, &g.graph));
+        NS_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        g.generation = pool.generation;
+        g.form = form;
+        g.ticks = ticks;
+        g.homeostasis = homeostasis;
+        g.in = d_in;
+        g.out = d_out;
+        g.gain = gain;
+        g.stream = st;
+        ++graph_captures;
+    }
+    hipEvent_t ev;
+    if (timing) timer.begin(st, form == NS_FORM_PER_TICK ? 1 : 0, ev);
+    NS_HIP(hipGraphLaunch(g.exec, st));
+    if (timing) timer.end(st);
+    ++graph_launches;
 }
 
 void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
@@ -1263,6 +1307,8 @@ void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float ga
 
 Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (step_graph.stream) (void)hipStreamSynchronize(step_graph.stream);
+    step_graph.release();
     pool.arena.release();
     io.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -1451,6 +1497,22 @@ int ns_gpu_step_device(void *p, const float *input, const uint8_t *mask, int tic
     }
 }
 
+int ns_gpu_set_graph(void *p, int on) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        if (on != 0 && on != 1) throw std::runtime_error("Invalid graph mode");
+        if (!on && b.step_graph.exec) {
+            if (b.step_graph.stream) NS_HIP(hipStreamSynchronize(b.step_graph.stream));
+            b.step_graph.release();
+        }
+        b.graph_mode = on != 0;
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 int ns_gpu_reserve(void *p, float slack, float spare) {
     try {
         auto &b = *static_cast<Batch *>(p);
@@ -1493,6 +1555,7 @@ int ns_gpu_set_timing(void *p, int on) {
         b.timer.ms[0] = b.timer.ms[1] = 0;
         b.timer.count[0] = b.timer.count[1] = 0;
         b.timing = on != 0;
+        if (b.timing) b.timer.reserve(512);   // 256 timed launches without creating events
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -1702,6 +1765,9 @@ int ns_gpu_info(void *p, ns_batch_info *info) {
         info->slot_synapses = b.pool.ns;
         info->placements = b.placements;
         info->relayouts = b.relayouts;
+        info->graph_captures = b.graph_captures;
+        info->graph_launches = b.graph_launches;
+        info->staging_grows = b.pool.staging.grows + b.pool.dstage_grows;
         int64_t big_n = 0;
         for (int k = 0; k < b.pool.nbig; ++k) big_n += b.pool.bnc[b.pool.fallback_brains + k];
         info->big_neurons = big_n;

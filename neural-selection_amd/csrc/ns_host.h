@@ -121,8 +121,11 @@ struct PlaceChunks {
 };
 
 // Staging buffers start at this size and double when outgrown (a regrowth
-// waits for the buffer's last use).
-constexpr size_t kStageMinBytes = size_t(16) << 20;
+// waits for the buffer's last use).  A growth maps new memory while the
+// previous launch still runs, which has stretched that launch by several ms,
+// so the floor covers a 1% turnover of 32,768 brains (about 7-14 MB of
+// images per step) twice over.
+constexpr size_t kStageMinBytes = size_t(64) << 20;
 
 // Pinned host staging for slot placements, double buffered so one buffer can
 // be filled while the other's copy is still in flight.
@@ -130,11 +133,13 @@ struct PinnedStaging {
     struct Buf { unsigned char *p = nullptr; size_t cap = 0; hipEvent_t done = nullptr; };
     Buf bufs[2];
     int next = 0;
+    int64_t grows = 0;
     Buf &take(size_t bytes) {
         Buf &b = bufs[next];
         next ^= 1;
         if (b.done) NS_HIP(hipEventSynchronize(b.done));
         if (b.cap < bytes) {
+            ++grows;
             if (b.p) (void)hipHostFree(b.p);
             b.p = nullptr;
             b.cap = 0;
@@ -227,6 +232,7 @@ public:
     };
     DeviceStage dstage[2];
     int stage_cur = 0, stage_next = 0;
+    int64_t dstage_grows = 0;
     hipStream_t copy_stream = nullptr;   // default priority
     // Copy the staged buffer to the device (one upload on copy_stream, which
     // `st` then waits for); entries [lo, hi) are then scattered on `st` by
@@ -254,6 +260,7 @@ public:
     uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
     int32_t *d_reset = nullptr;      // 2 x nb: slots to reset, then their seeds
     hipStream_t stream = nullptr;
+    uint64_t generation = 0;         // counts builds (device arrays and packs replaced)
 };
 
 // One call of `ticks` ticks over every brain whose run flag is set: the
@@ -263,19 +270,39 @@ struct KernelTimer;
 void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
               int64_t out_count, int homeostasis, hipStream_t st, KernelTimer *timer = nullptr);
 
-// HIP-event timing of tick-kernel launches (pairs resolved lazily).
+// HIP-event timing of tick-kernel launches (pairs resolved lazily).  Events
+// come from a pool created when timing is switched on and are reused once
+// resolved, so timed launches create no events while kernels run.
 struct KernelTimer {
     struct Pair { hipEvent_t a, b; int kind; };
     std::vector<Pair> pending;
+    std::vector<hipEvent_t> spare;
     double ms[2] = {0, 0};
     int64_t count[2] = {0, 0};
+    hipEvent_t take() {
+        hipEvent_t e = nullptr;
+        if (spare.empty()) {
+            NS_HIP(hipEventCreate(&e));
+        } else {
+            e = spare.back();
+            spare.pop_back();
+        }
+        return e;
+    }
+    void reserve(size_t n) {
+        while (spare.size() < n) {
+            hipEvent_t e = nullptr;
+            NS_HIP(hipEventCreate(&e));
+            spare.push_back(e);
+        }
+    }
     void begin(hipStream_t st, int kind, hipEvent_t &a) {
-        NS_HIP(hipEventCreate(&a));
+        a = take();
         NS_HIP(hipEventRecord(a, st));
         pending.push_back({a, nullptr, kind});
     }
     void end(hipStream_t st) {
-        NS_HIP(hipEventCreate(&pending.back().b));
+        pending.back().b = take();
         NS_HIP(hipEventRecord(pending.back().b, st));
     }
     void resolve() {
@@ -285,8 +312,8 @@ struct KernelTimer {
             NS_HIP(hipEventElapsedTime(&t, p.a, p.b));
             ms[p.kind] += t;
             count[p.kind] += 1;
-            (void)hipEventDestroy(p.a);
-            (void)hipEventDestroy(p.b);
+            spare.push_back(p.a);
+            spare.push_back(p.b);
         }
         pending.clear();
     }
@@ -295,6 +322,7 @@ struct KernelTimer {
             (void)hipEventDestroy(p.a);
             if (p.b) (void)hipEventDestroy(p.b);
         }
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
     }
 };
 
@@ -315,6 +343,21 @@ struct Batch {
     std::vector<int32_t> h_status;
     bool timing = false;
     KernelTimer timer;
+    // Graph mode: the captured step and the arguments it was captured for.
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        uint64_t generation = 0;
+        int form = 0, ticks = 0, homeostasis = 0;
+        const float *in = nullptr;
+        float *out = nullptr;
+        float gain = 0.0f;
+        hipStream_t stream = nullptr;
+        void release();
+    };
+    bool graph_mode = false;
+    StepGraph step_graph;
+    int64_t graph_captures = 0, graph_launches = 0;
 
     // Slot reserve for brain replacement: capacities rounded up a geometric
     // ladder of ratio 1 + slack, plus spare empty slots (a fraction `spare`

@@ -40,7 +40,8 @@ class BatchInfo(ct.Structure):
                 ('min_waves', ct.c_int), ('lds_synapses', ct.c_int),
                 ('slots', ct.c_int64), ('free_slots', ct.c_int64), ('slot_neurons', ct.c_int64),
                 ('slot_synapses', ct.c_int64), ('placements', ct.c_int64),
-                ('relayouts', ct.c_int64)]
+                ('relayouts', ct.c_int64), ('graph_captures', ct.c_int64),
+                ('graph_launches', ct.c_int64), ('staging_grows', ct.c_int64)]
 
 
 class ServeConfig(ct.Structure):
@@ -105,6 +106,7 @@ SIGNATURES = {
     'ns_gpu_info': ([_vp, ct.POINTER(BatchInfo)], ct.c_int),
     'ns_gpu_set_timing': ([_vp, ct.c_int], ct.c_int),
     'ns_gpu_reserve': ([_vp, ct.c_float, ct.c_float], ct.c_int),
+    'ns_gpu_set_graph': ([_vp, ct.c_int], ct.c_int),
     'ns_gpu_replace': ([_vp, ct.c_int, _i32, ct.POINTER(_vp), _u32, _vp], ct.c_int),
     'ns_device_math_digest': ([ct.c_int, ct.c_uint64, ct.c_uint64, ct.c_int64,
                                ct.POINTER(ct.c_uint64)], ct.c_int),
@@ -308,6 +310,10 @@ class GpuBatch:
 
     def set_timing(self, on=True):
         self._check(self.lib.ns_gpu_set_timing(self.ptr, int(on)))
+
+    def set_graph(self, on=True):
+        """Run each step as one replayed hipGraph launch (ns_gpu_set_graph)."""
+        self._check(self.lib.ns_gpu_set_graph(self.ptr, int(on)))
 
     def reserve(self, slack=0.19, spare=0.03):
         """Slot reserve for replacements (ns_gpu_reserve); state is kept."""

@@ -8,8 +8,9 @@ file (the figures bench.py reports beside its own HIP-event timing).
         --out profiles/k_resident_lds_c3.md
 
 --trace is a `rocprofv3 --kernel-trace --stats --output-format csv` output
-directory; the average over the last --last dispatches of the kernel (the timed
-steps of bench.py, which come after its warmup and autotuning) is what must
+directory; the average over the --last dispatches of the kernel that precede
+the last --skip (the timed steps of bench.py, after its autotuning and warmup
+and before its later legs) is what must
 agree with bench.py's `launch_ms`.  The counter directories are `rocprofv3
 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per dispatch); each counter
 is scaled by bytes / counter reading of a device-to-device copy of --copy-mib
@@ -53,6 +54,8 @@ def main():
     ap.add_argument('--kernel', required=True)
     ap.add_argument('--trace', required=True)
     ap.add_argument('--last', type=int, default=50)
+    ap.add_argument('--skip', type=int, default=0,
+                    help='dispatches after the timed window (later legs of the same run)')
     ap.add_argument('--fetch')
     ap.add_argument('--write')
     ap.add_argument('--copy-fetch')
@@ -71,7 +74,7 @@ def main():
         raise SystemExit(f'kernel {a.kernel} not in {a.trace}')
     rows.sort(key=lambda r: int(r['Start_Timestamp']))
     durs = [(int(r['End_Timestamp']) - int(r['Start_Timestamp'])) / 1e6 for r in rows]
-    timed = durs[-a.last:]
+    timed = durs[len(durs) - a.skip - a.last:len(durs) - a.skip]
     name = rows[-1]['Kernel_Name']
     stats = [r for r in _csv(a.trace, 'kernel_stats.csv')]
     top = sorted(stats, key=lambda r: -float(r['Percentage']))[:8]
@@ -79,6 +82,8 @@ def main():
     lines = [f'# {a.kernel}, workload {a.workload}, tile {a.tile}', '',
              f'Kernel: `{name}`', '', '## Kernel trace', '',
              f'Command: `{a.command}`', '',
+             f'- dispatches of this kernel: {len(durs)} (all); the timed steps are the {len(timed)} '
+             f'before the last {a.skip}' if a.skip else
              f'- dispatches of this kernel: {len(durs)} (all), last {len(timed)} = the timed steps',
              f'- average duration of the timed steps: {statistics.mean(timed):.4f} ms '
              f'(min {min(timed):.4f}, max {max(timed):.4f})',
