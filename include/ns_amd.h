@@ -327,6 +327,18 @@ int ns_pool_export_records(void *pool, ns_checkpoint_neuron *neurons,
 int ns_world_neural_step(void *pool, int64_t first_tick, const float *senses,
                          int *motors, const uint8_t *frozen);
 
+/* ns_world_neural_step over device arrays, as main.cu keeps them (d.senses
+ * written by update_senses, d.motors read by body_step): senses and motors are
+ * device pointers (animal_capacity x 19 floats, animal_capacity x 5 ints,
+ * incremented on the device), frozen a host array or NULL.  Enqueued on
+ * `stream` (a hipStream_t, NULL = the pool's own) after earlier work there,
+ * with no host synchronisation except on the first step after an import or
+ * after skipped ticks (ring alignment) and when the run or freeze flags
+ * change.  Every other ns_pool_ and ns_world_ call waits for it. */
+int ns_world_neural_step_device(void *pool, int64_t first_tick,
+                                const float *senses, int *motors,
+                                const uint8_t *frozen, void *stream);
+
 /* homeostasis_step (main.cu:1287-1300): age[a] in world steps. */
 int ns_world_homeostasis(void *pool, const int *age, const uint8_t *frozen);
 

@@ -64,6 +64,7 @@ void launch_pack_order(const uint32_t *nflags, const Pack *packs, const int32_t 
                        uint16_t *perm, hipStream_t st);
 void launch_fill(float *x, int64_t n, float v, hipStream_t st);
 void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st);
+void launch_add_counts(int32_t *total, const float *counts, int64_t n, hipStream_t st);
 void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,
                      const float *input, float gain, float *out, int homeostasis,
                      hipStream_t st);

@@ -202,6 +202,13 @@ __global__ void k_fill_i32(int32_t *x, int64_t n, int32_t v) {
     if (i < n) x[i] = v;
 }
 
+// Motor spike counts of a world step added to the caller's totals
+// (main.cu:743 adds each spike as it happens; the sum is the same).
+__global__ void k_add_counts(int32_t *total, const float *counts, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) total[i] += int32_t(counts[i]);
+}
+
 // ---------------------------------------------------------------------------
 // Plain per-tick form
 // ---------------------------------------------------------------------------
@@ -1221,6 +1228,10 @@ void launch_fill(float *x, int64_t n, float v, hipStream_t st) {
 }
 void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, v);
+}
+
+void launch_add_counts(int32_t *total, const float *counts, int64_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_add_counts, dim3(blocks_for(n, 256)), dim3(256), 0, st, total, counts, n);
 }
 
 void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,

@@ -52,6 +52,7 @@ struct WorldPool {
     int32_t *d_age = nullptr;                 // one age per pool slot
     int age_cap = 0;
     hipStream_t stream = nullptr;
+    hipStream_t last_stream = nullptr;        // the caller's stream of the last device step
     int64_t next_tick = -1;                   // tick the device rings are aligned to
     bool rings_pending = true;                // records' rings not yet on the device
     int form = NS_FORM_RESIDENT;
@@ -234,16 +235,31 @@ struct WorldPool {
         next_tick = T;
     }
 
-    void set_flags(const uint8_t *frozen) {
+    // Waits for the pool's stream and the stream of the last device step.
+    void sync_all() {
+        if (last_stream) NS_HIP(hipStreamSynchronize(last_stream));
+        NS_HIP(hipStreamSynchronize(stream));
+    }
+
+    // Per-slot run and freeze flags; uploaded (after the work queued on `st`
+    // that may still read them) only when they change.
+    std::vector<uint8_t> h_run, h_frozen;
+    uint64_t flags_generation = 0;
+    void set_flags(const uint8_t *frozen, hipStream_t st) {
         std::vector<uint8_t> run(pool.nb), fz(pool.nb);
         for (int pb = 0; pb < pool.nb; ++pb) {
             const int a = pool.batch_of[pb];   // -1: a free slot, which never runs
             run[pb] = uint8_t((a >= 0 ? 1 : 0) | (pool.fallback[pb] ? 2 : 0));
             fz[pb] = frozen && a >= 0 ? frozen[a] : 0;
         }
-        NS_HIP(hipStreamSynchronize(stream));
+        if (flags_generation == pool.generation && run == h_run && fz == h_frozen) return;
+        NS_HIP(hipStreamSynchronize(st));
+        sync_all();
         NS_HIP(hipMemcpy(pool.view.brun, run.data(), run.size(), hipMemcpyHostToDevice));
         NS_HIP(hipMemcpy(pool.view.bfrozen, fz.data(), fz.size(), hipMemcpyHostToDevice));
+        h_run = std::move(run);
+        h_frozen = std::move(fz);
+        flags_generation = pool.generation;
     }
 
     void replace(int count, const int *animals, const ns_checkpoint_neuron *neurons, int neuron_count,
@@ -261,6 +277,7 @@ void WorldPool::replace(int count, const int *animals, const ns_checkpoint_neuro
     if (count == 0 && (neuron_count || synapse_count))
         throw std::runtime_error("Record owner not replaced");
     if (count == 0) return;
+    sync_all();                               // no queued step still reads the slots
     std::vector<int> listed(P, -1);
     for (int j = 0; j < count; ++j) {
         if (animals[j] < 0 || animals[j] >= P || listed[animals[j]] >= 0)
@@ -335,7 +352,7 @@ void WorldPool::replace(int count, const int *animals, const ns_checkpoint_neuro
     }
     if (rebuild_pool) {
         // survivors keep their device state through their records
-        NS_HIP(hipStreamSynchronize(stream));
+        sync_all();
         pull();
         for (int j = 0; j < count; ++j)
             set_animal(animals[j], std::move(nr[j]), std::move(sr[j]), std::move(fresh[j]));
@@ -382,7 +399,7 @@ void WorldPool::replace(int count, const int *animals, const ns_checkpoint_neuro
     pool.scatter_range(0, int(stage.entries.size()), stream);
     pool.order_touched(stream);
     NS_HIP(hipGetLastError());
-    NS_HIP(hipStreamSynchronize(stream));
+    sync_all();
     for (int j = 0; j < count; ++j) placements += new_slot[j] >= 0;
 }
 }  // namespace
@@ -452,7 +469,7 @@ int ns_pool_import_records(void *p, const ns_checkpoint_neuron *neurons, int neu
             if (nr[synapses[i].owner].empty()) throw std::runtime_error("Synapse owner has no neurons");
             sr[synapses[i].owner].push_back(synapses[i]);
         }
-        NS_HIP(hipStreamSynchronize(w.stream));
+        w.sync_all();
         for (int a = 0; a < w.P; ++a) {
             HostBrain hb = WorldPool::make_brain(nr[a], sr[a]);
             w.set_animal(a, std::move(nr[a]), std::move(sr[a]), std::move(hb));
@@ -470,7 +487,7 @@ int ns_pool_reserve(void *p, float slack, float spare) {
         auto &w = *static_cast<WorldPool *>(p);
         if (!(slack >= 0 && slack <= 4) || !(spare >= 0 && spare <= 4))
             throw std::runtime_error("Invalid slot reserve");
-        NS_HIP(hipStreamSynchronize(w.stream));
+        w.sync_all();
         w.pull();
         w.slack = slack;
         w.spare = spare;
@@ -524,7 +541,7 @@ int ns_pool_export_records(void *p, ns_checkpoint_neuron *neurons, int neuron_ca
         if (neuron_capacity < w.nrecords || synapse_capacity < w.srecords ||
             (w.nrecords && !neurons) || (w.srecords && !synapses))
             throw std::runtime_error("Record buffer too small");
-        NS_HIP(hipStreamSynchronize(w.stream));
+        w.sync_all();
         w.pull();
         // ascending slot order over every animal's records
         std::vector<std::pair<int, const ns_checkpoint_neuron *>> on;
@@ -554,8 +571,9 @@ int ns_world_neural_step(void *p, int64_t first_tick, const float *senses, int *
         auto &w = *static_cast<WorldPool *>(p);
         if (first_tick < 0) throw std::runtime_error("Invalid world tick");
         if (w.nreal == 0) return 0;
+        w.sync_all();
         w.align_rings(first_tick);
-        w.set_flags(frozen);
+        w.set_flags(frozen, w.stream);
         NS_HIP(hipMemcpy(w.d_senses, senses, sizeof(float) * w.P * kSenses, hipMemcpyHostToDevice));
         run_pool(w.pool, w.form, kTicksPerStep, w.d_senses, 230.0f, w.d_out, int64_t(w.P) * kMotors,
                  0, w.stream);
@@ -563,7 +581,7 @@ int ns_world_neural_step(void *p, int64_t first_tick, const float *senses, int *
         std::vector<float> out(size_t(w.P) * kMotors);
         NS_HIP(hipMemcpyAsync(out.data(), w.d_out, sizeof(float) * out.size(), hipMemcpyDeviceToHost,
                               w.stream));
-        NS_HIP(hipStreamSynchronize(w.stream));
+        w.sync_all();
         for (int a = 0; a < w.P; ++a)
             if (w.real[a])
                 for (int m = 0; m < kMotors; ++m) motors[a * kMotors + m] += int(out[a * kMotors + m]);
@@ -575,11 +593,39 @@ int ns_world_neural_step(void *p, int64_t first_tick, const float *senses, int *
     }
 }
 
+int ns_world_neural_step_device(void *p, int64_t first_tick, const float *senses, int *motors,
+                                const uint8_t *frozen, void *stream) {
+    try {
+        auto &w = *static_cast<WorldPool *>(p);
+        if (first_tick < 0 || !senses || !motors) throw std::runtime_error("Invalid world step");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : w.stream;
+        if (w.nreal == 0) return 0;
+        // a previous step queued on another stream finishes first
+        if (w.last_stream && w.last_stream != st) NS_HIP(hipStreamSynchronize(w.last_stream));
+        if (w.rings_pending || first_tick != w.next_tick) {
+            NS_HIP(hipStreamSynchronize(st));
+            w.sync_all();
+            w.align_rings(first_tick);
+        }
+        w.set_flags(frozen, st);
+        run_pool(w.pool, w.form, kTicksPerStep, senses, 230.0f, w.d_out, int64_t(w.P) * kMotors, 0, st);
+        launch_add_counts(motors, w.d_out, int64_t(w.P) * kMotors, st);
+        NS_HIP(hipGetLastError());
+        w.next_tick = first_tick + kTicksPerStep;
+        w.last_stream = st;
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 int ns_world_homeostasis(void *p, const int *age, const uint8_t *frozen) {
     try {
         auto &w = *static_cast<WorldPool *>(p);
         if (w.nreal == 0) return 0;
-        w.set_flags(frozen);
+        w.sync_all();
+        w.set_flags(frozen, w.stream);
         std::vector<int32_t> a(w.pool.nb);
         for (int pb = 0; pb < w.pool.nb; ++pb) a[pb] = w.pool.batch_of[pb] >= 0 ? age[w.pool.batch_of[pb]] : 0;
         if (w.age_cap < w.pool.nb) {          // slots (spares included) can outnumber animals
@@ -590,7 +636,7 @@ int ns_world_homeostasis(void *p, const int *age, const uint8_t *frozen) {
         NS_HIP(hipMemcpy(w.d_age, a.data(), sizeof(int32_t) * a.size(), hipMemcpyHostToDevice));
         launch_world_homeostasis(w.pool.view, w.pool.nn, w.d_age, w.stream);
         NS_HIP(hipGetLastError());
-        NS_HIP(hipStreamSynchronize(w.stream));
+        w.sync_all();
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());

@@ -116,6 +116,7 @@ SIGNATURES = {
     'ns_pool_export_records': ([_vp, _vp, ct.c_int, _vp, ct.c_int, _i32, _i32], ct.c_int),
     'ns_world_neural_step': ([_vp, ct.c_int64, _f, _i32, _u8], ct.c_int),
     'ns_world_homeostasis': ([_vp, _i32, _u8], ct.c_int),
+    'ns_world_neural_step_device': ([_vp, ct.c_int64, _vp, _vp, _u8, _vp], ct.c_int),
     'ns_pool_replace_records': ([_vp, ct.c_int, _i32, _vp, ct.c_int, _vp, ct.c_int], ct.c_int),
     'ns_pool_reserve': ([_vp, ct.c_float, ct.c_float], ct.c_int),
     'ns_pool_info': ([_vp, ct.POINTER(PoolStats)], ct.c_int),

@@ -457,3 +457,97 @@ def test_replace_into_empty_pool_and_after_every_death():
     finally:
         L.ns_pool_destroy(A)
         L.ns_pool_destroy(B)
+
+
+class _Hip:
+    """The HIP runtime libns_amd runs on (the same loaded library): device
+    buffers and a stream for the device-pointer entry points."""
+
+    def __init__(self):
+        ns_amd.lib()
+        self.h = ct.CDLL('libamdhip64.so.7')
+        self.h.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+        self.h.hipFree.argtypes = [ct.c_void_p]
+        self.h.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+        self.h.hipStreamCreate.argtypes = [ct.POINTER(ct.c_void_p)]
+        self.h.hipStreamDestroy.argtypes = [ct.c_void_p]
+        self.h.hipStreamSynchronize.argtypes = [ct.c_void_p]
+        for f in ('hipMalloc', 'hipFree', 'hipMemcpy', 'hipStreamCreate', 'hipStreamDestroy',
+                  'hipStreamSynchronize'):
+            getattr(self.h, f).restype = ct.c_int
+
+    def alloc(self, nbytes):
+        p = ct.c_void_p()
+        assert self.h.hipMalloc(ct.byref(p), nbytes) == 0
+        return p.value
+
+    def put(self, dst, arr):
+        assert self.h.hipMemcpy(dst, arr.ctypes.data, arr.nbytes, 1) == 0      # host to device
+
+    def get(self, src, arr):
+        assert self.h.hipMemcpy(arr.ctypes.data, src, arr.nbytes, 2) == 0      # device to host
+        return arr
+
+
+def test_device_step_equals_host_step():
+    """ns_world_neural_step_device (senses and motors in device memory,
+    stream-ordered) equals ns_world_neural_step: motors every step and
+    records at the end, across homeostasis, a replacement and a skipped
+    tick."""
+    L = ns_amd.lib()
+    hip = _Hip()
+    nr, sr, _ = build_world(seed=41, animals=8)
+    dn, ds, _ = build_world(seed=42, animals=2)
+    P, ncap, scap = 10, 4000, 16000
+    A = L.ns_pool_create(P, ncap, scap)
+    B = L.ns_pool_create(P, ncap, scap)
+    assert A and B, ns_amd.error()
+    print(f'[world-device] animals={P} neurons={len(nr)} synapses={len(sr)}', flush=True)
+    stream = ct.c_void_p()
+    assert hip.h.hipStreamCreate(ct.byref(stream)) == 0
+    d_senses = hip.alloc(P * 19 * 4)
+    d_motors = hip.alloc(P * 5 * 4)
+    frozen = np.zeros(P, np.uint8)
+    frozen[1] = 1
+    try:
+        for pl in (A, B):
+            assert L.ns_pool_reserve(pl, 0.19, 0.25) == 0
+            assert L.ns_pool_import_records(pl, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == 0
+        motors_h = np.zeros(P * 5, np.int32)
+        hip.put(d_motors, np.zeros(P * 5, np.int32))
+        tick = 700
+        for step in range(8):
+            senses = (np.random.default_rng(50 + step).random(P * 19) < 0.5).astype(np.float32)
+            assert hip.h.hipStreamSynchronize(stream) == 0
+            hip.put(d_senses, senses)
+            tick += 32 if step == 5 else 16
+            assert L.ns_world_neural_step(A, tick, senses.ctypes.data_as(O._f), motors_h.ctypes.data_as(O._i32),
+                                          frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
+            assert L.ns_world_neural_step_device(B, tick, d_senses, d_motors, frozen.ctypes.data_as(O._u8),
+                                                 stream) == 0, ns_amd.error()
+            age = (np.arange(P) + step).astype(np.int32)
+            for pl in (A, B):
+                assert L.ns_world_homeostasis(pl, age.ctypes.data_as(O._i32), frozen.ctypes.data_as(O._u8)) == 0
+            assert hip.h.hipStreamSynchronize(stream) == 0
+            got = hip.get(d_motors, np.zeros(P * 5, np.int32))
+            assert np.array_equal(motors_h, got), f'motors at step {step}'
+            if step == 3:                                            # a birth in a free slot
+                en, es = _export(L, A, ncap, scap)
+                free_n = np.setdiff1d(np.arange(ncap), en['slot'][en['owner'] != 4])
+                free_s = np.setdiff1d(np.arange(scap), es['slot'][es['owner'] != 4])
+                k = int((dn['owner'] == 0).sum())
+                m = int((ds['owner'] == 0).sum())
+                bn, bs = _newborn(dn, ds, 0, 4, free_n[:k], free_s[:m])
+                d = np.array([4], np.int32)
+                for pl in (A, B):
+                    assert L.ns_pool_replace_records(pl, 1, d.ctypes.data_as(O._i32), bn.ctypes.data, len(bn),
+                                                     bs.ctypes.data, len(bs)) == 0, ns_amd.error()
+        xa, xb = _export(L, A, ncap, scap), _export(L, B, ncap, scap)
+        assert _same_records(xa[0], xb[0]) and _same_records(xa[1], xb[1])
+        assert xa[0]['state']['total_spikes'].sum() > 0
+    finally:
+        L.ns_pool_destroy(A)
+        L.ns_pool_destroy(B)
+        hip.h.hipFree(d_senses)
+        hip.h.hipFree(d_motors)
+        hip.h.hipStreamDestroy(stream)
