@@ -810,11 +810,13 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // Resident form, synapse state in LDS.
 //
 // Same tick order as k_resident, but every per-synapse value (weight and
-// plasticity traces, genes, flags, endpoints, this tick's release) lives in
+// plasticity traces, genes, flags, endpoints, this tick's arrival) lives in
 // dynamically sized LDS and threads stride over the pack's synapses.  Thread
 // registers then hold only neuron state, which keeps the kernel under 128
 // VGPRs.  PLASTIC = false drops the trace arrays for pools without plastic
-// synapses.
+// synapses.  A tick's releases take no per-synapse room: an arrival without
+// short-term plasticity releases its weight, and the short-term releases sit
+// in a compact array of kRelCap entries named by the arrival code.
 //
 // Per tick: phase A (synapse-parallel) applies plasticity of tick t-1, detects
 // the arrivals of tick t, marks the post neuron of every arrival and queues
@@ -826,8 +828,15 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // ---------------------------------------------------------------------------
 template <bool PLASTIC>
 __host__ __device__ constexpr int lds_synapse_bytes() {
-    return PLASTIC ? 4 * 9 + 1 + 1 + 2 : 4 * 5 + 1 + 1 + 2;
+    return PLASTIC ? 4 * 8 + 2 + 1 : 4 * 4 + 2 + 1;
 }
+
+// Arrival code of a synapse in a tick (ai_l): no arrival; an arrival whose
+// release is its weight (no short-term plasticity); k + 1 for the k-th queued
+// short-term arrival (release in crel_l[k]); or a queued arrival past the
+// kRelCap entries, whose release its post neuron's thread computes.
+constexpr uint16_t kArrNone = 0, kArrDirect = 0xffff, kArrSpill = 0xfffe;
+constexpr int kRelCap = 256;
 
 template <int WG, int KN, int MINW, bool PLASTIC>
 __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *packs,
@@ -849,6 +858,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __shared__ uint16_t cnt_l[kPackMotors];
     __shared__ int qlen;
     __shared__ unsigned events_l;
+    __shared__ float crel_l[kRelCap];          // releases of the tick's queued STP arrivals
+    __shared__ uint16_t cq_l[kRelCap];         // their synapses
 
     if (int(blockIdx.x) < nbig) {
         // the pack's synapse arrays are free in a big brain's workgroup
@@ -875,8 +886,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     const int ns = pk.ns;
 
     // synapse arrays (structure of arrays, smax entries each)
-    float *rel_l = reinterpret_cast<float *>(lds_syn);
-    float *w_l = rel_l + smax;
+    float *w_l = reinterpret_cast<float *>(lds_syn);
     float *w0_l = w_l + smax;
     uint32_t *fl_l = reinterpret_cast<uint32_t *>(w0_l + smax);
     uint32_t *pp_l = fl_l + smax;
@@ -884,9 +894,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     float *y_l = PLASTIC ? x_l + smax : x_l;
     float *pa_l = PLASTIC ? y_l + smax : x_l;
     float *qa_l = PLASTIC ? pa_l + smax : x_l;
-    uint16_t *queue_l = reinterpret_cast<uint16_t *>(PLASTIC ? qa_l + smax : x_l);
-    uint8_t *sbr_l = reinterpret_cast<uint8_t *>(queue_l + smax);
-    uint8_t *arr_l = sbr_l + smax;
+    uint16_t *ai_l = reinterpret_cast<uint16_t *>(PLASTIC ? qa_l + smax : x_l);   // arrival code
+    uint8_t *sbr_l = reinterpret_cast<uint8_t *>(ai_l + smax);
 
     // Setup issues its loads in as few dependent rounds as it can, with no
     // barrier between them (the run flags are already in LDS): slot order,
@@ -947,8 +956,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             pp_l[q] = pp[u];
             w0_l[q] = w0[u];
             w_l[q] = w[u];
-            rel_l[q] = 0.0f;
-            arr_l[q] = 0;
+            ai_l[q] = kArrNone;
             if (PLASTIC) {
                 pa_l[q] = pa[u];
                 qa_l[q] = qa[u];
@@ -1029,7 +1037,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                         if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
                             PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
                             plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q],
-                                            arr_l[q] != 0, hist_l[post] & 1, depol_l[post],
+                                            ai_l[q] != kArrNone, hist_l[post] & 1, depol_l[post],
                                             mod_l[post], etab);
                             w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
                             pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
@@ -1038,8 +1046,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                     const int64_t tick = btick_l[b] + t;
                     const int d = f & SF_DELAY;
                     const bool arr = tick >= d && ((hist_l[pre] >> (d - 1)) & 1u);
-                    arr_l[q] = arr;
-                    rel_l[q] = arr ? w_l[q] : 0.0f;
+                    ai_l[q] = arr ? kArrDirect : kArrNone;   // queued STP arrivals: below
                     if (arr) hit_l[post] = 1;
                     my_events += arr;
                     arrive_stp = arr && (route & R_STP);
@@ -1051,17 +1058,25 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                 int qb = 0;
                 if (lane == 0) qb = atomicAdd(&qlen, __popcll(m));
                 qb = __shfl(qb, 0);
-                if (arrive_stp) queue_l[qb + __popcll(m & ((uint64_t(1) << lane) - 1))] = uint16_t(q);
+                if (arrive_stp) {
+                    const int pos = qb + __popcll(m & ((uint64_t(1) << lane) - 1));
+                    if (pos < kRelCap) {
+                        cq_l[pos] = uint16_t(q);
+                        ai_l[q] = uint16_t(pos + 1);
+                    } else {
+                        ai_l[q] = kArrSpill;   // released by its post neuron's thread in phase B
+                    }
+                }
             }
         }
         __syncthreads();
         // Phase A2: short-term release over the compacted arrivals.
         {
-            const int n_q = qlen;
+            const int n_q = min(qlen, kRelCap);
             for (int j = tid; j < n_q; j += WG) {
-                const int q = queue_l[j];
+                const int q = cq_l[j];
                 const int64_t tick = btick_l[sbr_l[q]] + t;
-                rel_l[q] = release_at(p, s0 + q, fl_l[q], rel_l[q], tick, etab);
+                crel_l[j] = release_at(p, s0 + q, fl_l[q], w_l[q], tick, etab);
             }
         }
         __syncthreads();
@@ -1083,7 +1098,13 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int q1 = hit_l[n] ? int(in_l[n + 1]) : 0;
             hit_l[n] = 0;
             for (int q = in_l[n]; q < q1; ++q) {
-                const float r = rel_l[q];
+                // this tick's release: +0 without an arrival, the weight
+                // without short-term plasticity, else the queued release
+                const uint32_t a = ai_l[q];
+                float r = a == kArrNone ? 0.0f : a == kArrDirect ? w_l[q] : 0.0f;
+                if (a != kArrNone && a != kArrDirect)
+                    r = a == kArrSpill ? release_at(p, s0 + q, fl_l[q], w_l[q], tick, etab)
+                                       : crel_l[a - 1];
                 const uint32_t route = fl_l[q] >> SF_ROUTE_SHIFT;
                 if ((route & R_MOD) && r > 0) {
                     const unsigned mm = (route >> R_MOD_SHIFT) & 3u;
@@ -1124,7 +1145,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[sbr_l[q]]) {
                 const int post = pp_l[q] >> 16;
                 PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
-                plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], arr_l[q] != 0,
+                plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], ai_l[q] != kArrNone,
                                 hist_l[post] & 1, depol_l[post], mod_l[post], etab);
                 w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
             }
@@ -1157,7 +1178,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
             store_plast(p, gq, (fl_l[q] >> SF_PLAST_SHIFT) & 7, r);
         }
-        p.arrived[gq] = arr_l[q];
+        p.arrived[gq] = uint8_t(ai_l[q] != kArrNone);
     }
     for (int m = tid; m < pk.nm; m += WG) {
         const int b = m / p.outputs;
@@ -1347,14 +1368,14 @@ const TileShape kTiles[] = {
     NS_TILE(1024, 1, 8, 1),
     NS_TILE(128, 2, 8, 1),
     NS_TILE(128, 1, 4, 4),
-    NS_TILE_LDS(1024, 1, 1, 3584, 5632),
-    // two neurons per thread (2,048 per pack): spills about 65 registers per
+    NS_TILE_LDS(1024, 1, 1, 4160, 7680),
+    // two neurons per thread (2,048 per pack): spills about 85 registers per
     // lane to scratch, yet keeps brains of 1,025-2,048 neurons out of the
     // one-workgroup-per-brain path (8% faster on the 4M-neuron Pareto mix)
-    NS_TILE_LDS(1024, 2, 1, 3328, 5632),
+    NS_TILE_LDS(1024, 2, 1, 3808, 7040),
     NS_TILE_LDS(1024, 1, 1, 2048, 3072),
     NS_TILE_LDS(512, 1, 1, 1792, 2944),
-    NS_TILE_LDS(512, 2, 1, 3584, 5632),
+    NS_TILE_LDS(512, 2, 1, 4160, 7680),
     NS_TILE_LDS(256, 1, 1, 896, 1408),
     NS_TILE_LDS(256, 2, 1, 1792, 2944),
 };
