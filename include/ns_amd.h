@@ -500,7 +500,8 @@ int ns_checkpoint_save(const char *path, const ns_checkpoint_header *header,
 
 /* The same with a world pool's records (ns_pool_export_records /
  * ns_pool_import_records).  Loading checks the whole file before the pool
- * changes; world may be NULL (only *world_bytes is set). */
+ * changes, then loads the pool; world may be NULL (the world block is not
+ * copied, *world_bytes still gives its size). */
 int ns_pool_save_checkpoint(void *pool, const char *path,
                             const ns_checkpoint_header *header,
                             const void *world, size_t world_bytes);
