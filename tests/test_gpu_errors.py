@@ -40,6 +40,8 @@ def test_invalid_replacements_change_nothing():
                                 seeds.ctypes.data_as(ct.POINTER(ct.c_uint32)), None) == -1
         assert 'handle' in ns_amd.error()
         assert L.ns_gpu_replace(batch.ptr, -1, None, None, None, None) == -1
+        assert L.ns_gpu_set_graph(batch.ptr, 2) == -1
+        assert 'graph mode' in ns_amd.error()
         # nothing changed: the batch still equals the untouched oracle brains
         assert_state_equal(batch.export(), oracle_export_all(brains), 'after failed calls')
         for call in range(w.calls):

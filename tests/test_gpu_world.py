@@ -492,8 +492,8 @@ class _Hip:
 def test_device_step_equals_host_step():
     """ns_world_neural_step_device (senses and motors in device memory,
     stream-ordered) equals ns_world_neural_step: motors every step and
-    records at the end, across homeostasis, a replacement and a skipped
-    tick."""
+    records at the end, across homeostasis, a replacement, a change of the
+    freeze flags and a skipped tick."""
     L = ns_amd.lib()
     hip = _Hip()
     nr, sr, _ = build_world(seed=41, animals=8)
@@ -521,6 +521,8 @@ def test_device_step_equals_host_step():
             assert hip.h.hipStreamSynchronize(stream) == 0
             hip.put(d_senses, senses)
             tick += 32 if step == 5 else 16
+            if step == 4:
+                frozen[2] = 1                                        # flags change: uploaded again
             assert L.ns_world_neural_step(A, tick, senses.ctypes.data_as(O._f), motors_h.ctypes.data_as(O._i32),
                                           frozen.ctypes.data_as(O._u8)) == 0, ns_amd.error()
             assert L.ns_world_neural_step_device(B, tick, d_senses, d_motors, frozen.ctypes.data_as(O._u8),
