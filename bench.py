@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -30,10 +31,18 @@ import ns_workloads as W  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 # Algorithmic bytes per unit of work (SoA-minimal streaming form, one tick):
-NEURON_UPDATE_BYTES = 178
-SYNAPSE_UPDATE_BYTES_PLASTIC = 66
-SYNAPSE_UPDATE_BYTES_STATIC = 12
-EVENT_BYTES = 78
+def _header_bytes():
+    """The roofline's bytes per unit of work, from include/ns_amd.h."""
+    text = (ROOT / 'include' / 'ns_amd.h').read_text()
+    return {m.group(1): int(m.group(2))
+            for m in re.finditer(r'#define NS_BYTES_PER_(\w+) (\d+)', text)}
+
+
+_BYTES = _header_bytes()
+NEURON_UPDATE_BYTES = _BYTES['NEURON_UPDATE']
+SYNAPSE_UPDATE_BYTES_PLASTIC = _BYTES['SYNAPSE_UPDATE_PLASTIC']
+SYNAPSE_UPDATE_BYTES_STATIC = _BYTES['SYNAPSE_UPDATE_STATIC']
+EVENT_BYTES = _BYTES['EVENT']
 
 
 def log(msg):

@@ -110,6 +110,20 @@ typedef struct ns_checkpoint_synapse {
     ns_synapse_state state;
 } ns_checkpoint_synapse;
 
+/* Algorithmic bytes per unit of work of the per-tick update, the roofline's
+ * numerator (structure-of-arrays, minimal, per tick, against ~8 TB/s):
+ *   neuron-update (AdEx types 0/1/4): 82 B of state read and written, 4 B
+ *     log AMPA scale, 48 B of genes, 40 B of pending inputs read and cleared,
+ *     4 B sense level;
+ *   synapse-update, plasticity on: 20 B of genes, 40 B of plasticity state
+ *     read and written, about 6 B of post-neuron gather; plasticity off: 12 B;
+ *   event (an arrival): 58 B of short-term state read and written, 20 B of
+ *     short-term genes. */
+#define NS_BYTES_PER_NEURON_UPDATE 178
+#define NS_BYTES_PER_SYNAPSE_UPDATE_PLASTIC 66
+#define NS_BYTES_PER_SYNAPSE_UPDATE_STATIC 12
+#define NS_BYTES_PER_EVENT 78
+
 /* ------------------------------------------------------------------------ */
 /* Errors                                                                    */
 /* ------------------------------------------------------------------------ */
