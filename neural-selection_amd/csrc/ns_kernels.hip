@@ -828,8 +828,13 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // ---------------------------------------------------------------------------
 template <bool PLASTIC>
 __host__ __device__ constexpr int lds_synapse_bytes() {
-    return PLASTIC ? 4 * 8 + 2 + 1 : 4 * 4 + 2 + 1;
+    return PLASTIC ? 4 * 8 + 2 : 4 * 4 + 2;
 }
+
+// A synapse's endpoints and brain within its pack share one LDS word:
+// pre in bits 0-10, post in bits 11-21, brain in bits 22-28.
+constexpr int kPpPost = 11, kPpBrain = 22;
+constexpr uint32_t kPpMask = 0x7ffu;
 
 // Arrival code of a synapse in a tick (ai_l): no arrival; an arrival whose
 // release is its weight (no short-term plasticity); k + 1 for the k-th queued
@@ -845,6 +850,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                                                          const float *input, float gain,
                                                          float *out, int homeostasis, int smax) {
     constexpr int NMAX = WG * KN;
+    static_assert(NMAX <= int(kPpMask) + 1 && kPackBrains <= 128, "pack indexes fit the endpoint word");
     extern __shared__ __align__(16) unsigned char lds_syn[];
     __shared__ uint64_t etab[32];
     __shared__ uint32_t hist_l[NMAX];
@@ -895,7 +901,6 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     float *pa_l = PLASTIC ? y_l + smax : x_l;
     float *qa_l = PLASTIC ? pa_l + smax : x_l;
     uint16_t *ai_l = reinterpret_cast<uint16_t *>(PLASTIC ? qa_l + smax : x_l);   // arrival code
-    uint8_t *sbr_l = reinterpret_cast<uint8_t *>(ai_l + smax);
 
     // Setup issues its loads in as few dependent rounds as it can, with no
     // barrier between them (the run flags are already in LDS): slot order,
@@ -939,7 +944,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int64_t gq = s0 + (q0 + u * WG < ns ? q0 + u * WG : q0);
             b[u] = p.sbrain[gq] - pk.b0;
             f[u] = p.sflags[gq];
-            pp[u] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << 16);
+            pp[u] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << kPpPost);
             w0[u] = p.sg[SG_W0 * S + gq];
             w[u] = p.ss[SS_WEIGHT * S + gq];
             if (PLASTIC) {
@@ -952,8 +957,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int q = q0 + u * WG;
             if (q >= ns) break;
             fl_l[q] = brun_l[b[u]] ? f[u] : f[u] & ~(R_VALID << SF_ROUTE_SHIFT);
-            sbr_l[q] = uint8_t(b[u]);
-            pp_l[q] = pp[u];
+            pp_l[q] = pp[u] | (uint32_t(b[u]) << kPpBrain);
             w0_l[q] = w0[u];
             w_l[q] = w[u];
             ai_l[q] = kArrNone;
@@ -1030,8 +1034,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                 const uint32_t route = f >> SF_ROUTE_SHIFT;
                 if (route & R_VALID) {
                     const uint32_t pp = pp_l[q];
-                    const int pre = pp & 0xffff, post = pp >> 16;
-                    const int b = sbr_l[q];
+                    const int pre = pp & kPpMask, post = (pp >> kPpPost) & kPpMask;
+                    const int b = pp >> kPpBrain;
                     if (PLASTIC) {
                         const int profile = (f >> SF_PLAST_SHIFT) & 7;
                         if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
@@ -1075,7 +1079,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int n_q = min(qlen, kRelCap);
             for (int j = tid; j < n_q; j += WG) {
                 const int q = cq_l[j];
-                const int64_t tick = btick_l[sbr_l[q]] + t;
+                const int64_t tick = btick_l[pp_l[q] >> kPpBrain] + t;
                 crel_l[j] = release_at(p, s0 + q, fl_l[q], w_l[q], tick, etab);
             }
         }
@@ -1142,8 +1146,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const uint32_t f = fl_l[q];
             const uint32_t route = f >> SF_ROUTE_SHIFT;
             const int profile = (f >> SF_PLAST_SHIFT) & 7;
-            if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[sbr_l[q]]) {
-                const int post = pp_l[q] >> 16;
+            const uint32_t pp = pp_l[q];
+            if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[pp >> kPpBrain]) {
+                const int post = (pp >> kPpPost) & kPpMask;
                 PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
                 plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], ai_l[q] != kArrNone,
                                 hist_l[post] & 1, depol_l[post], mod_l[post], etab);
@@ -1172,7 +1177,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         p.hist[i] = hist_l[n];
     }
     for (int q = tid; q < ns; q += WG) {
-        if (!brun_l[sbr_l[q]]) continue;
+        if (!brun_l[pp_l[q] >> kPpBrain]) continue;
         const int64_t gq = s0 + q;
         if (PLASTIC) {
             PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
@@ -1368,14 +1373,14 @@ const TileShape kTiles[] = {
     NS_TILE(1024, 1, 8, 1),
     NS_TILE(128, 2, 8, 1),
     NS_TILE(128, 1, 4, 4),
-    NS_TILE_LDS(1024, 1, 1, 4160, 7680),
+    NS_TILE_LDS(1024, 1, 1, 4288, 8096),
     // two neurons per thread (2,048 per pack): spills about 85 registers per
     // lane to scratch, yet keeps brains of 1,025-2,048 neurons out of the
     // one-workgroup-per-brain path (8% faster on the 4M-neuron Pareto mix)
-    NS_TILE_LDS(1024, 2, 1, 3808, 7040),
+    NS_TILE_LDS(1024, 2, 1, 3936, 7424),
     NS_TILE_LDS(1024, 1, 1, 2048, 3072),
     NS_TILE_LDS(512, 1, 1, 1792, 2944),
-    NS_TILE_LDS(512, 2, 1, 4160, 7680),
+    NS_TILE_LDS(512, 2, 1, 4288, 8096),
     NS_TILE_LDS(256, 1, 1, 896, 1408),
     NS_TILE_LDS(256, 2, 1, 1792, 2944),
 };

@@ -53,8 +53,8 @@ def test_masks_and_resets_match():
 
 
 @pytest.mark.parametrize('tiles', [(256, 1, 4), (512, 2, 4), (1024, 1, 8), (128, 2, 8),
-                                   (256, 1, 4, 3), (1024, 1, 0, 1, 4160), (256, 1, 0, 1, 896),
-                                   (512, 2, 0, 1, 4160), (1024, 2, 0, 1, 3808)])
+                                   (256, 1, 4, 3), (1024, 1, 0, 1, 4288), (256, 1, 0, 1, 896),
+                                   (512, 2, 0, 1, 4288), (1024, 2, 0, 1, 3936)])
 def test_tile_shapes_agree(tiles):
     w = W.small_mix(seed=31, brains=10, calls=12, types=(0, 1, 4, 5, 6), lo=20, hi=120)
     _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=tiles)
@@ -66,11 +66,11 @@ def test_two_neurons_per_thread_packs():
     w = W.small_mix(seed=37, brains=24, calls=12, types=(0, 1, 4, 5, 6), lo=60, hi=120,
                     fanout=1.5)
     assert w.total_neurons > 1400
-    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=(1024, 2, 0, 1, 3808))
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=(1024, 2, 0, 1, 3936))
     assert info['resident_packs'] <= 2 and info['big_brains'] == 0
 
 
-@pytest.mark.parametrize('tiles', [(1024, 2, 0, 1, 3808), (1024, 1, 0, 1, 4160)])
+@pytest.mark.parametrize('tiles', [(1024, 2, 0, 1, 3936), (1024, 1, 0, 1, 4288)])
 def test_release_bursts_past_the_queue(tiles):
     """Every neuron driven, fan-out 48, 1 ms delays, strong weights: about
     2,000 short-term arrivals per brain per tick, so most of a pack's queued
