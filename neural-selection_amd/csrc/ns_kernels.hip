@@ -813,8 +813,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // plasticity traces, genes, flags, endpoints, this tick's arrival) lives in
 // dynamically sized LDS and threads stride over the pack's synapses.  Thread
 // registers then hold only neuron state, which keeps the kernel under 128
-// VGPRs.  PLASTIC = false drops the trace arrays for pools without plastic
-// synapses.  A tick's releases take no per-synapse room: an arrival without
+// VGPRs.  PLASTIC = false drops the trace arrays and the initial weight (the
+// plasticity bound) for pools without plastic synapses.  A tick's releases take no per-synapse room: an arrival without
 // short-term plasticity releases its weight, and the short-term releases sit
 // in a compact array of kRelCap entries named by the arrival code.
 //
@@ -828,7 +828,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // ---------------------------------------------------------------------------
 template <bool PLASTIC>
 __host__ __device__ constexpr int lds_synapse_bytes() {
-    return PLASTIC ? 4 * 8 + 2 : 4 * 4 + 2;
+    return PLASTIC ? 4 * 8 + 2 : 4 * 3 + 2;
 }
 
 // A synapse's endpoints and brain within its pack share one LDS word:
@@ -893,8 +893,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
 
     // synapse arrays (structure of arrays, smax entries each)
     float *w_l = reinterpret_cast<float *>(lds_syn);
-    float *w0_l = w_l + smax;
-    uint32_t *fl_l = reinterpret_cast<uint32_t *>(w0_l + smax);
+    float *w0_l = PLASTIC ? w_l + smax : w_l;   // initial weight: plasticity's bound only
+    uint32_t *fl_l = reinterpret_cast<uint32_t *>(PLASTIC ? w0_l + smax : w_l + smax);
     uint32_t *pp_l = fl_l + smax;
     float *x_l = reinterpret_cast<float *>(pp_l + smax);
     float *y_l = PLASTIC ? x_l + smax : x_l;
@@ -945,7 +945,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             b[u] = p.sbrain[gq] - pk.b0;
             f[u] = p.sflags[gq];
             pp[u] = uint32_t(p.spre[gq] - n0) | (uint32_t(p.spost[gq] - n0) << kPpPost);
-            w0[u] = p.sg[SG_W0 * S + gq];
+            if (PLASTIC) w0[u] = p.sg[SG_W0 * S + gq];
             w[u] = p.ss[SS_WEIGHT * S + gq];
             if (PLASTIC) {
                 pa[u] = p.ss[SS_PRE_AGE * S + gq];
@@ -958,7 +958,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             if (q >= ns) break;
             fl_l[q] = brun_l[b[u]] ? f[u] : f[u] & ~(R_VALID << SF_ROUTE_SHIFT);
             pp_l[q] = pp[u] | (uint32_t(b[u]) << kPpBrain);
-            w0_l[q] = w0[u];
+            if (PLASTIC) w0_l[q] = w0[u];
             w_l[q] = w[u];
             ai_l[q] = kArrNone;
             if (PLASTIC) {
@@ -1373,14 +1373,14 @@ const TileShape kTiles[] = {
     NS_TILE(1024, 1, 8, 1),
     NS_TILE(128, 2, 8, 1),
     NS_TILE(128, 1, 4, 4),
-    NS_TILE_LDS(1024, 1, 1, 4288, 8096),
+    NS_TILE_LDS(1024, 1, 1, 4288, 10432),
     // two neurons per thread (2,048 per pack): spills about 85 registers per
     // lane to scratch, yet keeps brains of 1,025-2,048 neurons out of the
     // one-workgroup-per-brain path (8% faster on the 4M-neuron Pareto mix)
-    NS_TILE_LDS(1024, 2, 1, 3936, 7424),
+    NS_TILE_LDS(1024, 2, 1, 3936, 9536),
     NS_TILE_LDS(1024, 1, 1, 2048, 3072),
     NS_TILE_LDS(512, 1, 1, 1792, 2944),
-    NS_TILE_LDS(512, 2, 1, 4288, 8096),
+    NS_TILE_LDS(512, 2, 1, 4288, 10432),
     NS_TILE_LDS(256, 1, 1, 896, 1408),
     NS_TILE_LDS(256, 2, 1, 1792, 2944),
 };

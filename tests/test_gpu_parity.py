@@ -98,3 +98,17 @@ def test_plasticity_free_pools_use_the_static_lds_kernel():
     w = W.small_mix(seed=35, brains=10, calls=10, types=(0, 1, 4, 5), plasticity=False)
     info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=10, check_every=5, tiles=(256, 1, 0, 1, 896))
     assert info['lds_synapses'] == 1408
+
+
+@pytest.mark.parametrize('tiles,capacity', [((1024, 2, 0, 1, 3936), 9536), ((1024, 1, 0, 1, 4288), 10432)])
+def test_static_packs_fill_the_lds(tiles, capacity):
+    """Without plastic synapses a pack's LDS holds no traces and no initial
+    weights, so the full-LDS tiles take 9,536 / 10,432 synapses per pack:
+    here two packs of about 8,700 synapses each, past what 18 bytes per
+    synapse would allow (8,096)."""
+    w = W.small_mix(seed=36, brains=14, calls=8, types=(0, 1, 4, 5), plasticity=False, lo=60, hi=110,
+                    fanout=14.0)
+    assert w.total_synapses > 2 * 8096
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=8, check_every=4, tiles=tiles)
+    assert info['lds_synapses'] == capacity
+    assert info['resident_packs'] == 2 and info['big_brains'] == 0
