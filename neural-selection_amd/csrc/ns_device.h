@@ -122,7 +122,6 @@ struct NeuronConst {
     float adp_decay;        // expf(-h / afterdepol_tau_ms) when tau > 0
     float rbase, rgain, revent;
     float rdecay;           // expf(-1 / receptor_decay_ms)
-    double rcap, rslope, rtauw;   // correctly rounded 1/c of the three divisors
     uint32_t flags;
 };
 
@@ -186,9 +185,6 @@ __device__ inline void load_neuron(const Pool &p, int64_t i, NeuronRegs &s, Neur
     g.reset = p.ng[NG_RESET * N + i];
     g.adp_spike = p.ng[NG_ADP_SPIKE * N + i];
     g.leak_slope = g.leak * g.slope;
-    g.rcap = 1.0 / double(g.cap);
-    g.rslope = 1.0 / double(g.slope);
-    g.rtauw = 1.0 / double(g.tauw);
     float tau = p.ng[NG_ADP_TAU * N + i];
     g.adp_decay = tau > 0 ? ns_expf(-0.25f / tau, etab) : 1.0f;
     g.flags = p.nflags[i];
@@ -261,8 +257,8 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
     for (int step = 0; step < 4; ++step) {
         float v = s.v;
         if (!(g.a == 0.0f && __float_as_uint(s.w) == 0u && __builtin_isfinite(v - g.rest)))
-            s.w += div_const(h * (g.a * (v - g.rest) - s.w), g.tauw, g.rtauw);
-        float x = clamp_range(div_const(v - g.thr, g.slope, g.rslope), -20.0f, 10.0f);
+            s.w += h * (g.a * (v - g.rest) - s.w) / g.tauw;
+        float x = clamp_range((v - g.thr) / g.slope, -20.0f, 10.0f);
         float soma = 0;
         if (cortical) {
             if (!(s.d0 >= 0.0f && s.d0 < kBranchQuiet)) soma += branch_io(s.d0, etab);
@@ -275,7 +271,7 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
         float dv = g.leak * (g.rest - v) + g.leak_slope * ns_expf(x, etab) +
                    soma * (0.0f - v) + s.inh * (-75.0f - v) + current + s.adp - s.w -
                    0.0f * (v - -68.0f);
-        s.v = v + div_const(h * dv, g.cap, g.rcap);
+        s.v = v + h * dv / g.cap;
         if (s.v >= 0.0f) {
             spike = 1;
             ++s.total;
