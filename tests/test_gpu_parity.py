@@ -70,6 +70,17 @@ def test_two_neurons_per_thread_packs():
     assert info['resident_packs'] <= 2 and info['big_brains'] == 0
 
 
+def test_full_packs_reach_every_index_bit():
+    """256 brains of 16 neurons on the two-neuron tile: two packs of 128
+    brains and 2,048 neurons each, so a pack's brain index reaches 127 and its
+    neuron index 2,047, the tops of their bit fields in the synapse's LDS
+    endpoint word."""
+    w = W.small_mix(seed=41, brains=256, calls=6, types=(0, 1, 4, 5, 6), lo=16, hi=16, fanout=1.5)
+    assert w.total_neurons == 2 * 2048
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=6, check_every=3, tiles=(1024, 2, 0, 1, 3936))
+    assert info['resident_packs'] == 2 and info['big_brains'] == 0
+
+
 @pytest.mark.parametrize('tiles', [(1024, 2, 0, 1, 3936), (1024, 1, 0, 1, 4288)])
 def test_release_bursts_past_the_queue(tiles):
     """Every neuron driven, fan-out 48, 1 ms delays, strong weights: about
