@@ -65,6 +65,7 @@ constexpr uint32_t R_BRANCH = 3u, R_INH = 1u << 2, R_MOD_SHIFT = 3, R_MOD = 3u <
 constexpr uint32_t R_VALID = 1u << 5, R_STP = 1u << 6, R_PAIR = 1u << 7;
 
 constexpr unsigned kModNE = 1, kMod5HT = 2, kModDA = 4;
+constexpr float kDepolGateMv = 2.3f;   // profile-2 LTP gate on the post's recent depolarization
 
 // World-rule ring length (src/main.cu:45-46 HISTORY = 9).
 constexpr int kWorldHistory = 9;
@@ -654,7 +655,7 @@ __device__ inline void plasticity_step(PlastRegs &r, int profile, unsigned modre
         r.pre_age = 0;
     }
     if (post_spike) {
-        if (r.pre_age <= 20.0f && (profile == 1 || depol >= 2.3f))
+        if (r.pre_age <= 20.0f && (profile == 1 || depol >= kDepolGateMv))
             r.weight += 0.016f * r.x * (2.0f * w0 - r.weight);
         r.y += 1.0f;
         r.post_age = 0;

@@ -843,6 +843,12 @@ constexpr uint32_t kPpMask = 0x7ffu;
 constexpr uint16_t kArrNone = 0, kArrDirect = 0xffff, kArrSpill = 0xfffe;
 constexpr int kRelCap = 256;
 
+// A neuron's LDS modulator byte (mod_l) holds the tick's modulator bits and,
+// in bit 7, whether its recent depolarization passes the profile-2 gate:
+// plasticity reads the depolarization only through that comparison.
+constexpr unsigned kDepolGate = 0x80u;
+__device__ inline float gate_depol(unsigned m) { return (m & kDepolGate) ? kDepolGateMv : 0.0f; }
+
 template <int WG, int KN, int MINW, bool PLASTIC>
 __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *packs,
                                                          const int32_t *big, int nbig,
@@ -854,8 +860,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     extern __shared__ __align__(16) unsigned char lds_syn[];
     __shared__ uint64_t etab[32];
     __shared__ uint32_t hist_l[NMAX];
-    __shared__ float depol_l[NMAX];
-    __shared__ uint8_t mod_l[NMAX];
+    __shared__ uint8_t mod_l[NMAX];          // modulator bits | kDepolGate
     __shared__ uint8_t hit_l[NMAX];
     __shared__ uint16_t in_l[NMAX + 1];
     __shared__ int64_t btick_l[kPackBrains];
@@ -918,7 +923,6 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         in_l[n] = uint16_t(p.nin[n0 + n] - s0);
         mod_l[n] = 0;
         hit_l[n] = 0;
-        depol_l[n] = 0;
     }
     if (tid == 0) {
         in_l[pk.nn] = uint16_t(ns);
@@ -1041,8 +1045,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                         if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
                             PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
                             plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q],
-                                            ai_l[q] != kArrNone, hist_l[post] & 1, depol_l[post],
-                                            mod_l[post], etab);
+                                            ai_l[q] != kArrNone, hist_l[post] & 1,
+                                            gate_depol(mod_l[post]), mod_l[post] & ~kDepolGate, etab);
                             w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
                             pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
                         }
@@ -1132,8 +1136,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int spike = neuron_tick(nr[k], nc[k], current, e0, e1, e2, e3, inh, p.cell, N,
                                           n0 + n, etab);
             hist_l[n] = (hist_l[n] << 1) | uint32_t(spike);
-            depol_l[n] = nr[k].depol;
-            mod_l[n] = uint8_t(mod);
+            mod_l[n] = uint8_t(mod | (nr[k].depol >= kDepolGateMv ? kDepolGate : 0u));
             if (!isfinite(nr[k].v)) p.bstatus[pk.b0 + nbr[k]] = -1;
         }
         __syncthreads();
@@ -1151,7 +1154,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                 const int post = (pp >> kPpPost) & kPpMask;
                 PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
                 plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], ai_l[q] != kArrNone,
-                                hist_l[post] & 1, depol_l[post], mod_l[post], etab);
+                                hist_l[post] & 1, gate_depol(mod_l[post]), mod_l[post] & ~kDepolGate,
+                                etab);
                 w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
             }
         }
@@ -1170,7 +1174,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         if (nc[k].flags & NF_ACTIVE) {
             store_neuron(p, i, nr[k]);
             p.spike[i] = uint8_t(hist_l[n] & 1u);
-            p.modbits[i] = mod_l[n];
+            p.modbits[i] = mod_l[n] & ~kDepolGate;
         } else if (!(nc[k].flags & NF_WORLD)) {
             p.spike[i] = 0;
         }
@@ -1373,14 +1377,18 @@ const TileShape kTiles[] = {
     NS_TILE(1024, 1, 8, 1),
     NS_TILE(128, 2, 8, 1),
     NS_TILE(128, 1, 4, 4),
-    NS_TILE_LDS(1024, 1, 1, 4288, 10432),
+    NS_TILE_LDS(1024, 1, 1, 4416, 10720),
     // two neurons per thread (2,048 per pack): spills about 85 registers per
     // lane to scratch, yet keeps brains of 1,025-2,048 neurons out of the
     // one-workgroup-per-brain path (8% faster on the 4M-neuron Pareto mix)
+    NS_TILE_LDS(1024, 2, 1, 4160, 10144),
+    // the same tile with about 6% lighter packs: a launch takes whole rounds
+    // of one pack per CU, and where a pool's packs fill a round only in part
+    // (2,048 brains at 20 synapses per neuron) lighter packs finish sooner
     NS_TILE_LDS(1024, 2, 1, 3936, 9536),
     NS_TILE_LDS(1024, 1, 1, 2048, 3072),
     NS_TILE_LDS(512, 1, 1, 1792, 2944),
-    NS_TILE_LDS(512, 2, 1, 4288, 10432),
+    NS_TILE_LDS(512, 2, 1, 4416, 10720),
     NS_TILE_LDS(256, 1, 1, 896, 1408),
     NS_TILE_LDS(256, 2, 1, 1792, 2944),
 };
