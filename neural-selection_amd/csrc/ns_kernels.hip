@@ -843,10 +843,13 @@ constexpr uint32_t kPpMask = 0x7ffu;
 constexpr uint16_t kArrNone = 0, kArrDirect = 0xffff, kArrSpill = 0xfffe;
 constexpr int kRelCap = 256;
 
-// A neuron's LDS modulator byte (mod_l) holds the tick's modulator bits and,
-// in bit 7, whether its recent depolarization passes the profile-2 gate:
-// plasticity reads the depolarization only through that comparison.
-constexpr unsigned kDepolGate = 0x80u;
+// A neuron's LDS modulator byte (mod_l) holds the tick's modulator bits
+// (kModBits); in bit 6, whether a synapse onto it arrived this tick (set in
+// phase A, where every writer stores the same byte, and read in phase B,
+// whose store of the byte clears it); and in bit 7, whether its recent
+// depolarization passes the profile-2 gate: plasticity reads the
+// depolarization only through that comparison.
+constexpr unsigned kModBits = kModNE | kMod5HT | kModDA, kArrivalMark = 0x40u, kDepolGate = 0x80u;
 __device__ inline float gate_depol(unsigned m) { return (m & kDepolGate) ? kDepolGateMv : 0.0f; }
 
 template <int WG, int KN, int MINW, bool PLASTIC>
@@ -860,8 +863,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     extern __shared__ __align__(16) unsigned char lds_syn[];
     __shared__ uint64_t etab[32];
     __shared__ uint32_t hist_l[NMAX];
-    __shared__ uint8_t mod_l[NMAX];          // modulator bits | kDepolGate
-    __shared__ uint8_t hit_l[NMAX];
+    __shared__ uint8_t mod_l[NMAX];          // modulator bits | kArrivalMark | kDepolGate
     __shared__ uint16_t in_l[NMAX + 1];
     __shared__ int64_t btick_l[kPackBrains];
     __shared__ uint8_t brun_l[kPackBrains], bfrozen_l[kPackBrains];
@@ -922,7 +924,6 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         hist_l[n] = p.hist[n0 + n];
         in_l[n] = uint16_t(p.nin[n0 + n] - s0);
         mod_l[n] = 0;
-        hit_l[n] = 0;
     }
     if (tid == 0) {
         in_l[pk.nn] = uint16_t(ns);
@@ -1046,7 +1047,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                             PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
                             plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q],
                                             ai_l[q] != kArrNone, hist_l[post] & 1,
-                                            gate_depol(mod_l[post]), mod_l[post] & ~kDepolGate, etab);
+                                            gate_depol(mod_l[post]), mod_l[post] & kModBits, etab);
                             w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
                             pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
                         }
@@ -1055,7 +1056,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                     const int d = f & SF_DELAY;
                     const bool arr = tick >= d && ((hist_l[pre] >> (d - 1)) & 1u);
                     ai_l[q] = arr ? kArrDirect : kArrNone;   // queued STP arrivals: below
-                    if (arr) hit_l[post] = 1;
+                    if (arr) mod_l[post] |= kArrivalMark;
                     my_events += arr;
                     arrive_stp = arr && (route & R_STP);
                 }
@@ -1103,8 +1104,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const int64_t tick = btick_l[nbr[k]] + t;
             float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
             unsigned mod = 0;
-            const int q1 = hit_l[n] ? int(in_l[n + 1]) : 0;
-            hit_l[n] = 0;
+            const int q1 = (mod_l[n] & kArrivalMark) ? int(in_l[n + 1]) : 0;
             for (int q = in_l[n]; q < q1; ++q) {
                 // this tick's release: +0 without an arrival, the weight
                 // without short-term plasticity, else the queued release
@@ -1154,7 +1154,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                 const int post = (pp >> kPpPost) & kPpMask;
                 PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
                 plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], ai_l[q] != kArrNone,
-                                hist_l[post] & 1, gate_depol(mod_l[post]), mod_l[post] & ~kDepolGate,
+                                hist_l[post] & 1, gate_depol(mod_l[post]), mod_l[post] & kModBits,
                                 etab);
                 w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
             }
@@ -1174,7 +1174,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         if (nc[k].flags & NF_ACTIVE) {
             store_neuron(p, i, nr[k]);
             p.spike[i] = uint8_t(hist_l[n] & 1u);
-            p.modbits[i] = mod_l[n] & ~kDepolGate;
+            p.modbits[i] = mod_l[n] & kModBits;
         } else if (!(nc[k].flags & NF_WORLD)) {
             p.spike[i] = 0;
         }
@@ -1377,18 +1377,18 @@ const TileShape kTiles[] = {
     NS_TILE(1024, 1, 8, 1),
     NS_TILE(128, 2, 8, 1),
     NS_TILE(128, 1, 4, 4),
-    NS_TILE_LDS(1024, 1, 1, 4416, 10720),
+    NS_TILE_LDS(1024, 1, 1, 4448, 10784),
     // two neurons per thread (2,048 per pack): spills about 85 registers per
     // lane to scratch, yet keeps brains of 1,025-2,048 neurons out of the
     // one-workgroup-per-brain path (8% faster on the 4M-neuron Pareto mix)
-    NS_TILE_LDS(1024, 2, 1, 4160, 10144),
+    NS_TILE_LDS(1024, 2, 1, 4224, 10272),
     // the same tile with about 6% lighter packs: a launch takes whole rounds
     // of one pack per CU, and where a pool's packs fill a round only in part
     // (2,048 brains at 20 synapses per neuron) lighter packs finish sooner
     NS_TILE_LDS(1024, 2, 1, 3936, 9536),
     NS_TILE_LDS(1024, 1, 1, 2048, 3072),
     NS_TILE_LDS(512, 1, 1, 1792, 2944),
-    NS_TILE_LDS(512, 2, 1, 4416, 10720),
+    NS_TILE_LDS(512, 2, 1, 4448, 10784),
     NS_TILE_LDS(256, 1, 1, 896, 1408),
     NS_TILE_LDS(256, 2, 1, 1792, 2944),
 };

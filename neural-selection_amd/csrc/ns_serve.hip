@@ -329,7 +329,7 @@ struct Server {
         max_n = cls.back();
         max_s = int(std::ceil(cls.back() * syn_ratio));
         pool.force_plastic = true;
-        pool.build(specs, outputs, Tile{1024, 1, 0, 1, 4416}, stream);
+        pool.build(specs, outputs, Tile{1024, 1, 0, 1, 4448}, stream);
         pool.nbatch = pool.nb;                 // rows are slots
         pool.pool_of.assign(pool.nb, -1);
         const int rows = std::max(pool.nb, 1);

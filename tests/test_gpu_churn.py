@@ -72,7 +72,7 @@ def test_replacements_match_fresh_brains(form):
     assert info['placements'] >= 10 and info['relayouts'] >= 1
 
 
-@pytest.mark.parametrize('tiles', [(1024, 1, 0, 1, 4416), (1024, 2, 0, 1, 4160), (256, 2, 4)])
+@pytest.mark.parametrize('tiles', [(1024, 1, 0, 1, 4448), (1024, 2, 0, 1, 4224), (256, 2, 4)])
 def test_replacements_on_lds_and_register_tiles(tiles):
     info = _run(ns_amd.FORM_RESIDENT, tiles, (0, 1, 4, 5, 6), ROUNDS)
     assert info['placements'] >= 10

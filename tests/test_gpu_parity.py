@@ -53,8 +53,8 @@ def test_masks_and_resets_match():
 
 
 @pytest.mark.parametrize('tiles', [(256, 1, 4), (512, 2, 4), (1024, 1, 8), (128, 2, 8),
-                                   (256, 1, 4, 3), (1024, 1, 0, 1, 4416), (256, 1, 0, 1, 896),
-                                   (512, 2, 0, 1, 4416), (1024, 2, 0, 1, 4160)])
+                                   (256, 1, 4, 3), (1024, 1, 0, 1, 4448), (256, 1, 0, 1, 896),
+                                   (512, 2, 0, 1, 4448), (1024, 2, 0, 1, 4224)])
 def test_tile_shapes_agree(tiles):
     w = W.small_mix(seed=31, brains=10, calls=12, types=(0, 1, 4, 5, 6), lo=20, hi=120)
     _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=tiles)
@@ -66,7 +66,7 @@ def test_two_neurons_per_thread_packs():
     w = W.small_mix(seed=37, brains=24, calls=12, types=(0, 1, 4, 5, 6), lo=60, hi=120,
                     fanout=1.5)
     assert w.total_neurons > 1400
-    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=(1024, 2, 0, 1, 4160))
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=12, check_every=4, tiles=(1024, 2, 0, 1, 4224))
     assert info['resident_packs'] <= 2 and info['big_brains'] == 0
 
 
@@ -77,11 +77,11 @@ def test_full_packs_reach_every_index_bit():
     endpoint word."""
     w = W.small_mix(seed=41, brains=256, calls=6, types=(0, 1, 4, 5, 6), lo=16, hi=16, fanout=1.5)
     assert w.total_neurons == 2 * 2048
-    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=6, check_every=3, tiles=(1024, 2, 0, 1, 4160))
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=6, check_every=3, tiles=(1024, 2, 0, 1, 4224))
     assert info['resident_packs'] == 2 and info['big_brains'] == 0
 
 
-@pytest.mark.parametrize('tiles', [(1024, 2, 0, 1, 4160), (1024, 1, 0, 1, 4416)])
+@pytest.mark.parametrize('tiles', [(1024, 2, 0, 1, 4224), (1024, 1, 0, 1, 4448)])
 def test_release_bursts_past_the_queue(tiles):
     """Every neuron driven, fan-out 48, 1 ms delays, strong weights: about
     2,000 short-term arrivals per brain per tick, so most of a pack's queued
@@ -111,10 +111,10 @@ def test_plasticity_free_pools_use_the_static_lds_kernel():
     assert info['lds_synapses'] == 1408
 
 
-@pytest.mark.parametrize('tiles,capacity', [((1024, 2, 0, 1, 4160), 10144), ((1024, 1, 0, 1, 4416), 10720)])
+@pytest.mark.parametrize('tiles,capacity', [((1024, 2, 0, 1, 4224), 10272), ((1024, 1, 0, 1, 4448), 10784)])
 def test_static_packs_fill_the_lds(tiles, capacity):
     """Without plastic synapses a pack's LDS holds no traces and no initial
-    weights, so the full-LDS tiles take 10,144 / 10,720 synapses per pack:
+    weights, so the full-LDS tiles take 10,272 / 10,784 synapses per pack:
     here two packs of about 8,700 synapses each, past what 18 bytes per
     synapse would allow (8,096)."""
     w = W.small_mix(seed=36, brains=14, calls=8, types=(0, 1, 4, 5), plasticity=False, lo=60, hi=110,
