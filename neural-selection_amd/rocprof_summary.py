@@ -86,7 +86,7 @@ def main():
              f'before the last {a.skip}' if a.skip else
              f'- dispatches of this kernel: {len(durs)} (all), last {len(timed)} = the timed steps',
              f'- average duration of the timed steps: {statistics.mean(timed):.4f} ms '
-             f'(min {min(timed):.4f}, max {max(timed):.4f})',
+             f'(median {statistics.median(timed):.4f}, min {min(timed):.4f}, max {max(timed):.4f})',
              '', '`--stats` top kernels (all dispatches of the run, autotuning included):', '',
              '| kernel | calls | average ms | % |', '|---|---|---|---|']
     for r in top:
