@@ -814,9 +814,10 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // dynamically sized LDS and threads stride over the pack's synapses.  Thread
 // registers then hold only neuron state, which keeps the kernel under 128
 // VGPRs.  PLASTIC = false drops the trace arrays and the initial weight (the
-// plasticity bound) for pools without plastic synapses.  A tick's releases take no per-synapse room: an arrival without
-// short-term plasticity releases its weight, and the short-term releases sit
-// in a compact array of kRelCap entries named by the arrival code.
+// plasticity bound) for pools without plastic synapses.  A tick's releases
+// take no per-synapse room: an arrival without short-term plasticity
+// releases its weight, and the short-term releases sit in a compact array of
+// kRelCap entries named by the arrival code.
 //
 // Per tick: phase A (synapse-parallel) applies plasticity of tick t-1, detects
 // the arrivals of tick t, marks the post neuron of every arrival and queues
