@@ -357,6 +357,10 @@ def main():
     ap.add_argument('--tiles', default='')
     ap.add_argument('--churn', type=float, default=None,
                     help='fraction of brains replaced by newborns every step (c3 default 0.01)')
+    ap.add_argument('--slack', type=float, default=0.05,
+                    help='churn: slot capacity rungs 1 + slack apart (ns_gpu_reserve)')
+    ap.add_argument('--spare', type=float, default=0.03,
+                    help='churn: empty spare slots per brain (ns_gpu_reserve)')
     ap.add_argument('--rates', default='100,1000,5000', help='c4: Poisson arrival rates (1/s)')
     ap.add_argument('--episodes', type=int, default=300, help='c4: episodes per rate')
     ap.add_argument('--episode-steps', type=int, default=500, help='c4: env steps per episode')
@@ -409,7 +413,7 @@ def main():
     if churn > 0:
         import ctypes as ct
         t1 = time.perf_counter()
-        batch.reserve(0.10, 0.03)
+        batch.reserve(args.slack, args.spare)
         newborn = W.c3_newborns(count=2048)
         births = ns_amd.GenomeHandles([newborn.brain(i) for i in range(newborn.count)], w.channels,
                                       max_delay=w.max_delay)
@@ -589,6 +593,7 @@ def main():
     }
     if births is not None:
         line['churn'] = {'fraction_per_step': churn, 'brains_per_step': len(plans[0][0]),
+                         'slack': args.slack, 'spare': args.spare,
                          'placements': i1['placements'], 'relayouts': i1['relayouts'],
                          'staging_grows': i1['staging_grows'],
                          'slots': i1['slots'], 'free_slots': i1['free_slots'],
