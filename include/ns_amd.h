@@ -228,7 +228,11 @@ int ns_gpu_step_device(void *batch, const float *input, const uint8_t *mask,
 
 /* Copy state out / in, AoS in genome order.  history9 is count_neurons x 9
  * bytes laid out like gym_adapter/core.cuh's history[i][9] ring
- * (slot = tick % 9).  ticks: one int64 per brain. Any pointer may be NULL. */
+ * (slot = tick % 9).  ticks: one int64 per brain. Any pointer may be NULL.
+ * Nine slots hold the whole delay history only for delays up to 8 ticks:
+ * with a non-NULL history9, a batch made with a delay bound above 8
+ * (ns_create_ex / ns_gpu_create_bulk max_delay > 8) is refused; its rings
+ * travel as spike bit registers (ns_gpu_export/import_spike_bits). */
 int ns_gpu_export(void *batch, ns_neuron_state *neurons,
                   ns_synapse_state *synapses, uint8_t *history9,
                   int64_t *ticks);
@@ -237,8 +241,11 @@ int ns_gpu_import(void *batch, const ns_neuron_state *neurons,
                   const int64_t *ticks);
 
 /* Raw 32-bit spike shift registers (bit k = spike k ticks before the last
- * completed tick), one per neuron in genome order. */
+ * completed tick), one per neuron in genome order.  Import writes them back,
+ * so the whole ring of a batch with delays up to NS_MAX_DELAY round-trips.
+ * No counterpart in the reference (its rings have 9 slots). */
 int ns_gpu_export_spike_bits(void *batch, uint32_t *bits);
+int ns_gpu_import_spike_bits(void *batch, const uint32_t *bits);
 
 typedef struct ns_batch_info {
     int64_t brains, neurons, synapses;

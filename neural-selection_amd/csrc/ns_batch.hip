@@ -1655,10 +1655,21 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
     }
 }
 
+// A 9-slot ring holds the last 9 ticks, which is the whole delay history only
+// for batches whose delays are at most 8 ticks (gym_adapter/core.cuh:18).
+static void check_history9(const Batch &b) {
+    int md = 0;
+    for (const auto &hb : b.brains) md = std::max(md, hb.max_delay);
+    if (md > 8)
+        throw std::runtime_error("Delay bound " + std::to_string(md) +
+                                 " exceeds the 9-slot history; use the spike bit registers");
+}
+
 int ns_gpu_export(void *p, ns_neuron_state *neurons, ns_synapse_state *synapses, uint8_t *history9,
                   int64_t *ticks) {
     try {
         auto &b = *static_cast<Batch *>(p);
+        if (history9) check_history9(b);
         std::vector<ns_neuron_state> n;
         std::vector<ns_synapse_state> s;
         std::vector<uint32_t> h;
@@ -1695,6 +1706,7 @@ int ns_gpu_import(void *p, const ns_neuron_state *neurons, const ns_synapse_stat
                   const uint8_t *history9, const int64_t *ticks) {
     try {
         auto &b = *static_cast<Batch *>(p);
+        if (history9) check_history9(b);
         std::vector<ns_neuron_state> n;
         std::vector<ns_synapse_state> s;
         std::vector<uint32_t> h;
@@ -1736,6 +1748,24 @@ int ns_gpu_export_spike_bits(void *p, uint32_t *bits) {
         std::vector<int64_t> tk;
         b.pool.download_state(n, s, h, tk);
         std::memcpy(bits, h.data(), h.size() * sizeof(uint32_t));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int ns_gpu_import_spike_bits(void *p, const uint32_t *bits) {
+    try {
+        auto &b = *static_cast<Batch *>(p);
+        if (!bits) throw std::runtime_error("No spike registers");
+        std::vector<ns_neuron_state> n;
+        std::vector<ns_synapse_state> s;
+        std::vector<uint32_t> h;
+        std::vector<int64_t> tk;
+        b.pool.download_state(n, s, h, tk);
+        std::memcpy(h.data(), bits, h.size() * sizeof(uint32_t));
+        b.pool.upload_state(n, s, h, tk);
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());

@@ -103,6 +103,7 @@ SIGNATURES = {
     'ns_gpu_export': ([_vp, _vp, _vp, _u8, _i64], ct.c_int),
     'ns_gpu_import': ([_vp, _vp, _vp, _u8, _i64], ct.c_int),
     'ns_gpu_export_spike_bits': ([_vp, _u32], ct.c_int),
+    'ns_gpu_import_spike_bits': ([_vp, _u32], ct.c_int),
     'ns_gpu_info': ([_vp, ct.POINTER(BatchInfo)], ct.c_int),
     'ns_gpu_set_timing': ([_vp, ct.c_int], ct.c_int),
     'ns_gpu_reserve': ([_vp, ct.c_float, ct.c_float], ct.c_int),
@@ -188,6 +189,7 @@ class GenomeHandles:
         self.lib = L
         self.ptrs = []
         self.counts, self.edges = [], []
+        self.max_delay = max_delay
         try:
             for nr, sr, mo in genomes:
                 nr = np.ascontiguousarray(nr, np.float32)
@@ -238,6 +240,7 @@ class GpuBatch:
         self.count = len(counts)
         self.channels = channels
         self.outputs = outputs
+        self.max_delay = max_delay
         self.neuron_counts = np.ascontiguousarray(counts, np.int32)
         self.edge_counts = np.ascontiguousarray(edges, np.int32)
         if handles:
@@ -332,6 +335,8 @@ class GpuBatch:
         for i, w in zip(idx, which):
             self.neuron_counts[i] = handles.counts[w]
             self.edge_counts[i] = handles.edges[w]
+        if len(idx):
+            self.max_delay = max(self.max_delay, handles.max_delay)
 
     def info(self):
         i = BatchInfo()
@@ -339,11 +344,14 @@ class GpuBatch:
         return {k: getattr(i, k) for k, _ in BatchInfo._fields_}
 
     def export(self):
-        """(neuron states, synapse states, history9, ticks) in genome order."""
+        """(neuron states, synapse states, history9, ticks) in genome order;
+        history9 is None when the delay bound exceeds 8 (nine slots would not
+        hold the ring: read spike_bits() instead)."""
         from_dtypes = _state_dtypes()
         ns = np.zeros(int(self.neuron_counts.sum()), from_dtypes[0])
         ss = np.zeros(int(self.edge_counts.sum()), from_dtypes[1])
-        h9 = np.zeros((int(self.neuron_counts.sum()), 9), np.uint8)
+        h9 = (np.zeros((int(self.neuron_counts.sum()), 9), np.uint8) if self.max_delay <= 8
+              else None)
         ticks = np.zeros(self.count, np.int64)
         self._check(self.lib.ns_gpu_export(self.ptr, ns.ctypes.data, ss.ctypes.data,
                                            _ptr(h9, _u8), _ptr(ticks, _i64)))
@@ -358,6 +366,10 @@ class GpuBatch:
         bits = np.zeros(int(self.neuron_counts.sum()), np.uint32)
         self._check(self.lib.ns_gpu_export_spike_bits(self.ptr, _ptr(bits, _u32)))
         return bits
+
+    def import_spike_bits(self, bits):
+        bits = np.ascontiguousarray(bits, np.uint32)
+        self._check(self.lib.ns_gpu_import_spike_bits(self.ptr, _ptr(bits, _u32)))
 
     def close(self):
         if self.ptr:

@@ -76,3 +76,23 @@ def test_maze_requests_mix_equals_alone_equals_cpu():
         nr, sr, mo, seed, tseed = reqs[k]
         cpu = MR.replay(nr, sr, mo, seed, tseed, MAZES, HORIZON)
         assert same(cpu, mixed[k]), (k, cpu, mixed[k])
+
+
+def test_largest_maze_requests_match_cpu():
+    """Maze requests of 2,048 and 1,400 neurons (the largest C5 sizes) beside
+    a small one; each equals the CPU replay."""
+    rng = np.random.default_rng(19)
+    reqs = []
+    for i, size in enumerate((2048, 60, 1400)):
+        nr, sr, mo = W.maze_genome(rng, size)
+        reqs.append((nr, sr, mo, 700 + i, 5000 + i))
+    cfg = ns_amd.ServeConfig(neuron_capacity=200000, synapse_capacity=1600000, episode_capacity=1024,
+                             steps_per_episode=1, ticks_per_step=16, steps_per_launch=4,
+                             noise_sigma=0.0, input_gain=230.0, max_episode_neurons=2048,
+                             task=ns_amd.TASK_MAZE, maze_episodes=MAZES, maze_horizon=HORIZON)
+    print(f'[maze] sizes {[len(r[0]) for r in reqs]}', flush=True)
+    mixed = run(reqs, cfg)
+    for k in (0, 2):
+        nr, sr, mo, seed, tseed = reqs[k]
+        cpu = MR.replay(nr, sr, mo, seed, tseed, MAZES, HORIZON)
+        assert same(cpu, mixed[k]), (k, cpu, mixed[k])

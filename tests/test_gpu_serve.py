@@ -90,3 +90,25 @@ def test_admission_waits_for_free_capacity():
     assert len(res) == len(eps) and stats.admitted == len(eps) and stats.finished == len(eps)
     alone, _ = run([eps[3]], config())
     assert alone[0][0] == res[3][0] and np.array_equal(alone[0][1], res[3][1])
+
+
+def test_largest_episode_classes_match_cpu():
+    """Episodes of the largest slot classes (2,600 and 4,096 neurons: one
+    workgroup each, state in HBM) beside small ones; each result equals the
+    request alone and the CPU replay."""
+    rng = np.random.default_rng(12)
+    eps = []
+    for i, size in enumerate((4096, 40, 2600, 150)):
+        nr, sr, mo = W.episode_genome(rng, size)
+        eps.append((nr, sr, mo, 3000 + i, 90 + i))
+    print(f'[serve] sizes {[len(e[0]) for e in eps]} x {STEPS} steps', flush=True)
+    cfg = config(neurons=400000, synapses=2000000)
+    mixed, stats = run(eps, cfg)
+    assert stats.finished == len(eps)
+    alone, _ = run([eps[0]], config(neurons=400000, synapses=2000000))
+    assert alone[0][0] == mixed[0][0] and np.array_equal(alone[0][1], mixed[0][1])
+    for k in (0, 2):
+        nr, sr, mo, seed, tseed = eps[k]
+        ret, x, m = replay_linear(nr, sr, mo, seed, tseed, STEPS, 0.01)
+        assert ret == mixed[k][0], (k, ret, mixed[k][0])
+        assert np.array_equal(x, mixed[k][1]) and m == mixed[k][2]

@@ -8,6 +8,7 @@ import pytest
 import ns_amd
 import ns_workloads as W
 import oracle as O
+from parity_util import Hip
 
 pytestmark = pytest.mark.gpu
 
@@ -459,43 +460,13 @@ def test_replace_into_empty_pool_and_after_every_death():
         L.ns_pool_destroy(B)
 
 
-class _Hip:
-    """The HIP runtime libns_amd runs on (the same loaded library): device
-    buffers and a stream for the device-pointer entry points."""
-
-    def __init__(self):
-        ns_amd.lib()
-        self.h = ct.CDLL('libamdhip64.so.7')
-        self.h.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
-        self.h.hipFree.argtypes = [ct.c_void_p]
-        self.h.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
-        self.h.hipStreamCreate.argtypes = [ct.POINTER(ct.c_void_p)]
-        self.h.hipStreamDestroy.argtypes = [ct.c_void_p]
-        self.h.hipStreamSynchronize.argtypes = [ct.c_void_p]
-        for f in ('hipMalloc', 'hipFree', 'hipMemcpy', 'hipStreamCreate', 'hipStreamDestroy',
-                  'hipStreamSynchronize'):
-            getattr(self.h, f).restype = ct.c_int
-
-    def alloc(self, nbytes):
-        p = ct.c_void_p()
-        assert self.h.hipMalloc(ct.byref(p), nbytes) == 0
-        return p.value
-
-    def put(self, dst, arr):
-        assert self.h.hipMemcpy(dst, arr.ctypes.data, arr.nbytes, 1) == 0      # host to device
-
-    def get(self, src, arr):
-        assert self.h.hipMemcpy(arr.ctypes.data, src, arr.nbytes, 2) == 0      # device to host
-        return arr
-
-
 def test_device_step_equals_host_step():
     """ns_world_neural_step_device (senses and motors in device memory,
     stream-ordered) equals ns_world_neural_step: motors every step and
     records at the end, across homeostasis, a replacement, a change of the
     freeze flags and a skipped tick."""
     L = ns_amd.lib()
-    hip = _Hip()
+    hip = Hip()
     nr, sr, _ = build_world(seed=41, animals=8)
     dn, ds, _ = build_world(seed=42, animals=2)
     P, ncap, scap = 10, 4000, 16000
