@@ -553,6 +553,8 @@ void plasticity(const SG &g, SS &s, bool arrived, bool post_spike, bool pair, fl
 // src/main.cu:659-801 (delivery, mod-event stamps), 710-746 (neuron_step),
 // 1287-1300 (homeostasis_step).
 // ---------------------------------------------------------------------------
+constexpr int kWorldSenses = 19, kWorldMotors = 5;   // main.cu S, M
+
 struct Brain {
     std::vector<NG> genes;
     std::vector<SG> edges;
@@ -560,6 +562,7 @@ struct Brain {
     std::vector<SS> synapses;
     std::vector<int> motors, sensory;
     std::vector<uint8_t> receptor;        // receptor path for this neuron
+    std::vector<int> ports;               // world: neuron_port (motor j = port kWorldSenses + j)
     std::vector<uint32_t> orig_n, orig_s;
     std::vector<uint8_t> history;         // count x ring
     std::vector<float> exc, inh;
@@ -611,7 +614,10 @@ bool brain_tick(Brain &b, const float *input, float gain, uint8_t *raster) {
             int ch = b.sensory[i];
             if (ch >= 0) {
                 float level = input[ch];
-                current = b.receptor[i] ? receptor_drive(g, s, level) : gain * level;
+                // gym: core.cuh:53 (input_gain * level); world: main.cu:736
+                // (level * 230 pA whatever the caller's gain)
+                current = b.receptor[i] ? receptor_drive(g, s, level)
+                                        : b.world ? level * 230.0f : gain * level;
             }
             membrane_tick(g, s, current, &b.exc[size_t(i) * 4], b.inh[i]);
             if (!std::isfinite(s.voltage_mv)) ok = false;
@@ -725,7 +731,16 @@ bool step_brain(Brain &b, const float *input, int ticks, float gain, float *outp
     for (size_t j = 0; j < b.motors.size(); ++j) output[j] = 0;
     for (int t = 0; t < ticks; ++t, ++b.tick) {
         ok &= brain_tick(b, input, gain, raster ? raster + size_t(t) * n : nullptr);
-        for (size_t j = 0; j < b.motors.size(); ++j) output[j] += b.neurons[b.motors[j]].spike;
+        if (b.ports.empty()) {
+            for (size_t j = 0; j < b.motors.size(); ++j) output[j] += b.neurons[b.motors[j]].spike;
+        } else {
+            // main.cu:742-743: every active neuron on a motor port counts its spike
+            for (int i = 0; i < n; ++i) {
+                const int m = b.ports[i] - kWorldSenses;
+                if (b.genes[i].active && b.neurons[i].spike && m >= 0 && m < kWorldMotors)
+                    output[m] += 1;
+            }
+        }
     }
     if (!b.world) {
         for (int j = 0; j < n; ++j)                          // core.cuh:70-72
@@ -775,7 +790,41 @@ void *ora_create_records(int count, int edges, int channels, const ns_neuron_gen
     return b;
 }
 
+// One animal's brain under the world rules from its checkpoint genes (pre/post
+// animal-local) and neuron ports, with the routing main.cu derives from them:
+// a sensory channel in [0, 19) reads that sense level (main.cu:729-737),
+// through receptor_current for type 5 below channel 6 and as 230 pA per level
+// otherwise; motor j counts the spikes of neurons on port 19 + j
+// (main.cu:742-743).  ora_step's input is the animal's 19 sense levels and its
+// output the 5 motor counts.
+void *ora_create_world(int count, int edges, const ns_neuron_gene *genes, const ns_synapse_gene *syn,
+                       const int *ports, int max_delay) {
+    auto b = new Brain;
+    b->channels = kWorldSenses;
+    b->ring = max_delay + 1;
+    b->world = 1;
+    b->genes.assign(genes, genes + count);
+    b->edges.assign(syn, syn + edges);
+    b->ports.assign(ports, ports + count);
+    for (int i = 0; i < count; ++i) {
+        const int ch = genes[i].sensory_channel;
+        const bool sensed = ch >= 0 && ch < kWorldSenses;
+        b->sensory.push_back(sensed ? ch : -1);
+        b->receptor.push_back(sensed && genes[i].type == 5 && ch < 6);
+    }
+    b->motors.assign(kWorldMotors, 0);
+    size_scratch(*b);
+    return b;
+}
+
 void ora_destroy(void *p) { delete static_cast<Brain *>(p); }
+
+// The brain's gene records (neurons, then synapses with brain-local pre/post).
+void ora_genes(void *p, ns_neuron_gene *genes, ns_synapse_gene *syn) {
+    auto &b = *static_cast<Brain *>(p);
+    if (genes) std::memcpy(genes, b.genes.data(), b.genes.size() * sizeof(NG));
+    if (syn && !b.edges.empty()) std::memcpy(syn, b.edges.data(), b.edges.size() * sizeof(SG));
+}
 
 void ora_original_indices(void *p, const uint32_t *n, const uint32_t *s) {
     auto &b = *static_cast<Brain *>(p);

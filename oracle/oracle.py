@@ -79,7 +79,10 @@ def oracle_lib():
         lib.ora_create_records.argtypes = [ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _i32, _u8,
                                            ct.c_int, _i32, ct.c_int, ct.c_int]
         lib.ora_create_records.restype = _vp
+        lib.ora_create_world.argtypes = [ct.c_int, ct.c_int, _vp, _vp, _i32, ct.c_int]
+        lib.ora_create_world.restype = _vp
         lib.ora_destroy.argtypes = [_vp]
+        lib.ora_genes.argtypes = [_vp, _vp, _vp]
         lib.ora_original_indices.argtypes = [_vp, _u32, _u32]
         lib.ora_reset.argtypes = [_vp, ct.c_uint32]
         lib.ora_step.argtypes = [_vp, _f, ct.c_int, ct.c_float, _f, _u8]
@@ -192,6 +195,12 @@ class OracleBrain:
         self.lib.ora_export(self.ptr, ns.ctypes.data, ss.ctypes.data, ptr(h9, _u8), None,
                             ptr(tick, _i64), None)
         return ns, ss, h9, int(tick[0])
+
+    def genes(self):
+        ng = np.zeros(self.count, NEURON_GENE)
+        sg = np.zeros(self.edges, SYNAPSE_GENE)
+        self.lib.ora_genes(self.ptr, ng.ctypes.data, sg.ctypes.data)
+        return ng, sg
 
     def close(self):
         if self.ptr:

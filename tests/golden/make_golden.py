@@ -3,7 +3,7 @@ path (gym_adapter/native.cpp + src/physiology.cuh + gym_adapter/core.cuh,
 host-compiled by oracle/Makefile into oracle/_ref/libns_ref.so).
 
 Run in a container that has the reference checkout (make -C oracle first):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture ...]   (default: all)
 
 Fixtures (numpy .npz, data only):
   c1_reference.npz    C1 workload (64 brains x 32 neurons, 200 calls x 16
@@ -15,6 +15,13 @@ Fixtures (numpy .npz, data only):
   mix_reference.npz   ragged brains with every cell type (0-6), plasticity
                       profiles 1-5, modulators and inactive cells: per-call
                       outputs and state hashes, final state.
+  world_reduction.npz gym-path runs of brains in which every world-only rule
+                      is neutral (no cortical type-0 cells, so homeostasis
+                      does nothing; every cell active; receptor sensors on
+                      channels below 6): per-call outputs and state hashes
+                      and the final state, for the oracle's world harness to
+                      reduce to.  Two cases: AdEx/LFS cells over 30 calls
+                      and LP/Prinz cells over 8.
   functions.npz       per-function input/output records for neuron_advance
                       (all 7 types), receptor_current, synapse_release,
                       synapse_plasticity_step (profiles 0-5), homeostatic
@@ -107,6 +114,28 @@ def mix_fixture():
                         final_history=np.concatenate([f[2] for f in final]),
                         final_ticks=np.array([f[3] for f in final], np.int64))
     print('mix_reference.npz', outputs.sum())
+
+
+WORLD_CASES = {
+    'adex': dict(seed=81, brains=8, calls=30, types=(1, 4, 5, 6), active_p=1.0, modulator_p=0.15,
+                 lo=12, hi=60),
+    'slow': dict(seed=82, brains=3, calls=8, types=(1, 2, 3, 4, 5), active_p=1.0, modulator_p=0.15,
+                 lo=8, hi=24),
+}
+
+
+def world_reduction_fixture():
+    rec = {}
+    for name, kw in WORLD_CASES.items():
+        w = W.small_mix(**kw)
+        outputs, hashes, final = run_workload(w, kw['calls'], keep_final=w.count)
+        rec[f'{name}_outputs'] = outputs.astype(np.uint8)
+        rec[f'{name}_hashes'] = hashes
+        rec[f'{name}_final_neurons'] = O.cat_records([f[0] for f in final], O.NEURON_STATE).view(np.uint8)
+        rec[f'{name}_final_synapses'] = O.cat_records([f[1] for f in final], O.SYNAPSE_STATE).view(np.uint8)
+        rec[f'{name}_final_history'] = np.concatenate([f[2] for f in final])
+        print('world_reduction', name, outputs.sum())
+    np.savez_compressed(OUT / 'world_reduction.npz', **rec)
 
 
 def functions_fixture():
@@ -268,7 +297,7 @@ if __name__ == '__main__':
     if not O.ref_available():
         sys.exit('oracle/_ref/libns_ref.so missing: build it with make -C oracle (needs the '
                  'reference checkout)')
-    c1_fixture()
-    c1_ticks_fixture()
-    mix_fixture()
-    functions_fixture()
+    fixtures = {'c1': c1_fixture, 'c1_ticks': c1_ticks_fixture, 'mix': mix_fixture,
+                'world_reduction': world_reduction_fixture, 'functions': functions_fixture}
+    for name in sys.argv[1:] or list(fixtures):
+        fixtures[name]()

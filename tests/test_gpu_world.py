@@ -96,16 +96,9 @@ def oracle_world(nr, sr, brains):
         sgl = np.ascontiguousarray(syn['gene']).copy()
         sgl['pre'] = [local[int(x)] for x in syn['gene']['pre']]
         sgl['post'] = [local[int(x)] for x in syn['gene']['post']]
-        chs = mine['gene']['sensory_channel'].astype(np.int32)
-        sensory = np.where((chs >= 0) & (chs < 19), a * 19 + chs, -1).astype(np.int32)
-        receptor = ((mine['gene']['type'] == 5) & (chs >= 0) & (chs < 6)).astype(np.uint8)
-        motors = np.zeros(5, np.int32)
-        for i, p in enumerate(mine['port']):
-            if p >= 19:
-                motors[p - 19] = i
-        h = lib.ora_create_records(len(g), len(sgl), 19 * len(brains), g.ctypes.data, sgl.ctypes.data,
-                                   sensory.ctypes.data_as(O._i32), receptor.ctypes.data_as(O._u8), 5,
-                                   motors.ctypes.data_as(O._i32), 8, 1)
+        ports = np.ascontiguousarray(mine['port'], np.int32)
+        h = lib.ora_create_world(len(g), len(sgl), g.ctypes.data, sgl.ctypes.data,
+                                 ports.ctypes.data_as(O._i32), 8)
         ns = np.ascontiguousarray(mine['state'])
         ss = np.ascontiguousarray(syn['state'])
         lib.ora_import(h, ns.ctypes.data, ss.ctypes.data, None, 0)
@@ -142,7 +135,8 @@ def test_world_steps_match_oracle_and_round_trip_records(types):
                 lib.ora_set_tick(h, tick)
                 lib.ora_set_frozen(h, int(frozen[a]))
                 out = np.zeros(5, np.float32)
-                lib.ora_step(h, senses.ctypes.data_as(O._f), 16, 230.0, out.ctypes.data_as(O._f), None)
+                mine = np.ascontiguousarray(senses[a * 19:(a + 1) * 19])
+                lib.ora_step(h, mine.ctypes.data_as(O._f), 16, 230.0, out.ctypes.data_as(O._f), None)
                 ora_motors[a * 5:(a + 1) * 5] += out
                 lib.ora_world_homeostasis(h, int(age[a]))
         assert np.array_equal(motors, ora_motors.astype(np.int32))
