@@ -154,6 +154,29 @@ void ns_original_indices(void *brain, const uint32_t *neurons,
 /* gym_adapter/native.cpp:88 */
 void ns_destroy(void *brain);
 
+/* gym_adapter/native.cpp:89-94 ns_reset: tick 0 and the initial state of
+ * every neuron and synapse (physiology.cuh:179-202, 393-404) seeded
+ * seed + original * 7919 / + original * 104729, with the original indices of
+ * ns_original_indices when set, else the genome order.  Host only. */
+void ns_reset(void *brain, uint32_t seed);
+
+/* gym_adapter/native.cpp:95-109 ns_step: `ticks` (1..10000) one-ms ticks of
+ * the handle's brain with constant encoded input (channels floats, finite, in
+ * [0,1]) and input gain (>= 0); output = per-motor spike counts; the same
+ * checks and messages.  The handle's state moves through a one-brain device
+ * pool (made on the first call, held until ns_destroy) and back,
+ * synchronously, on the same kernels as ns_gpu_step.  A nonfinite voltage
+ * returns -1 ("Nonfinite neural state") with every tick run and that state
+ * kept, as the reference's device batch does (gym_adapter/gpu.cu:48-64); the
+ * reference's CPU path stops inside the tick instead. */
+int ns_step(void *brain, const float *input, int ticks, float gain, float *output);
+
+/* A handle's state (no reference counterpart): neuron and synapse records
+ * in genome order, the spike shift registers (bit k = spike k ticks before
+ * the last completed tick) and the tick.  Any pointer may be NULL. */
+int ns_export(void *brain, ns_neuron_state *neurons, ns_synapse_state *synapses,
+              uint32_t *spike_bits, int64_t *tick);
+
 /* gym_adapter/gpu.cu:67-109 ns_gpu_create.  Borrows `count` handles made by
  * ns_create (or ns_create_ex); the caller may destroy them afterwards.  All
  * brains must share channels and outputs.  Uploads genes into the

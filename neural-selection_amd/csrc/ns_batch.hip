@@ -138,9 +138,6 @@ HostBrain *parse_gym_brain(int count, int edges, int channels, int outputs, cons
 }
 
 static inline bool excitatory_cell(unsigned t) { return t == 0 || t == 4 || t == 5; }
-static inline int site_count(const ns_synapse_gene &g) {   // physiology.cuh:376-378
-    return g.release_sites ? (g.release_sites > 64 ? 64 : g.release_sites) : 14;
-}
 
 // ---------------------------------------------------------------------------
 // Pool builder
@@ -996,11 +993,14 @@ void DevicePool::upload_state(const std::vector<ns_neuron_state> &in_n,
 // ---------------------------------------------------------------------------
 // Batch
 // ---------------------------------------------------------------------------
-static Tile default_tile(const std::vector<HostBrain> &brains) {
+Tile default_tile(const std::vector<const HostBrain *> &brains) {
     // Pick the smallest standard tile whose pack holds the typical (90th
     // percentile) brain; ns_gpu_autotune refines this by measurement.
     std::vector<int> n, s;
-    for (const auto &b : brains) { n.push_back(int(b.genes.size())); s.push_back(int(b.edges.size())); }
+    for (const HostBrain *b : brains) {
+        n.push_back(int(b->genes.size()));
+        s.push_back(int(b->edges.size()));
+    }
     std::sort(n.begin(), n.end());
     std::sort(s.begin(), s.end());
     const int n90 = n[n.size() * 9 / 10], s90 = s[s.size() * 9 / 10];
@@ -1025,7 +1025,9 @@ void Batch::create(std::vector<HostBrain> &&owned, int) {
     specs.clear();
     for (int i = 0; i < count; ++i)
         specs.push_back(BrainSpec{&brains[i], i, i * channels, i * outputs, world});
-    const Tile t0 = default_tile(brains);
+    std::vector<const HostBrain *> views;
+    for (const auto &b : brains) views.push_back(&b);
+    const Tile t0 = default_tile(views);
     pool.build(slot_specs(t0), outputs, t0, stream);
     d_input = io.alloc<float>(int64_t(count) * channels);
     d_output = io.alloc<float>(int64_t(count) * outputs);
@@ -1330,7 +1332,7 @@ const char *ns_error(void) { return g_error.c_str(); }
 void *ns_create(int count, int edges, int channels, int outputs, const float *ng, const float *sg,
                 const int *motors) {
     try {
-        return parse_gym_brain(count, edges, channels, outputs, ng, sg, motors, 8);
+        return make_handle(parse_gym_brain(count, edges, channels, outputs, ng, sg, motors, 8));
     } catch (const std::exception &e) {
         set_error(e.what());
         return nullptr;
@@ -1340,7 +1342,7 @@ void *ns_create(int count, int edges, int channels, int outputs, const float *ng
 void *ns_create_ex(int count, int edges, int channels, int outputs, const float *ng,
                    const float *sg, const int *motors, int max_delay) {
     try {
-        return parse_gym_brain(count, edges, channels, outputs, ng, sg, motors, max_delay);
+        return make_handle(parse_gym_brain(count, edges, channels, outputs, ng, sg, motors, max_delay));
     } catch (const std::exception &e) {
         set_error(e.what());
         return nullptr;
@@ -1348,12 +1350,12 @@ void *ns_create_ex(int count, int edges, int channels, int outputs, const float 
 }
 
 void ns_original_indices(void *p, const uint32_t *neurons, const uint32_t *synapses) {
-    auto &b = *static_cast<HostBrain *>(p);
+    auto &b = handle_brain(p);
     b.orig_n.assign(neurons, neurons + b.genes.size());
     if (!b.edges.empty()) b.orig_s.assign(synapses, synapses + b.edges.size());
 }
 
-void ns_destroy(void *p) { delete static_cast<HostBrain *>(p); }
+void ns_destroy(void *p) { delete static_cast<GenomeHandle *>(p); }
 
 static void *make_batch(std::vector<HostBrain> &&brains, bool world) {
     auto *batch = new Batch;
@@ -1372,7 +1374,7 @@ void *ns_gpu_create(void **handles, int count) {
         if (count < 1) throw std::runtime_error("Empty GPU batch");
         std::vector<HostBrain> brains;
         brains.reserve(count);
-        for (int i = 0; i < count; ++i) brains.push_back(*static_cast<HostBrain *>(handles[i]));
+        for (int i = 0; i < count; ++i) brains.push_back(handle_brain(handles[i]));
         return make_batch(std::move(brains), false);
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -1539,7 +1541,7 @@ int ns_gpu_replace(void *p, int count, const int *indices, void **handles, const
         fresh.reserve(count);
         for (int j = 0; j < count; ++j) {
             if (!handles[j]) throw std::runtime_error("Invalid genome handle");
-            fresh.push_back(*static_cast<HostBrain *>(handles[j]));
+            fresh.push_back(handle_brain(handles[j]));
         }
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b.stream;
         b.replace(count, indices, std::move(fresh), seeds, st);

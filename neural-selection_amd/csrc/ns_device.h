@@ -70,6 +70,51 @@ constexpr float kDepolGateMv = 2.3f;   // profile-2 LTP gate on the post's recen
 // World-rule ring length (src/main.cu:45-46 HISTORY = 9).
 constexpr int kWorldHistory = 9;
 
+// ---------------------------------------------------------------------------
+// Initial states (physiology.cuh:179-202, 393-404), shared by the device reset
+// kernels and the host-side genome handles.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t mix_seed(uint32_t seed) {   // physiology.cuh:183-186
+    uint32_t x = seed ? seed : 1;
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Cell union of a fresh neuron of `type` (LP: lp_cell.cuh initial values;
+// Prinz: prinz_cell.cuh; LFS: voltage at rest, threshold at the gene's);
+// returns its initial voltage.
+__host__ __device__ inline float initial_cell(uint32_t type, float rest, float thr, float *cell) {
+    for (int k = 0; k < 16; ++k) cell[k] = 0.0f;
+    float v = rest;
+    if (type == 2) {
+        const float lp[16] = {-56, 0, .2f, 0, 1, 0, 0, 1, 0, 0, 1, 1, 0, .04f, .01f, -56};
+        for (int k = 0; k < 16; ++k) cell[k] = lp[k];
+        v = cell[15] - 9.342518882f;
+    } else if (type == 3) {
+        cell[0] = -50.0f; cell[1] = .05f;
+        for (int k = 0; k < 4; ++k) cell[9 + k] = 1.0f;
+        v = cell[0];
+    } else if (type == 6) {
+        cell[0] = rest;
+        cell[1] = thr;
+    }
+    return v;
+}
+
+// xorshift generator state of a fresh neuron or synapse seeded `seed` (never
+// 0: physiology.cuh:188, 399).
+__host__ __device__ inline uint32_t initial_rng(uint32_t seed) {
+    const uint32_t x = mix_seed(seed);
+    return x ? x : 1;
+}
+
+// Ready mask of `sites` release sites (physiology.cuh:402-403).
+__host__ __device__ inline uint64_t initial_ready(int sites) {
+    return sites == 64 ? ~uint64_t(0) : (uint64_t(1) << sites) - 1;
+}
+
 struct Pool {
     int64_t ncap, scap;
     // neurons

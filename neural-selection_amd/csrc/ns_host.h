@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <algorithm>
 #include <stdexcept>
 #include <string>
@@ -43,23 +44,51 @@ bool has_plastic_pair(const HostBrain &b);
 bool has_slow_cell(const HostBrain &b);
 HostBrain *parse_gym_brain(int count, int edges, int channels, int outputs, const float *ng,
                            const float *sg, const int *motors, int max_delay);
+inline int site_count(const ns_synapse_gene &g) {   // physiology.cuh:376-378
+    return g.release_sites ? (g.release_sites > 64 ? 64 : g.release_sites) : 14;
+}
 
-// Owning list of device allocations.
+// A carved device range ran out (see DeviceArena::carve).
+struct CarveFull : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Owning list of device allocations, or, after carve(), consecutive 256-byte
+// aligned ranges of one caller-owned device block (no allocation; CarveFull
+// when the block is exhausted).
 struct DeviceArena {
     std::vector<void *> blocks;
     int64_t bytes = 0;
+    unsigned char *carve_base = nullptr;
+    size_t carve_cap = 0, carve_used = 0;
     template <class T> T *alloc(int64_t count) {
         if (count <= 0) count = 1;
+        const size_t n = sizeof(T) * size_t(count);
+        if (carve_base) {
+            const size_t off = (carve_used + 255) & ~size_t(255);
+            if (off > carve_cap || n > carve_cap - off)
+                throw CarveFull("static device block of " + std::to_string(carve_cap) +
+                                " bytes exhausted");
+            carve_used = off + n;
+            bytes += int64_t(n);
+            return reinterpret_cast<T *>(carve_base + off);
+        }
         void *p = nullptr;
-        NS_HIP(hipMalloc(&p, sizeof(T) * size_t(count)));
+        NS_HIP(hipMalloc(&p, n));
         blocks.push_back(p);
-        bytes += int64_t(sizeof(T)) * count;
+        bytes += int64_t(n);
         return static_cast<T *>(p);
+    }
+    void carve(void *base, size_t cap) {
+        release();
+        carve_base = static_cast<unsigned char *>(base);
+        carve_cap = cap;
     }
     void release() {
         for (void *p : blocks) (void)hipFree(p);
         blocks.clear();
         bytes = 0;
+        carve_used = 0;
     }
     ~DeviceArena() { release(); }
 };
@@ -381,5 +410,36 @@ struct Batch {
              hipStream_t st);
     ~Batch();
 };
+
+// Default tile for a set of brains (the pool builder's first layout).
+Tile default_tile(const std::vector<const HostBrain *> &brains);
+
+// One-brain device pool a genome handle steps on (ns_step): pool arrays plus
+// the call's input and output rows.
+struct HandleDevice {
+    DevicePool pool;
+    float *d_in = nullptr, *d_out = nullptr;
+};
+
+// Genome handle of ns_create / ns_create_ex: the genome and the state the
+// reference's CPU handle keeps with it (gym_adapter/native.cpp:23-34):
+// neuron and synapse states, the delay ring as spike shift registers, and the
+// tick.  ns_step's one-brain device pool is made on its first call.
+struct GenomeHandle {
+    HostBrain brain;
+    std::vector<ns_neuron_state> neurons;
+    std::vector<ns_synapse_state> synapses;
+    std::vector<uint32_t> hist;
+    int64_t tick = 0;
+    std::unique_ptr<HandleDevice> device;
+};
+inline HostBrain &handle_brain(void *h) { return static_cast<GenomeHandle *>(h)->brain; }
+GenomeHandle *make_handle(HostBrain *parsed);
+// ns_step's argument checks (native.cpp:97-98), the one-brain pool layout and
+// one step on it, shared by the allocating and the static-pool entry points.
+void check_step(const HostBrain &b, const float *input, int ticks, float gain);
+void build_one_brain(DevicePool &pool, const HostBrain &b, float *&d_in, float *&d_out);
+void step_one_brain(DevicePool &pool, GenomeHandle &h, float *d_in, float *d_out,
+                    const float *input, int ticks, float gain, float *output);
 
 }  // namespace nsa

@@ -28,14 +28,6 @@ __device__ inline bool need_ok(const Pool &p, int32_t b, uint8_t need) {
     return (p.brun[b] & need) == need;
 }
 
-__device__ inline uint32_t mix_seed(uint32_t seed) {   // physiology.cuh:183-186
-    uint32_t x = seed ? seed : 1;
-    x ^= x >> 16; x *= 0x7feb352du;
-    x ^= x >> 15; x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
-}
-
 // ---------------------------------------------------------------------------
 // Reset: initial_neuron_state / initial_synapse_state (physiology.cuh:179-202,
 // 393-404) seeded as gym_adapter/gpu.cu:18-23.  need = 1: brains with brun&1.
@@ -45,28 +37,13 @@ __device__ inline uint32_t mix_seed(uint32_t seed) {   // physiology.cuh:183-186
 // prinz_cell.cuh); `orig` is its original genome index.
 __device__ inline void reset_neuron(const Pool &p, int64_t i, uint32_t seed, uint32_t orig) {
     const int64_t N = p.ncap;
-    uint32_t type = p.nflags[i] & NF_TYPE;
-    float rest = p.ng[NG_REST * N + i];
+    const uint32_t type = p.nflags[i] & NF_TYPE;
     for (int k = 0; k < NS_FIELDS; ++k) p.ns[k * N + i] = 0.0f;
     float cell[kCellFields];
-    for (int k = 0; k < kCellFields; ++k) cell[k] = 0.0f;
-    float v = rest;
-    if (type == 2) {
-        const float lp[16] = {-56, 0, .2f, 0, 1, 0, 0, 1, 0, 0, 1, 1, 0, .04f, .01f, -56};
-        for (int k = 0; k < 16; ++k) cell[k] = lp[k];
-        v = cell[15] - 9.342518882f;
-    } else if (type == 3) {
-        cell[0] = -50.0f; cell[1] = .05f;
-        for (int k = 0; k < 4; ++k) cell[9 + k] = 1.0f;
-        v = cell[0];
-    } else if (type == 6) {
-        cell[0] = rest;
-        cell[1] = p.ng[NG_THR * N + i];
-    }
+    const float v = initial_cell(type, p.ng[NG_REST * N + i], p.ng[NG_THR * N + i], cell);
     for (int k = 0; k < kCellFields; ++k) p.cell[k * N + i] = cell[k];
     p.ns[NS_V * N + i] = v;
-    uint32_t x = mix_seed(seed + orig * 7919u);
-    p.rrng[i] = x ? x : 1;
+    p.rrng[i] = initial_rng(seed + orig * 7919u);
     p.total[i] = 0;
     p.spike[i] = 0;
     p.hist[i] = 0;
@@ -77,17 +54,14 @@ __device__ inline void reset_neuron(const Pool &p, int64_t i, uint32_t seed, uin
 // initial_synapse_state (physiology.cuh:393-404) of synapse q.
 __device__ inline void reset_synapse(const Pool &p, int64_t q, uint32_t seed, uint32_t orig) {
     const int64_t S = p.scap;
-    uint32_t x = mix_seed(seed + orig * 104729u);
-    if (!x) x = 1;
     float init[SS_FIELDS] = {};
     init[SS_WEIGHT] = p.sg[SG_W0 * S + q];
     init[SS_DA_AGE] = -1; init[SS_SPINE] = 1; init[SS_SAGE] = -1;
     init[SS_PRE_AGE] = 1000000; init[SS_POST_AGE] = 1000000; init[SS_SLOW] = 1;
     for (int k = 0; k < SS_FIELDS; ++k) p.ss[k * S + q] = init[k];
-    int sites = (p.sflags[q] >> SF_SITES_SHIFT) & 127;
-    p.ready[q] = sites == 64 ? ~uint64_t(0) : (uint64_t(1) << sites) - 1;
+    p.ready[q] = initial_ready((p.sflags[q] >> SF_SITES_SHIFT) & 127);
     p.last[q] = -1000000;
-    p.srng[q] = x;
+    p.srng[q] = initial_rng(seed + orig * 104729u);
     p.arrived[q] = 0;
     p.released[q] = 0;
 }
