@@ -171,6 +171,21 @@ void ns_reset(void *brain, uint32_t seed);
  * reference's CPU path stops inside the tick instead. */
 int ns_step(void *brain, const float *input, int ticks, float gain, float *output);
 
+/* ns_step without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid it).  The same checks, kernels and
+ * results as ns_step, with the one-brain pool laid out afresh on each call in
+ * a static device array of NS_STEP_STATIC_BYTES per GPU (held until the
+ * process ends) instead of memory of the handle's own.  Brains whose pool
+ * does not fit (about 300 B per neuron and 120 B per synapse: 4,096 neurons
+ * with 16,384 synapses fit) are refused with an error and the handle
+ * unchanged.  Calls on one GPU share the array and the last call's contents
+ * stay in it; this process's calls are serialised per GPU, and overlapping
+ * calls from outside that lock (another copy of the library in the process)
+ * would race. */
+#define NS_STEP_STATIC_BYTES (4u << 20)
+int ns_step_static(void *brain, const float *input, int ticks, float gain, float *output);
+
 /* A handle's state (no reference counterpart): neuron and synapse records
  * in genome order, the spike shift registers (bit k = spike k ticks before
  * the last completed tick) and the tick.  Any pointer may be NULL. */

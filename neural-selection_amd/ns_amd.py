@@ -93,6 +93,7 @@ SIGNATURES = {
     'ns_reset': ([_vp, ct.c_uint32], None),
     'ns_step': ([_vp, _f, ct.c_int, ct.c_float, _f], ct.c_int),
     'ns_export': ([_vp, _vp, _vp, _u32, _i64], ct.c_int),
+    'ns_step_static': ([_vp, _f, ct.c_int, ct.c_float, _f], ct.c_int),
     'ns_gpu_create': ([ct.POINTER(_vp), ct.c_int], _vp),
     'ns_gpu_create_bulk': ([ct.c_int, _i32, _i32, ct.c_int, ct.c_int, _f, _f, _i32, ct.c_int,
                             ct.c_int], _vp),
