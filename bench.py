@@ -81,41 +81,79 @@ def describe(name, w, churn=0.0):
     return base
 
 
-def cpu_baseline(w, seconds=15.0, max_neurons=65536):
-    """The CPU oracle (bit-identical to the reference's CPU path) on the host
-    cores, over the first brains of the same workload."""
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process
+    may use, the CPU model and the C library."""
+    import platform
+    model = ''
+    try:
+        for line in Path('/proc/cpuinfo').read_text().splitlines():
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {'nproc': os.cpu_count(), 'usable_cpus': usable, 'model': model,
+            'libc': ' '.join(platform.libc_ver())}
+
+
+# Host threads for the CPU legs: the CPUs this process may use, at most the
+# 16 that one GPU's share of the machine holds.
+HOST_THREADS_CAP = 16
+
+
+def cpu_baseline(w, seconds=10.0, share=1 / 32):
+    """The CPU oracle (bit-identical to the reference's CPU path) on a sample
+    of the same workload's brains: every 1/share-th brain in size order, so
+    the sample spans the size distribution (its largest brains included).
+    Timed twice: one thread, then threaded over brains on the host's share."""
     import ctypes as ct
     import oracle as O
     lib = O.oracle_lib()
-    n_b, n_n = 0, 0
-    while n_b < w.count and n_n + int(w.neuron_counts[n_b]) <= max_neurons:
-        n_n += int(w.neuron_counts[n_b])
-        n_b += 1
-    n_b = max(n_b, 1)
+    order = np.argsort(w.neuron_counts, kind='stable')
+    step = max(1, int(round(1 / share)))
+    pick = np.sort(order[step // 2::step])
     seeds = w.seeds()
     brains = []
-    for i in range(n_b):
-        n, s, m = w.brain(i)
+    for i in pick:
+        n, s, m = w.brain(int(i))
         b = O.OracleBrain(n, s, m, w.channels, max_delay=max(8, w.max_delay))
         b.reset(seeds[i])
         brains.append(b)
-    threads = min(os.cpu_count() or 1, 16)
+    n_b = len(brains)
     arr = (ct.c_void_p * n_b)(*[b.ptr for b in brains])
     out = np.zeros((n_b, w.outputs), np.float32)
     status = np.zeros(n_b, np.int32)
-    neurons = int(w.neuron_counts[:n_b].sum())
-    calls, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds and calls < 20000:
-        x = np.ascontiguousarray(w.inputs(calls)[:n_b])
-        lib.ora_batch_step(arr, n_b, x.ctypes.data_as(O._f), w.channels, None, w.ticks, w.gain,
-                           out.ctypes.data_as(O._f), w.outputs, status.ctypes.data_as(O._i32),
-                           threads)
-        calls += 1
-    dt = time.perf_counter() - t0
-    value = neurons * w.ticks * calls / dt
-    return {'value': value, 'unit': 'neuron-updates/s', 'cores': threads, 'kind': 'port',
-            'sample': f'first {n_b} brains ({neurons} neurons) of the same workload, {calls} calls '
-                      f'x {w.ticks} ticks, {dt:.1f} s wall, oracle/liboracle.so threaded over brains'}
+    neurons = int(w.neuron_counts[pick].sum())
+    host = host_info()
+    threads = max(1, min(host['usable_cpus'], HOST_THREADS_CAP))
+
+    def leg(nthreads):
+        calls, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds and calls < 20000:
+            x = np.ascontiguousarray(w.inputs(calls)[pick])
+            lib.ora_batch_step(arr, n_b, x.ctypes.data_as(O._f), w.channels, None, w.ticks, w.gain,
+                               out.ctypes.data_as(O._f), w.outputs, status.ctypes.data_as(O._i32),
+                               nthreads)
+            calls += 1
+        dt = time.perf_counter() - t0
+        return neurons * w.ticks * calls / dt, calls, dt
+
+    one, calls1, dt1 = leg(1)
+    many, calls_n, dt_n = leg(threads)
+    return {'value': many, 'unit': 'neuron-updates/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{n_b} brains ({neurons} neurons, {neurons / w.total_neurons:.1%} of the pool): '
+                      f'every {step}th brain in size order, {int(w.neuron_counts[pick].min())} to '
+                      f'{int(w.neuron_counts[pick].max())} neurons; {calls_n} calls x {w.ticks} ticks '
+                      f'in {dt_n:.1f} s, oracle/liboracle.so threaded over brains',
+            'single_thread': {'value': one, 'cores': 1, 'calls': calls1, 'seconds': dt1},
+            'host': host,
+            'threads_note': f'{threads} threads: the CPUs this process may use, capped at the '
+                            f'{HOST_THREADS_CAP}-CPU share of one GPU'}
 
 
 def serving_cpu_baseline(reqs, run_one, work_of, budget=1.5e9):
@@ -131,7 +169,7 @@ def serving_cpu_baseline(reqs, run_one, work_of, budget=1.5e9):
             break
         sample.append(r)
         est += w
-    threads = min(os.cpu_count() or 1, 16)
+    threads = max(1, min(host_info()['usable_cpus'], HOST_THREADS_CAP))
     t0 = time.perf_counter()
     done, lat = 0.0, []
     with cf.ThreadPoolExecutor(threads) as ex:
@@ -175,56 +213,64 @@ def serving_genomes(count, seed, lo=16, hi=4096):
     return out
 
 
+def serve_run(eps, rate, episode_steps):
+    """One continuous-batching run: `eps` submitted at Poisson times of rate
+    `rate` per second while earlier ones run; per-episode latency (submit to
+    finish), admission wait and whole-run throughput."""
+    import ctypes as ct
+    L = ns_amd.lib()
+    f = ct.POINTER(ct.c_float)
+    total_neurons = sum(len(e[0]) for e in eps)
+    cfg = ns_amd.ServeConfig(neuron_capacity=1 << 22, synapse_capacity=1 << 24,
+                             episode_capacity=8192, steps_per_episode=episode_steps,
+                             ticks_per_step=16, steps_per_launch=25, noise_sigma=0.01,
+                             input_gain=230.0)
+    srv = L.ns_serve_create(ct.byref(cfg))
+    if not srv:
+        raise SystemExit(ns_amd.error())
+    gaps = np.random.default_rng(int(rate)).exponential(1.0 / rate, len(eps))
+    t0 = time.perf_counter()
+    due = t0
+    for k, (nr, sr, mo, seed, tseed) in enumerate(eps):
+        due += gaps[k]
+        delay = due - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        if L.ns_serve_submit(srv, len(nr), len(sr), nr.ctypes.data_as(f), sr.ctypes.data_as(f),
+                             mo.ctypes.data_as(ct.POINTER(ct.c_int)), seed, tseed) < 0:
+            raise SystemExit(ns_amd.error())
+    if L.ns_serve_drain(srv):
+        raise SystemExit(ns_amd.error())
+    wall = time.perf_counter() - t0
+    res = (ns_amd.ServeResult * len(eps))()
+    n = L.ns_serve_poll(srv, res, len(eps))
+    stats = ns_amd.ServeStats()
+    L.ns_serve_stats_get(srv, ct.byref(stats))
+    L.ns_serve_destroy(srv)
+    lat = np.array([r.finish_s - r.submit_s for r in res[:n]]) * 1e3
+    wait = np.array([r.admit_s - r.submit_s for r in res[:n]]) * 1e3
+    nu = total_neurons * episode_steps * 16
+    return {'rate_per_s': rate, 'episodes': n, 'wall_s': wall,
+            'episodes_per_s': n / wall, 'neuron_updates_per_s': nu / wall,
+            'latency_ms': {'p50': float(np.percentile(lat, 50)),
+                           'p99': float(np.percentile(lat, 99)),
+                           'max': float(lat.max())},
+            'admission_wait_ms': {'p50': float(np.percentile(wait, 50)),
+                                  'p99': float(np.percentile(wait, 99))},
+            'launch_rounds': stats.launches, 'resident_cohorts_peak': stats.resident_peak,
+            'device_busy_s': stats.busy_s}
+
+
 def bench_serving(args):
     """Continuous batching under Poisson arrivals: per-episode latency (submit
     to finish) and whole-run throughput, one run per arrival rate."""
-    import ctypes as ct
-    L = ns_amd.lib()
     rates = [float(x) for x in args.rates.split(',')]
     eps = serving_genomes(args.episodes, seed=17)
     total_neurons = sum(len(e[0]) for e in eps)
-    f = ct.POINTER(ct.c_float)
     runs = []
     for rate in rates:
-        cfg = ns_amd.ServeConfig(neuron_capacity=1 << 22, synapse_capacity=1 << 24,
-                                 episode_capacity=8192, steps_per_episode=args.episode_steps,
-                                 ticks_per_step=16, steps_per_launch=25, noise_sigma=0.01,
-                                 input_gain=230.0)
-        srv = L.ns_serve_create(ct.byref(cfg))
-        if not srv:
-            raise SystemExit(ns_amd.error())
-        gaps = np.random.default_rng(int(rate)).exponential(1.0 / rate, len(eps))
         log(f'serving: {len(eps)} episodes at {rate:g}/s')
-        t0 = time.perf_counter()
-        due = t0
-        for k, (nr, sr, mo, seed, tseed) in enumerate(eps):
-            due += gaps[k]
-            delay = due - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            if L.ns_serve_submit(srv, len(nr), len(sr), nr.ctypes.data_as(f), sr.ctypes.data_as(f),
-                                 mo.ctypes.data_as(ct.POINTER(ct.c_int)), seed, tseed) < 0:
-                raise SystemExit(ns_amd.error())
-        if L.ns_serve_drain(srv):
-            raise SystemExit(ns_amd.error())
-        wall = time.perf_counter() - t0
-        res = (ns_amd.ServeResult * len(eps))()
-        n = L.ns_serve_poll(srv, res, len(eps))
-        stats = ns_amd.ServeStats()
-        L.ns_serve_stats_get(srv, ct.byref(stats))
-        L.ns_serve_destroy(srv)
-        lat = np.array([r.finish_s - r.submit_s for r in res[:n]]) * 1e3
-        wait = np.array([r.admit_s - r.submit_s for r in res[:n]]) * 1e3
-        nu = total_neurons * args.episode_steps * 16
-        runs.append({'rate_per_s': rate, 'episodes': n, 'wall_s': wall,
-                     'episodes_per_s': n / wall, 'neuron_updates_per_s': nu / wall,
-                     'latency_ms': {'p50': float(np.percentile(lat, 50)),
-                                    'p99': float(np.percentile(lat, 99)),
-                                    'max': float(lat.max())},
-                     'admission_wait_ms': {'p50': float(np.percentile(wait, 50)),
-                                           'p99': float(np.percentile(wait, 99))},
-                     'launch_rounds': stats.launches, 'resident_cohorts_peak': stats.resident_peak,
-                     'device_busy_s': stats.busy_s})
+        runs.append(serve_run(eps, rate, args.episode_steps))
         log(f'  p50 {runs[-1]["latency_ms"]["p50"]:.1f} ms, p99 {runs[-1]["latency_ms"]["p99"]:.1f} ms, '
             f'{runs[-1]["neuron_updates_per_s"]:.3e} neuron-updates/s')
     top = runs[-1]
@@ -367,6 +413,9 @@ def main():
     ap.add_argument('--maze-requests', type=int, default=8192, help='c5: concurrent maze requests')
     ap.add_argument('--mazes', type=int, default=10, help='c5: mazes per request')
     ap.add_argument('--maze-horizon', type=int, default=100, help='c5: steps per maze at most')
+    ap.add_argument('--no-serving', action='store_true', help='leave out the C4 serving leg')
+    ap.add_argument('--serving-rate', type=float, default=1000.0, help='serving leg: arrivals per s')
+    ap.add_argument('--serving-episodes', type=int, default=200, help='serving leg: episodes')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only copy MIB MiB device to device (HBM counter calibration)')
     args = ap.parse_args()
@@ -425,9 +474,19 @@ def main():
             which = rng.integers(0, newborn.count, per_step)
             arr = (ct.c_void_p * per_step)(*[births.ptrs[j] for j in which])
             seeds = np.ascontiguousarray(rng.integers(1, 2**32 - 1, per_step), np.uint32)
-            plans.append((idx, arr, seeds))
+            plans.append((idx, arr, seeds, which))
         log(f'churn: {per_step} brains replaced per step; slot reserve built in '
             f'{time.perf_counter() - t1:.1f} s')
+    # The live pool, brain by brain, as births and deaths change it: neurons,
+    # synapses and plastic excitatory pairs of each brain's current genome.
+    live = {'n': w.neuron_counts.astype(np.int64), 's': w.edge_counts.astype(np.int64),
+            'p': W.plastic_pair_counts(w)}
+    born = None
+    if births is not None:
+        born = {'n': newborn.neuron_counts.astype(np.int64), 's': newborn.edge_counts.astype(np.int64),
+                'p': W.plastic_pair_counts(newborn)}
+    totals = {k: int(v.sum()) for k, v in live.items()}
+    launched = []            # live totals of every launched step, in order
     info = batch.info()
     log(f'packs {info["resident_packs"]} fallback brains {info["fallback_brains"]} '
         f'({info["fallback_neurons"]} neurons) device {info["device_bytes"] / 2**30:.2f} GiB')
@@ -442,12 +501,15 @@ def main():
     def births_for(k):
         if births is not None:
             import ctypes as ct
-            idx, arr, seeds = plans[k]
+            idx, arr, seeds, which = plans[k % len(plans)]
             rc = batch.lib.ns_gpu_replace(batch.ptr, len(idx), idx.ctypes.data_as(ct.POINTER(ct.c_int)),
                                           arr, seeds.ctypes.data_as(ct.POINTER(ct.c_uint32)),
                                           stream.cuda_stream)
             if rc:
                 raise RuntimeError(ns_amd.error())
+            for key in live:
+                totals[key] += int(born[key][which].sum() - live[key][idx].sum())
+                live[key][idx] = born[key][which]
 
     # A world step launches its ticks, then places the births and deaths the
     # next step starts from (host work overlapping this launch, device work
@@ -455,7 +517,11 @@ def main():
     def step(k):
         batch.step_device(inputs[k % patterns].data_ptr(), w.ticks, w.gain, out.data_ptr(),
                           stream=stream.cuda_stream)
+        launched.append(dict(totals))
         births_for(k + 1)
+
+    def live_sum(first, key):
+        return sum(t[key] for t in launched[first:])
 
     # Timed steps: each step one replay of a captured hipGraph (the output
     # clears and the whole 16-tick launch), replacements eager between them.
@@ -468,6 +534,7 @@ def main():
     i0 = batch.info()
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
+    timed0 = len(launched)
     t0 = time.perf_counter()
     for k in range(args.steps):
         evs[k].record(stream)
@@ -479,17 +546,25 @@ def main():
     step_ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)])
     log(f'{args.steps} steps in {wall:.3f} s; step p50 {np.percentile(step_ms, 50):.3f} ms')
 
-    nu = w.total_neurons * w.ticks * args.steps
-    su = w.total_synapses * w.ticks * args.steps
+    # neuron- and synapse-updates of the live pool at each timed launch
+    nu = live_sum(timed0, 'n') * w.ticks
+    su = live_sum(timed0, 's') * w.ticks
+    su_pairs = live_sum(timed0, 'p') * w.ticks
     ev = i1['events'] - i0['events']
     value = nu / wall
     res_launches = i1['resident_launches'] - i0['resident_launches']
     res_ms = (i1['resident_ms'] - i0['resident_ms']) / max(res_launches, 1)
-    res_neurons = w.total_neurons - i1['fallback_neurons']      # packs + big brains
-    res_synapses = w.total_synapses - i1['fallback_synapses']
+    # the resident launch's share per launch (brains on the per-tick path, LP
+    # and Prinz cells, excluded; plastic pairs are counted only where no brain
+    # takes that path, which holds for the c1/c2/c3 genomes)
+    assert i1['fallback_neurons'] == 0 or not plastic
+    res_neurons = nu / w.ticks / args.steps - i1['fallback_neurons']
+    res_synapses = su / w.ticks / args.steps - i1['fallback_synapses']
+    res_pairs = su_pairs / w.ticks / args.steps
     res_events = (i1['resident_events'] - i0['resident_events']) / max(res_launches, 1)
-    syn_bytes = SYNAPSE_UPDATE_BYTES_PLASTIC if plastic else SYNAPSE_UPDATE_BYTES_STATIC
-    alg_bytes = (NEURON_UPDATE_BYTES * res_neurons * w.ticks + syn_bytes * res_synapses * w.ticks
+    alg_bytes = (NEURON_UPDATE_BYTES * res_neurons * w.ticks
+                 + SYNAPSE_UPDATE_BYTES_PLASTIC * res_pairs * w.ticks
+                 + SYNAPSE_UPDATE_BYTES_STATIC * (res_synapses - res_pairs) * w.ticks
                  + EVENT_BYTES * res_events)
     achieved = alg_bytes / (res_ms * 1e-3) / 1e9 if res_ms > 0 else 0.0
     per_tick_share = (i1['per_tick_ms'] - i0['per_tick_ms']) / max(args.steps, 1)
@@ -501,14 +576,18 @@ def main():
     for k in range(2):
         step(k)
     torch.cuda.synchronize()
+    plain0 = len(launched)
     t1 = time.perf_counter()
     for k in range(args.plain_steps):
         step(k)
     torch.cuda.synchronize()
     plain_wall = time.perf_counter() - t1
-    plain_value = w.total_neurons * w.ticks * args.plain_steps / plain_wall
-    plain_alg = (NEURON_UPDATE_BYTES * w.total_neurons * w.ticks
-                 + syn_bytes * w.total_synapses * w.ticks + EVENT_BYTES * ev / args.steps)
+    plain_value = live_sum(plain0, 'n') * w.ticks / plain_wall
+    per = max(args.plain_steps, 1)
+    plain_alg = (NEURON_UPDATE_BYTES * live_sum(plain0, 'n') / per * w.ticks
+                 + SYNAPSE_UPDATE_BYTES_PLASTIC * live_sum(plain0, 'p') / per * w.ticks
+                 + SYNAPSE_UPDATE_BYTES_STATIC * (live_sum(plain0, 's') - live_sum(plain0, 'p')) / per
+                 * w.ticks + EVENT_BYTES * ev / args.steps)
 
     def leg(steps, graph):
         # graph: each step one replay of a captured hipGraph (ns_gpu_set_graph);
@@ -517,26 +596,39 @@ def main():
         for k in range(2):
             step(k)
         torch.cuda.synchronize()
+        first = len(launched)
         t = time.perf_counter()
         for k in range(steps):
             step(k)
         torch.cuda.synchronize()
         t = time.perf_counter() - t
         batch.set_graph(False)
-        return t
+        return t, live_sum(first, 'n') * w.ticks / t
 
-    plain_graph_wall = leg(args.plain_steps, True)
-    plain_graph_value = w.total_neurons * w.ticks * args.plain_steps / plain_graph_wall
+    plain_graph_wall, plain_graph_value = leg(args.plain_steps, True)
     batch.set_form(ns_amd.FORM_RESIDENT)
-    other_wall = leg(args.steps, args.eager)        # the resident form in the other launch mode
-    other_value = w.total_neurons * w.ticks * args.steps / other_wall
+    # the resident form in the other launch mode
+    other_wall, other_value = leg(args.steps, args.eager)
     log(f'resident {value:.3e} ({"eager" if args.eager else "graph"}; '
         f'{"graph" if args.eager else "eager"} {other_value:.3e}) vs plain per-tick '
         f'{plain_value:.3e} (graph replays {plain_graph_value:.3e}) neuron-updates/s')
 
+    serving = None
+    if not args.no_serving:
+        # p50/p99 batched-evaluation latency: a short C4 run through the
+        # continuous-batching loop (its own device pool; the world-step pool
+        # is released first)
+        batch.close()
+        log(f'serving leg: {args.serving_episodes} C4 episodes at {args.serving_rate:g}/s')
+        eps = serving_genomes(args.serving_episodes, seed=17)
+        serving = serve_run(eps, args.serving_rate, args.episode_steps)
+        serving['desc'] = (f'C4 episodes (log-uniform 16..4096 neurons, 4-d linear task, '
+                           f'{args.episode_steps} steps x 16 ticks), Poisson arrivals')
+        log(f'  p50 {serving["latency_ms"]["p50"]:.1f} ms, p99 {serving["latency_ms"]["p99"]:.1f} ms')
+
     cpu = None
     if not args.no_cpu:
-        log('timing the CPU oracle on a bounded sample')
+        log('timing the CPU oracle on a bounded sample (one thread, then the host share)')
         cpu = cpu_baseline(w)
 
     tile = (i1['threads'], i1['neurons_per_thread'], i1['synapses_per_thread'], i1['min_waves'],
@@ -548,6 +640,15 @@ def main():
         kernel = 'k_resident'
         kernel_full = f'k_resident<{tile[0]}, {tile[1]}, {tile[2]}, {tile[3]}>'
     traffic = load_traffic(kernel, args.workload, tile_key) if tile_key else None
+    measured = traffic / (res_ms * 1e-3) / 1e9 if traffic and res_ms else None
+    # what bounds the launch, read from the measurement: the streaming
+    # roofline is HBM's, but a launch that moves far less than the HBM peak
+    # allows in its time is held by latency (barriers, dependent LDS and HBM
+    # round trips), not bandwidth
+    limiter = None
+    if measured:
+        limiter = ('hbm bandwidth' if measured >= 0.6 * HBM_PEAK_GBPS else
+                   f'latency: measured HBM traffic {measured / HBM_PEAK_GBPS:.0%} of peak')
     line = {
         'metric': 'neuron-updates/s + synaptic events/s on 1 GPU; p50/p99 batched-eval latency',
         'value': value,
@@ -587,8 +688,23 @@ def main():
         'roofline': {'bound': 'hbm', 'kernel': kernel_full, 'achieved': achieved,
                      'peak': HBM_PEAK_GBPS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBPS,
                      'traffic': traffic,
-                     'measured_GBps': traffic / (res_ms * 1e-3) / 1e9 if traffic and res_ms else None,
-                     'launch_ms': res_ms, 'algorithmic_bytes_per_launch': alg_bytes},
+                     'measured_GBps': measured,
+                     'measured_frac': measured / HBM_PEAK_GBPS if measured else None,
+                     'limiter': limiter,
+                     'launch_ms': res_ms, 'algorithmic_bytes_per_launch': alg_bytes,
+                     'bytes_model': {'neuron_update': NEURON_UPDATE_BYTES,
+                                     'plastic_pair_synapse_update': SYNAPSE_UPDATE_BYTES_PLASTIC,
+                                     'other_synapse_update': SYNAPSE_UPDATE_BYTES_STATIC,
+                                     'event': EVENT_BYTES,
+                                     'per_launch': {'neuron_updates': res_neurons * w.ticks,
+                                                    'plastic_pair_synapse_updates': res_pairs * w.ticks,
+                                                    'other_synapse_updates':
+                                                        (res_synapses - res_pairs) * w.ticks,
+                                                    'events': res_events}}},
+        'live_pool': {'neurons_per_step': nu / w.ticks / args.steps,
+                      'synapses_per_step': su / w.ticks / args.steps,
+                      'plastic_pairs_per_step': su_pairs / w.ticks / args.steps},
+        'serving_c4': serving,
         'cpu_baseline': cpu,
     }
     if births is not None:
@@ -599,7 +715,8 @@ def main():
                          'slots': i1['slots'], 'free_slots': i1['free_slots'],
                          'slot_neurons': i1['slot_neurons'], 'slot_synapses': i1['slot_synapses']}
     print(json.dumps(line), flush=True)
-    batch.close()
+    if args.no_serving:
+        batch.close()
     if births is not None:
         births.close()
 

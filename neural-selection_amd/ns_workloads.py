@@ -212,6 +212,24 @@ def small_mix(seed=11, brains=8, calls=40, plasticity=True, max_delay=8, types=(
                      0.4, 230.0, 16, calls, active_p)
 
 
+def plastic_pair_counts(w):
+    """Per brain, the synapses that run plasticity every tick on the gym rules:
+    valid (synapse and both cells active) excitatory pairs (pre and post of an
+    excitatory type 0/4/5, neither inhibitory, pre not a modulator) with a
+    plasticity profile (gym_adapter/core.cuh:60-66, physiology.cuh:459-470).
+    The others cost a static synapse-update."""
+    n0 = np.concatenate([[0], np.cumsum(w.neuron_counts, dtype=np.int64)[:-1]])
+    brain = np.repeat(np.arange(w.count), w.edge_counts)
+    syn = w.synapses
+    pre = syn[:, 0].astype(np.int64) + n0[brain]
+    post = syn[:, 1].astype(np.int64) + n0[brain]
+    nr = w.neurons
+    active = nr[:, 2] > 0
+    exc = np.isin(nr[:, 0], (0, 4, 5)) & (nr[:, 3] == 0)
+    pair = (syn[:, 3] > 0) & active[pre] & active[post] & exc[pre] & (nr[pre, 4] == 0) & exc[post]
+    return np.bincount(brain, weights=pair & (syn[:, 9] > 0), minlength=w.count).astype(np.int64)
+
+
 def maze_genome(rng, neurons, fanout=4.0):
     """One maze-task genome: 9 sensor neurons on channels 0-8 (walls, goal
     direction, reward), the reward sensor a dopamine modulator, 4 motor
