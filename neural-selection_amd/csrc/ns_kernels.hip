@@ -800,6 +800,16 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
 // (neuron-parallel) sums the inputs of marked neurons in ascending synapse
 // order and integrates every neuron.  A neuron with no arrival this tick has
 // only +0 releases, so its sums are +0 without reading them.
+//
+// Quiet plasticity steps are deferred, exactly.  A pair synapse whose step for
+// tick t-1 has no input (no arrival, no spike of its post, no modulator at
+// it) only decays its traces, ages and bounds its weight, and nothing reads
+// that state before its next step with an input, its next arrival (whose
+// release reads the weight) or the write-back.  So such steps are counted in
+// the synapse's LDS word and replayed there, in tick order, by the same
+// operations: the reference's quiet-interval shortcut (physiology.cuh:559-586,
+// main.cu:1049-1106) without its closed form, so bit for bit the per-tick
+// result.
 // ---------------------------------------------------------------------------
 template <bool PLASTIC>
 __host__ __device__ constexpr int lds_synapse_bytes() {
@@ -811,12 +821,61 @@ __host__ __device__ constexpr int lds_synapse_bytes() {
 constexpr int kPpPost = 11, kPpBrain = 22;
 constexpr uint32_t kPpMask = 0x7ffu;
 
-// Arrival code of a synapse in a tick (ai_l): no arrival; an arrival whose
-// release is its weight (no short-term plasticity); k + 1 for the k-th queued
-// short-term arrival (release in crel_l[k]); or a queued arrival past the
-// kRelCap entries, whose release its post neuron's thread computes.
-constexpr uint16_t kArrNone = 0, kArrDirect = 0xffff, kArrSpill = 0xfffe;
+// A synapse's 16-bit LDS word (ai_l): in bits 0-8 its arrival code for the
+// tick: no arrival; an arrival whose release is its weight (no short-term
+// plasticity); k + 1 for the k-th queued short-term arrival (release in
+// crel_l[k]); or a queued arrival past the kRelCap entries, whose release its
+// post neuron's thread computes.  In bits 9-15, how many of its plasticity
+// steps are deferred (see phase A).
+constexpr uint32_t kArrNone = 0, kArrDirect = 0x101, kArrSpill = 0x102, kArrMask = 0x1ff;
 constexpr int kRelCap = 256;
+constexpr int kPendShift = 9;
+constexpr uint32_t kPendMax = 0x7f;
+
+// k plasticity steps of a pair synapse for ticks with no arrival, no spike of
+// its post neuron and no modulator at it: plasticity_step with those inputs
+// (physiology.cuh:459-555) decays the two traces, advances the ages by one
+// and bounds the weight, each variable on its own.  So the steps are replayed
+// variable by variable: each trace by its k multiplications in order; an age
+// that is a whole number below 2^24 - 128 by one addition of k (every
+// intermediate sum is then a float exactly, so the k additions of 1 give the
+// same value), any other age by k additions; the weight by one bound, which
+// is idempotent.  Profiles 4 and 5 count their dopamine / structural age up
+// to a reset value instead of decaying a trace.
+__device__ inline float add_steps(float age, uint32_t k) {
+    if (age >= 0.0f && age <= 16777088.0f && age == __builtin_truncf(age)) return age + float(k);
+    for (uint32_t i = 0; i < k; ++i) age += 1.0f;
+    return age;
+}
+
+__device__ inline float count_steps(float x, uint32_t k, float reset_above) {
+    for (uint32_t i = 0; i < k && x >= 0.0f; ++i) {
+        x += 1.0f;
+        if (x > reset_above) x = -1.0f;
+    }
+    return x;
+}
+
+__device__ inline void replay_quiet_steps(PlastRegs &r, int profile, float w0, uint32_t k) {
+    if (!k) return;
+    r.pre_age = add_steps(r.pre_age, k);
+    if (profile == 5) {
+        r.x = count_steps(r.x, k, 5000.0f);
+        return;
+    }
+    if (profile == 4) {
+        r.x = count_steps(r.x, k, 20000.0f);
+    } else {
+        const float dx = profile == 3 ? kDecayEligLtp : kDecayLtp;
+        const float dy = profile == 3 ? kDecayEligLtd : kDecayLtd;
+        for (uint32_t i = 0; i < k; ++i) {
+            r.x *= dx;
+            r.y *= dy;
+        }
+        r.post_age = add_steps(r.post_age, k);
+    }
+    r.weight = clamp_range(r.weight, 0.0f, 2.0f * w0);
+}
 
 // A neuron's LDS modulator byte (mod_l) holds the tick's modulator bits
 // (kModBits); in bit 6, whether a synapse onto it arrived this tick (set in
@@ -940,7 +999,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             pp_l[q] = pp[u] | (uint32_t(b[u]) << kPpBrain);
             if (PLASTIC) w0_l[q] = w0[u];
             w_l[q] = w[u];
-            ai_l[q] = kArrNone;
+            ai_l[q] = uint16_t(kArrNone);
             if (PLASTIC) {
                 pa_l[q] = pa[u];
                 qa_l[q] = qa[u];
@@ -1016,21 +1075,37 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                     const uint32_t pp = pp_l[q];
                     const int pre = pp & kPpMask, post = (pp >> kPpPost) & kPpMask;
                     const int b = pp >> kPpBrain;
-                    if (PLASTIC) {
-                        const int profile = (f >> SF_PLAST_SHIFT) & 7;
-                        if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
-                            PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
-                            plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q],
-                                            ai_l[q] != kArrNone, hist_l[post] & 1,
-                                            gate_depol(mod_l[post]), mod_l[post] & kModBits, etab);
-                            w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
-                            pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
-                        }
-                    }
                     const int64_t tick = btick_l[b] + t;
                     const int d = f & SF_DELAY;
                     const bool arr = tick >= d && ((hist_l[pre] >> (d - 1)) & 1u);
-                    ai_l[q] = arr ? kArrDirect : kArrNone;   // queued STP arrivals: below
+                    uint32_t pend = 0;
+                    if (PLASTIC) {
+                        const int profile = (f >> SF_PLAST_SHIFT) & 7;
+                        const uint32_t word = ai_l[q];
+                        pend = word >> kPendShift;
+                        if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
+                            // the step of tick t-1, deferred when it has no input
+                            // and no arrival at t reads the weight next
+                            const unsigned pm = mod_l[post];
+                            const unsigned modrec = (f >> SF_MODREC_SHIFT) & 7;
+                            const bool arrived = (word & kArrMask) != kArrNone;
+                            const bool spiked = hist_l[post] & 1;
+                            if (arrived || spiked || (pm & kModBits & modrec) || arr || pend == kPendMax) {
+                                PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
+                                const float w0 = w0_l[q];
+                                replay_quiet_steps(r, profile, w0, pend);
+                                plasticity_step(r, profile, modrec, w0, arrived, spiked, gate_depol(pm),
+                                                pm & kModBits, etab);
+                                w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
+                                pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
+                                pend = 0;
+                            } else {
+                                ++pend;
+                            }
+                        }
+                    }
+                    // queued STP arrivals take their code below
+                    ai_l[q] = uint16_t((pend << kPendShift) | (arr ? kArrDirect : kArrNone));
                     if (arr) mod_l[post] |= kArrivalMark;
                     my_events += arr;
                     arrive_stp = arr && (route & R_STP);
@@ -1044,11 +1119,12 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                 qb = __shfl(qb, 0);
                 if (arrive_stp) {
                     const int pos = qb + __popcll(m & ((uint64_t(1) << lane) - 1));
+                    const uint32_t keep = ai_l[q] & ~kArrMask;   // deferred steps
                     if (pos < kRelCap) {
                         cq_l[pos] = uint16_t(q);
-                        ai_l[q] = uint16_t(pos + 1);
+                        ai_l[q] = uint16_t(keep | uint32_t(pos + 1));
                     } else {
-                        ai_l[q] = kArrSpill;   // released by its post neuron's thread in phase B
+                        ai_l[q] = uint16_t(keep | kArrSpill);   // released by its post neuron's thread in phase B
                     }
                 }
             }
@@ -1083,7 +1159,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             for (int q = in_l[n]; q < q1; ++q) {
                 // this tick's release: +0 without an arrival, the weight
                 // without short-term plasticity, else the queued release
-                const uint32_t a = ai_l[q];
+                const uint32_t a = ai_l[q] & kArrMask;
                 float r = a == kArrNone ? 0.0f : a == kArrDirect ? w_l[q] : 0.0f;
                 if (a != kArrNone && a != kArrDirect)
                     r = a == kArrSpill ? release_at(p, s0 + q, fl_l[q], w_l[q], tick, etab)
@@ -1127,8 +1203,11 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const uint32_t pp = pp_l[q];
             if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[pp >> kPpBrain]) {
                 const int post = (pp >> kPpPost) & kPpMask;
+                const uint32_t word = ai_l[q];
                 PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
-                plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0_l[q], ai_l[q] != kArrNone,
+                const float w0 = w0_l[q];
+                replay_quiet_steps(r, profile, w0, word >> kPendShift);
+                plasticity_step(r, profile, (f >> SF_MODREC_SHIFT) & 7, w0, (word & kArrMask) != kArrNone,
                                 hist_l[post] & 1, gate_depol(mod_l[post]), mod_l[post] & kModBits,
                                 etab);
                 w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y; pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
@@ -1162,7 +1241,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
             store_plast(p, gq, (fl_l[q] >> SF_PLAST_SHIFT) & 7, r);
         }
-        p.arrived[gq] = uint8_t(ai_l[q] != kArrNone);
+        p.arrived[gq] = uint8_t((ai_l[q] & kArrMask) != kArrNone);
     }
     for (int m = tid; m < pk.nm; m += WG) {
         const int b = m / p.outputs;

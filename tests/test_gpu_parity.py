@@ -123,3 +123,28 @@ def test_static_packs_fill_the_lds(tiles, capacity):
     info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=8, check_every=4, tiles=tiles)
     assert info['lds_synapses'] == capacity
     assert info['resident_packs'] == 2 and info['big_brains'] == 0
+
+
+@pytest.mark.parametrize('tiles', [(1024, 2, 0, 1, 4224), (1024, 1, 0, 1, 4448)])
+def test_long_quiet_intervals_replay_exactly(tiles):
+    """Calls of 300 ticks with sparse input: pair synapses defer their quiet
+    plasticity steps for long stretches (up to the 127-step cap that forces a
+    replay) and replay them at arrivals, post spikes, modulator events and
+    write-back, bit-identical to the oracle's step of every tick."""
+    w = W.small_mix(seed=57, brains=10, calls=4, types=(0, 1, 4, 5), lo=20, hi=90, fanout=3.0,
+                    modulator_p=0.08)
+    w.input_p = 0.15
+    batch = ns_amd.GpuBatch(workload=w)
+    try:
+        batch.set_tiles(*tiles)
+        batch.reset(w.seeds())
+        brains = oracle_brains(w)
+        for call, ticks in enumerate((300, 17, 300, 5)):
+            x = w.inputs(call)
+            out, status = batch.step(x, ticks, w.gain)
+            oo, orc = oracle_step_all(brains, x, ticks, w.gain)
+            assert np.array_equal(out, oo), f'call {call}: motor outputs differ'
+            assert_state_equal(batch.export(), oracle_export_all(brains), f'call {call}')
+        assert batch.info()['events'] > 0
+    finally:
+        batch.close()
