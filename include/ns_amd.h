@@ -568,6 +568,50 @@ int ns_pool_load_checkpoint(void *pool, const char *path,
                             ns_checkpoint_header *header, void *world,
                             size_t world_capacity, size_t *world_bytes);
 
+/* ------------------------------------------------------------------------ */
+/* 5. The world step around the neural ticks (src/main.cu)                   */
+/* ------------------------------------------------------------------------ */
+
+/* A world of food and animals whose brains are a world-rules batch
+ * (ns_gpu_create_bulk with world_rules = 1, 19 sensory channels, 5 motors):
+ * update_senses (main.cu:619-657) feeds the batch's input, its motor spike
+ * counts drive body_step (main.cu:1146-1249), then bites (main.cu:1251-1285),
+ * world homeostasis (main.cu:1287-1300) and regrowth (main.cu:1138-1144).
+ * torus = 1 wraps the width x height grid (the C2 world); 0 clamps as the
+ * reference's does.  Each food kind A/B/C covers `food_density` of the cells
+ * at the start (main.cu:458-462 with 0.28 / 3 there); an empty cell regrows
+ * with probability `regrowth` per step.  Where the reference resolves
+ * contention by atomics (one food item, the prey bitten first, the bite that
+ * kills) this step takes the animals in ascending index, so it is
+ * deterministic; its sines and cosines are computed in double and rounded
+ * once, not the reference's fast-math intrinsics.  Reproduction and the
+ * behavior observers are not part of it. */
+typedef struct ns_ecology_config {
+    int width, height, torus;
+    float food_density, regrowth;
+    uint64_t seed;
+    int instant_motors;                /* main.cu instant_motors: force = spikes / 16 */
+} ns_ecology_config;
+
+/* One animal's body (main.cu:75-84 Animal, the fields this step uses). */
+typedef struct ns_world_animal {
+    float x, y, heading, energy, health, fullness, pain;
+    float taste[4], muscle[3];
+    int32_t age, alive;
+} ns_world_animal;
+
+void *ns_ecology_create(void *batch, const ns_ecology_config *config);
+void ns_ecology_destroy(void *ecology);
+/* One world step: the 16 neural ticks of every living animal on its senses,
+ * then body, bites, homeostasis, regrowth and the next senses.  `step`
+ * numbers the regrowth draws (main.cu:1142).  Synchronous. */
+int ns_ecology_step(void *ecology, int64_t step);
+/* Animals (batch order), food stock and kind per cell (row-major), senses
+ * (animals x 19) and the last motor spike counts (animals x 5); any may be
+ * NULL. */
+int ns_ecology_export(void *ecology, ns_world_animal *animals, int32_t *food, uint8_t *kind,
+                      float *senses, float *motors);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,18 @@ class CheckpointHeader(ct.Structure):
                 ('lives_size', ct.c_uint64), ('output', ct.c_char * 512)]
 
 
+class EcologyConfig(ct.Structure):
+    _fields_ = [('width', ct.c_int), ('height', ct.c_int), ('torus', ct.c_int),
+                ('food_density', ct.c_float), ('regrowth', ct.c_float), ('seed', ct.c_uint64),
+                ('instant_motors', ct.c_int)]
+
+
+WORLD_ANIMAL = np.dtype([('x', '<f4'), ('y', '<f4'), ('heading', '<f4'), ('energy', '<f4'),
+                         ('health', '<f4'), ('fullness', '<f4'), ('pain', '<f4'),
+                         ('taste', '<f4', 4), ('muscle', '<f4', 3), ('age', '<i4'),
+                         ('alive', '<i4')])
+
+
 # Exported symbols of include/ns_amd.h with their ctypes signatures.
 SIGNATURES = {
     'ns_error': ([], ct.c_char_p),
@@ -142,6 +154,10 @@ SIGNATURES = {
     'ns_pool_save_checkpoint': ([_vp, ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t], ct.c_int),
     'ns_pool_load_checkpoint': ([_vp, ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t,
                                  ct.POINTER(ct.c_size_t)], ct.c_int),
+    'ns_ecology_create': ([_vp, ct.POINTER(EcologyConfig)], _vp),
+    'ns_ecology_destroy': ([_vp], None),
+    'ns_ecology_step': ([_vp, ct.c_int64], ct.c_int),
+    'ns_ecology_export': ([_vp, _vp, _i32, _u8, _f, _f], ct.c_int),
     'ns_maze_layout': ([ct.c_uint32, ct.c_uint32, ct.POINTER(ct.c_uint32), ct.POINTER(ct.c_uint32)],
                        None),
 }
@@ -385,6 +401,45 @@ class GpuBatch:
 
     def __exit__(self, *a):
         self.close()
+
+    def __del__(self):
+        self.close()
+
+
+class Ecology:
+    """The world step around a world-rules batch (ns_ecology_*)."""
+
+    def __init__(self, batch, width=256, height=256, torus=True, food_density=0.05,
+                 regrowth=1e-3, seed=1, instant_motors=False):
+        self.lib = lib()
+        self.batch = batch
+        self.count = batch.count
+        self.cells = width * height
+        cfg = EcologyConfig(width, height, int(torus), food_density, regrowth, seed,
+                            int(instant_motors))
+        self.ptr = self.lib.ns_ecology_create(batch.ptr, ct.byref(cfg))
+        if not self.ptr:
+            raise RuntimeError(error())
+
+    def step(self, step):
+        if self.lib.ns_ecology_step(self.ptr, step):
+            raise RuntimeError(error())
+
+    def export(self):
+        an = np.zeros(self.count, WORLD_ANIMAL)
+        food = np.zeros(self.cells, np.int32)
+        kind = np.zeros(self.cells, np.uint8)
+        senses = np.zeros((self.count, 19), np.float32)
+        motors = np.zeros((self.count, 5), np.float32)
+        if self.lib.ns_ecology_export(self.ptr, an.ctypes.data, _ptr(food, _i32), _ptr(kind, _u8),
+                                      _ptr(senses, _f), _ptr(motors, _f)):
+            raise RuntimeError(error())
+        return an, food, kind, senses, motors
+
+    def close(self):
+        if self.ptr:
+            self.lib.ns_ecology_destroy(self.ptr)
+            self.ptr = None
 
     def __del__(self):
         self.close()

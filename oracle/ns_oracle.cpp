@@ -26,6 +26,7 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <limits>
 
 namespace {
 
@@ -932,6 +933,254 @@ int ora_batch_step(void **brains, int count, const float *inputs, int channels, 
     for (auto &th : pool) th.join();
     return 0;
 }
+
+// ---- the world step around the neural ticks (main.cu:612-657, 1138-1300) --
+// A restatement of the C2 world: toroidal (or the reference's clamped) grid,
+// food kinds at a density each, regrowth; the reference's atomics replaced
+// by ascending animal order: the lowest-index eater takes a cell's item,
+// cells list their living animals in ascending index, attackers bite in
+// index order within their cell.  Sines and cosines: Cody-Waite reduction by
+// pi/2 (fdlibm's three-part constant) and Taylor polynomials in double,
+// rounded once to float.
+}  // extern "C"
+namespace {
+
+void world_sincos(float angle, float *s, float *c) {
+    const double x = (double)angle;
+    const double k = std::rint(x * 0.63661977236758134308);
+    double r = x - k * 1.57079632673412561417e+00;
+    r = r - k * 6.07710050650619224932e-11;
+    r = r - k * 2.02226624879595063154e-21;
+    const double r2 = r * r;
+    const double sr = r + r * r2 * (-1.0 / 6 + r2 * (1.0 / 120 + r2 * (-1.0 / 5040 + r2 * (1.0 / 362880 +
+                      r2 * (-1.0 / 39916800 + r2 * (1.0 / 6227020800.0))))));
+    const double cr = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24 + r2 * (-1.0 / 720 + r2 * (1.0 / 40320 +
+                      r2 * (-1.0 / 3628800 + r2 * (1.0 / 479001600.0))))));
+    const long long q = (long long)k & 3;
+    *s = (float)(q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr);
+    *c = (float)(q == 0 ? cr : q == 1 ? -sr : q == 2 ? -cr : sr);
+}
+
+uint64_t eco_random64(uint64_t &s) {                      // main.cu:331-334
+    s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+    return s * 2685821657736338717ULL;
+}
+float eco_uniform(uint64_t &s) { return (float)((eco_random64(s) >> 40) * (1.0 / 16777216.0)); }
+uint64_t eco_seed_for(uint64_t seed, uint64_t i) {        // main.cu:341-346
+    uint64_t x = seed + 0x9e3779b97f4a7c15ULL * (i + 1);
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return (x ^ (x >> 31)) | 1ULL;
+}
+float eco_clamp(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+
+struct WorldAnimal {                                      // ns_world_animal
+    float x, y, heading, energy, health, fullness, pain, taste[4], muscle[3];
+    int32_t age, alive;
+};
+
+struct Ecology {
+    int P = 0, W = 0, H = 0, torus = 1, instant = 0;
+    float regrowth = 0;
+    uint64_t seed = 0;
+    std::vector<WorldAnimal> an;
+    std::vector<int> active, food, claim;
+    std::vector<uint8_t> kind, eat, bite;
+    std::vector<float> senses, motors;
+    std::vector<std::vector<int>> lists;                  // per cell, ascending
+
+    int cell(float x, float y) const {                    // main.cu:347-351 / the torus
+        if (torus) {
+            int ix = int(floorf(x)) % W, iy = int(floorf(y)) % H;
+            if (ix < 0) ix += W;
+            if (iy < 0) iy += H;
+            return iy * W + ix;
+        }
+        return int(floorf(eco_clamp(y, 0.0f, float(H) - 0.001f))) * W +
+               int(floorf(eco_clamp(x, 0.0f, float(W) - 0.001f)));
+    }
+    float place(float v, int extent) const {
+        if (!torus) return eco_clamp(v, 0.0f, float(extent) - 0.001f);
+        if (v < 0.0f) v += float(extent);
+        if (v >= float(extent)) v -= float(extent);
+        return v;
+    }
+    void index() {                                        // main.cu:612-617
+        for (auto &l : lists) l.clear();
+        for (int a = 0; a < P; ++a)
+            if (an[a].alive == 1) lists[cell(an[a].x, an[a].y)].push_back(a);
+    }
+    void sense(int a) {                                   // main.cu:619-657
+        const WorldAnimal &b = an[a];
+        float *out = &senses[size_t(a) * 19];
+        float rs[3], rc[3];
+        for (int d = 0; d < 3; ++d) world_sincos(b.heading + float(d - 1) * 0.7f, &rs[d], &rc[d]);
+        for (int s = 0; s < 6; ++s) {
+            const int d = s % 3;
+            float level = 0;
+            for (int range = 1; range <= 4; ++range) {
+                const int c = cell(b.x + rc[d] * float(range), b.y + rs[d] * float(range));
+                if (kind[c] == s / 3 && food[c] > 0) level += 0.25f;
+            }
+            out[s] = eco_clamp(level, 0, 1);
+        }
+        out[6] = eco_clamp((16.0f - b.energy) / 16.0f, 0, 1);
+        out[7] = eco_clamp(b.fullness / 8.0f, 0, 1);
+        out[8] = eco_clamp(b.pain / 3.0f, 0, 1);
+        for (int k = 0; k < 3; ++k) out[9 + k] = eco_clamp(b.taste[k], 0, 1);
+        for (int d = 0; d < 3; ++d) {
+            float level = 0;
+            for (int range = 1; range <= 4; ++range) {
+                const int c = cell(b.x + rc[d] * float(range), b.y + rs[d] * float(range));
+                for (int other : lists[c])
+                    if (other != a && an[other].alive == 1) level += 0.25f;
+            }
+            out[12 + d] = eco_clamp(level, 0, 1);
+        }
+        out[15] = eco_clamp(b.taste[3], 0, 1);
+        const int here = cell(b.x, b.y);
+        for (int k = 0; k < 3; ++k) out[16 + k] = food[here] > 0 && kind[here] == k ? 1.0f : 0.0f;
+    }
+};
+
+}  // namespace
+extern "C" {
+
+void *ora_ecology_create(int animals, int width, int height, int torus, float food_density,
+                         float regrowth, uint64_t seed, int instant, const int *active_neurons) {
+    auto *e = new Ecology;
+    e->P = animals; e->W = width; e->H = height; e->torus = torus; e->instant = instant;
+    e->regrowth = regrowth; e->seed = seed;
+    e->an.assign(animals, WorldAnimal{});
+    e->active.assign(active_neurons, active_neurons + animals);
+    for (int a = 0; a < animals; ++a) {                   // main.cu:530-545
+        uint64_t rng = eco_seed_for(seed ^ 0x5eedf00du, uint64_t(a));
+        WorldAnimal &b = e->an[a];
+        b.x = eco_uniform(rng) * (torus ? float(width) : float(width - 1));
+        b.y = eco_uniform(rng) * (torus ? float(height) : float(height - 1));
+        b.heading = eco_uniform(rng) * 2.0f * 3.14159265358979323846f;
+        b.energy = 14.0f;
+        b.health = 10.0f;
+        b.alive = 1;
+    }
+    const int cells = width * height;
+    e->food.assign(cells, 0);
+    e->kind.assign(cells, 0);
+    for (int c = 0; c < cells; ++c) {                     // main.cu:458-462
+        uint64_t rng = eco_seed_for(seed, uint64_t(c));
+        e->kind[c] = uint8_t(int(eco_uniform(rng) * 3));
+        e->food[c] = eco_uniform(rng) < 3.0f * food_density ? 1 : 0;
+    }
+    e->claim.assign(cells, 0);
+    e->lists.assign(cells, {});
+    e->eat.assign(animals, 0);
+    e->bite.assign(animals, 0);
+    e->senses.assign(size_t(animals) * 19, 0.0f);
+    e->motors.assign(size_t(animals) * 5, 0.0f);
+    e->index();
+    for (int a = 0; a < animals; ++a)
+        if (e->an[a].alive == 1) e->sense(a);
+    return e;
+}
+
+void ora_ecology_destroy(void *p) { delete static_cast<Ecology *>(p); }
+
+// One world step with brains[a] the animal's world-rules oracle brain.
+void ora_ecology_step(void *p, void **brains, int64_t step) {
+    Ecology &e = *static_cast<Ecology *>(p);
+    const int P = e.P;
+    for (int a = 0; a < P; ++a) {                         // the 16 neural ticks
+        float *m = &e.motors[size_t(a) * 5];
+        for (int k = 0; k < 5; ++k) m[k] = 0;
+        if (e.an[a].alive == 1)
+            step_brain(*static_cast<Brain *>(brains[a]), &e.senses[size_t(a) * 19], 16, 230.0f, m, nullptr);
+    }
+    std::fill(e.claim.begin(), e.claim.end(), std::numeric_limits<int>::max());
+    std::vector<int> food_cell(P, 0);
+    for (int a = 0; a < P; ++a) {                         // body_step, main.cu:1146-1213
+        WorldAnimal &b = e.an[a];
+        if (b.alive != 1) continue;
+        const float old_x = b.x, old_y = b.y;
+        const float *mo = &e.motors[size_t(a) * 5];
+        float force[3];
+        for (int m = 0; m < 3; ++m)
+            force[m] = e.instant ? mo[m] / 16.0f
+                                 : (b.muscle[m] = fminf(1.0f, b.muscle[m] * 0.92311635f + mo[m] * 0.30f));
+        const float thrust = force[0], left = force[1], right = force[2];
+        e.eat[a] = mo[3] > 0;
+        e.bite[a] = mo[4] > 0;
+        b.heading += (right - left) * 1.2f;
+        float s, c;
+        world_sincos(b.heading, &s, &c);
+        b.x = e.place(old_x + c * thrust * 1.5f, e.W);
+        b.y = e.place(old_y + s * thrust * 1.5f, e.H);
+        b.energy -= 0.003f + float(e.active[a]) * 0.00002f + thrust * 0.02f + (e.bite[a] ? 0.02f : 0.0f);
+        b.fullness = fmaxf(0, b.fullness - 0.05f);
+        b.pain *= 0.8f;
+        for (int k = 0; k < 4; ++k) b.taste[k] *= 0.7f;
+        food_cell[a] = e.cell(old_x, old_y);
+        if (e.eat[a]) e.claim[food_cell[a]] = std::min(e.claim[food_cell[a]], a);
+    }
+    for (int a = 0; a < P; ++a) {                         // feed, starvation, age, death
+        WorldAnimal &b = e.an[a];
+        if (b.alive != 1) continue;
+        const int fc = food_cell[a];
+        if (e.eat[a] && e.claim[fc] == a && e.food[fc] > 0) {
+            e.food[fc] -= 1;
+            const int kind = e.kind[fc];
+            b.taste[kind] = 1;
+            b.fullness += 1;
+            if (kind == 0) b.energy = fminf(30, b.energy + 4);
+            if (kind == 1) { b.energy = fminf(30, b.energy + 2); b.health -= 1.4f; b.pain += 1.4f; }
+            if (b.fullness > 8) { b.health -= 0.8f; b.pain += 0.8f; }
+        }
+        if (b.energy <= 0) { b.energy = 0; b.health -= 0.15f; b.pain += 0.15f; }
+        ++b.age;
+        if (b.health <= 0 || b.age >= 10000) b.alive = 0;
+    }
+    e.index();
+    for (const auto &l : e.lists)                         // bite_animals, main.cu:1251-1275
+        for (int a : l) {
+            if (e.an[a].alive != 1 || !e.bite[a]) continue;
+            for (int prey : l) {
+                if (prey == a || e.an[prey].alive != 1) continue;
+                const float previous = e.an[prey].health;
+                e.an[prey].health = previous - 2.0f;
+                e.an[prey].pain += 2.0f;
+                if (previous <= 2.0f) {
+                    e.an[prey].alive = -2;
+                    e.an[a].energy += fmaxf(0.0f, e.an[prey].energy) * 0.45f;
+                    e.an[a].taste[3] = 1.0f;
+                    e.an[a].fullness += 1.0f;
+                }
+                break;
+            }
+        }
+    for (int a = 0; a < P; ++a) {                         // finish_predation; homeostasis_step
+        if (e.an[a].alive == -2) e.an[a].alive = 0;
+        if (e.an[a].alive == 1) ora_world_homeostasis(brains[a], e.an[a].age);
+    }
+    for (int c = 0; c < e.W * e.H; ++c) {                 // regrow, main.cu:1138-1144
+        if (e.food[c] != 0) continue;
+        uint64_t rng = eco_seed_for(e.seed + uint64_t(step), uint64_t(c));
+        if (eco_uniform(rng) < e.regrowth) e.food[c] = 1;
+    }
+    e.index();
+    for (int a = 0; a < P; ++a)
+        if (e.an[a].alive == 1) e.sense(a);
+}
+
+void ora_ecology_export(void *p, void *animals, int32_t *food, uint8_t *kind, float *senses, float *motors) {
+    const Ecology &e = *static_cast<Ecology *>(p);
+    if (animals) std::memcpy(animals, e.an.data(), e.an.size() * sizeof(WorldAnimal));
+    if (food) std::memcpy(food, e.food.data(), e.food.size() * sizeof(int32_t));
+    if (kind) std::memcpy(kind, e.kind.data(), e.kind.size());
+    if (senses) std::memcpy(senses, e.senses.data(), e.senses.size() * sizeof(float));
+    if (motors) std::memcpy(motors, e.motors.data(), e.motors.size() * sizeof(float));
+}
+
+float ora_world_sin(float angle) { float s, c; world_sincos(angle, &s, &c); return s; }
+float ora_world_cos(float angle) { float s, c; world_sincos(angle, &s, &c); return c; }
 
 // ---- function-level probes (pinned against oracle/_ref's ref_fn_*) --------
 void ora_fn_profile(int type, ns_neuron_gene *out) { *out = profile_for(type); }

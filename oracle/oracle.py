@@ -83,6 +83,15 @@ def oracle_lib():
         lib.ora_create_world.restype = _vp
         lib.ora_destroy.argtypes = [_vp]
         lib.ora_genes.argtypes = [_vp, _vp, _vp]
+        lib.ora_ecology_create.argtypes = [ct.c_int] * 4 + [ct.c_float, ct.c_float, ct.c_uint64,
+                                                             ct.c_int, _i32]
+        lib.ora_ecology_create.restype = _vp
+        lib.ora_ecology_destroy.argtypes = [_vp]
+        lib.ora_ecology_step.argtypes = [_vp, ct.POINTER(_vp), ct.c_int64]
+        lib.ora_ecology_export.argtypes = [_vp, _vp, _i32, _u8, _f, _f]
+        for name in ('ora_world_sin', 'ora_world_cos'):
+            getattr(lib, name).argtypes = [ct.c_float]
+            getattr(lib, name).restype = ct.c_float
         lib.ora_original_indices.argtypes = [_vp, _u32, _u32]
         lib.ora_reset.argtypes = [_vp, ct.c_uint32]
         lib.ora_step.argtypes = [_vp, _f, ct.c_int, ct.c_float, _f, _u8]
@@ -247,6 +256,69 @@ class RefBrain:
     def close(self):
         if self.ptr:
             self.lib.ns_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+WORLD_ANIMAL = np.dtype([('x', '<f4'), ('y', '<f4'), ('heading', '<f4'), ('energy', '<f4'),
+                         ('health', '<f4'), ('fullness', '<f4'), ('pain', '<f4'),
+                         ('taste', '<f4', 4), ('muscle', '<f4', 3), ('age', '<i4'),
+                         ('alive', '<i4')])
+
+
+def world_brain(neurons, synapses, motors, seed, max_delay=8, channels=19):
+    """An oracle brain under the world rules (main.cu:710-746) from a gym
+    genome: the gene keeps the row's sensory channel, motor j sits on port
+    19 + j; reset with `seed` as the gym path seeds."""
+    lib = oracle_lib()
+    gym = OracleBrain(neurons, synapses, motors, channels, max_delay=max_delay)
+    ng, sg = gym.genes()
+    gym.close()
+    ng['sensory_channel'] = np.asarray(neurons, np.float32)[:, 1].astype(np.int8)
+    ports = np.full(len(ng), -1, np.int32)
+    for j, m in enumerate(motors):
+        ports[m] = 19 + j
+    h = lib.ora_create_world(len(ng), len(sg), ng.ctypes.data, sg.ctypes.data, ptr(ports, _i32),
+                             max_delay)
+    lib.ora_reset(h, int(seed) & 0xffffffff)
+    return h
+
+
+class OracleEcology:
+    """The oracle's restatement of the world step, with world-rule brains."""
+
+    def __init__(self, w, width, height, torus, food_density, regrowth, seed, instant=False):
+        lib = oracle_lib()
+        self.lib = lib
+        seeds = w.seeds()
+        self.brains = [world_brain(*w.brain(i), seeds[i], max(8, w.max_delay)) for i in range(w.count)]
+        active = np.ascontiguousarray([int((w.brain(i)[0][:, 2] > 0).sum()) for i in range(w.count)],
+                                      np.int32)
+        self.count, self.cells = w.count, width * height
+        self.ptr = lib.ora_ecology_create(w.count, width, height, int(torus), food_density, regrowth,
+                                          seed, int(instant), ptr(active, _i32))
+        self.arr = (_vp * w.count)(*self.brains)
+
+    def step(self, step):
+        self.lib.ora_ecology_step(self.ptr, self.arr, step)
+
+    def export(self):
+        an = np.zeros(self.count, WORLD_ANIMAL)
+        food = np.zeros(self.cells, np.int32)
+        kind = np.zeros(self.cells, np.uint8)
+        senses = np.zeros((self.count, 19), np.float32)
+        motors = np.zeros((self.count, 5), np.float32)
+        self.lib.ora_ecology_export(self.ptr, an.ctypes.data, ptr(food, _i32), ptr(kind, _u8),
+                                    ptr(senses, _f), ptr(motors, _f))
+        return an, food, kind, senses, motors
+
+    def close(self):
+        if self.ptr:
+            self.lib.ora_ecology_destroy(self.ptr)
+            for h in self.brains:
+                self.lib.ora_destroy(h)
             self.ptr = None
 
     def __del__(self):
