@@ -102,6 +102,21 @@ typedef struct ns_checkpoint_neuron {
     uint8_t pad0_[7];
 } ns_checkpoint_neuron;
 
+/* CheckpointNeuron of a world build with delays of 1-16 ms: main.cu:46
+ * HISTORY = D + 1 = 17 ring slots (352 B).  The same fields at the same
+ * offsets up to the ring.  A layout version of its own: HISTORY and the
+ * record size enter checkpoint_layout (checkpoint_impl.cuh:21-32), so its
+ * files carry ns_checkpoint_layout_ex(17). */
+typedef struct ns_checkpoint_neuron17 {
+    int32_t slot, owner, next, port;
+    ns_neuron_gene gene;
+    ns_neuron_state state;
+    float pending_exc[4], pending_inh;
+    uint64_t mod_event_tick[6];
+    uint8_t spike_history[17];
+    uint8_t pad0_[7];
+} ns_checkpoint_neuron17;
+
 /* src/main.cu:173-177 CheckpointSynapse (136 B). */
 typedef struct ns_checkpoint_synapse {
     int32_t slot, owner, next;
@@ -358,9 +373,19 @@ int ns_device_math_digest(int fn, uint64_t lo, uint64_t hi, int64_t chunk,
 /* ------------------------------------------------------------------------ */
 
 /* Pool over checkpoint records.  Capacities bound the record slots
- * (main.cu:33-35: 32768 animals, 4194304 neurons, 8388608 synapses). */
+ * (main.cu:33-35: 32768 animals, 4194304 neurons, 8388608 synapses).  Its
+ * records have the reference's 9-slot rings (delays 1-8). */
 void *ns_pool_create(int animal_capacity, int neuron_capacity,
                      int synapse_capacity);
+/* The same with a ring of `history` slots: 9 (as ns_pool_create) or 17, the
+ * layout of a build with delays of 1-16 ms (main.cu:46 HISTORY = D + 1),
+ * whose records are ns_checkpoint_neuron17 and go through the *_records17
+ * calls below; the 9-slot calls refuse such a pool and the 17-slot calls a
+ * 9-slot pool.  An inactive neuron keeps its ring slot for `history` ticks
+ * (main.cu:718). */
+void *ns_pool_create_ex(int animal_capacity, int neuron_capacity,
+                        int synapse_capacity, int history);
+int ns_pool_history(void *pool);       /* 9 or 17; -1 on a bad handle */
 void ns_pool_destroy(void *pool);
 
 /* Load occupied neuron/synapse records (ascending slot order, as written by
@@ -417,6 +442,25 @@ int ns_pool_replace_records(void *pool, int animal_count, const int *animals,
                             int neuron_count,
                             const ns_checkpoint_synapse *synapses,
                             int synapse_count);
+
+/* The record calls of a 17-slot pool (ns_pool_create_ex(..., 17)): the same
+ * checks and results as ns_pool_import_records, ns_pool_export_records and
+ * ns_pool_replace_records, with ns_checkpoint_neuron17 records and synapse
+ * delays of 1-16. */
+int ns_pool_import_records17(void *pool, const ns_checkpoint_neuron17 *neurons,
+                             int neuron_count,
+                             const ns_checkpoint_synapse *synapses,
+                             int synapse_count);
+int ns_pool_export_records17(void *pool, ns_checkpoint_neuron17 *neurons,
+                             int neuron_capacity,
+                             ns_checkpoint_synapse *synapses,
+                             int synapse_capacity, int *neuron_count,
+                             int *synapse_count);
+int ns_pool_replace_records17(void *pool, int animal_count, const int *animals,
+                              const ns_checkpoint_neuron17 *neurons,
+                              int neuron_count,
+                              const ns_checkpoint_synapse *synapses,
+                              int synapse_count);
 
 /* Slot reserve for ns_pool_replace_records, as ns_gpu_reserve: rebuilds the
  * pool (state kept) with capacities rounded up a ladder of ratio 1 + slack and
@@ -528,6 +572,10 @@ typedef struct ns_checkpoint_header {
  * capacities and struct sizes (32768 animals, 4194304 neurons, 8388608
  * synapses, 19 senses, 5 motors, a 128 x 128 world, ...). */
 uint64_t ns_checkpoint_layout(void);
+/* The layout hash of a build with `history` ring slots (9: the value of
+ * ns_checkpoint_layout; 17: HISTORY 17 and the 352-byte neuron record);
+ * 0 for any other history. */
+uint64_t ns_checkpoint_layout_ex(int history);
 
 /* Read a whole checkpoint file with the reference's checks, in its order
  * (main.cu:1986-2000 header, checkpoint_impl.cuh:201-366 state, 368-383
@@ -538,12 +586,23 @@ uint64_t ns_checkpoint_layout(void);
  * the records, and the world sections as one block in file order: observer
  * totals and histories, animals, food, kind, senses, motors, counters,
  * traces, then the plasticity-cohort extension when flags & 64.  NULL on
- * failure. */
+ * failure.  Both layout versions are read (ns_checkpoint_layout_ex(9) and
+ * (17)); ns_checkpoint_history tells which, and the records come out of
+ * ns_checkpoint_get (9 slots) or ns_checkpoint_get17 (17 slots), the other
+ * call failing.  The whole file is held in host memory: at the world's
+ * capacities up to 1.44 GB of neuron records (1.48 GB with 17 slots) and
+ * 1.14 GB of synapse records; ns_pool_load_checkpoint holds them a second
+ * time while it groups them per animal. */
 void *ns_checkpoint_open(const char *path);
+int ns_checkpoint_history(void *checkpoint);   /* 9 or 17; -1 on a bad handle */
 int ns_checkpoint_get(void *checkpoint, ns_checkpoint_header *header,
                       const void **world, size_t *world_bytes,
                       const ns_checkpoint_neuron **neurons,
                       const ns_checkpoint_synapse **synapses);
+int ns_checkpoint_get17(void *checkpoint, ns_checkpoint_header *header,
+                        const void **world, size_t *world_bytes,
+                        const ns_checkpoint_neuron17 **neurons,
+                        const ns_checkpoint_synapse **synapses);
 void ns_checkpoint_close(void *checkpoint);
 
 /* checkpoint_impl.cuh:135-199, 385-413 checkpoint_atomic_save: gzip level 1
@@ -551,14 +610,21 @@ void ns_checkpoint_close(void *checkpoint);
  * header's record counts are set from the arguments; the world block is laid
  * out as ns_checkpoint_open returns it; records must be in ascending slot
  * order and owned by living animals whose counts match.  Nothing is written
- * when a check fails. */
+ * when a check fails, and a write that fails removes path.tmp.  The header's
+ * layout must be ns_checkpoint_layout() (ns_checkpoint_layout_ex(17) for
+ * ns_checkpoint_save17). */
 int ns_checkpoint_save(const char *path, const ns_checkpoint_header *header,
                        const void *world, size_t world_bytes,
                        const ns_checkpoint_neuron *neurons, int neuron_count,
                        const ns_checkpoint_synapse *synapses, int synapse_count);
+int ns_checkpoint_save17(const char *path, const ns_checkpoint_header *header,
+                         const void *world, size_t world_bytes,
+                         const ns_checkpoint_neuron17 *neurons, int neuron_count,
+                         const ns_checkpoint_synapse *synapses, int synapse_count);
 
 /* The same with a world pool's records (ns_pool_export_records /
- * ns_pool_import_records).  Loading checks the whole file before the pool
+ * ns_pool_import_records, or their 17-slot forms: the file's layout version
+ * must be the pool's).  Loading checks the whole file before the pool
  * changes, then loads the pool; world may be NULL (the world block is not
  * copied, *world_bytes still gives its size). */
 int ns_pool_save_checkpoint(void *pool, const char *path,

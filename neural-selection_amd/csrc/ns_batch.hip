@@ -206,7 +206,11 @@ struct SlotImage {
         const HostBrain &b = *spec->brain;
         const int nc = int(b.genes.size()), sc = int(b.edges.size());
         bout0 = spec->output_base;
-        for (int j = 0; j < outs; ++j) motor[j] = b.motors[j] >= 0 ? int32_t(n0 + b.motors[j]) : -1;
+        // main.cu:718,742-743: under the world rules an inactive neuron counts
+        // no spikes (its register bit 0 is a kept ring slot, not a spike)
+        for (int j = 0; j < outs; ++j)
+            motor[j] = b.motors[j] >= 0 && (!spec->world || b.genes[b.motors[j]].active)
+                           ? int32_t(n0 + b.motors[j]) : -1;
         for (int i = 0; i < nc; ++i) {
             const ns_neuron_gene &g = b.genes[i];
             const float f[NG_FIELDS] = {g.capacitance_pf, g.leak_ns, g.rest_mv, g.threshold_mv,
@@ -219,7 +223,7 @@ struct SlotImage {
             for (int k = 0; k < 8; ++k) nx[size_t(NX_PRINZ + k) * cn + i] = g.prinz_g[k];
             nx[size_t(NX_AMBIENT) * cn + i] = g.prinz_ambient_leak_ns;
             nflags[i] = uint32_t(g.type) | (g.active ? NF_ACTIVE : 0) |
-                        (b.receptor[i] ? NF_RECEPTOR : 0) | (spec->world ? NF_WORLD : 0);
+                        (b.receptor[i] ? NF_RECEPTOR : 0) | (spec->world ? world_flags(b.max_delay) : 0);
             ninput[i] = b.sensory[i] >= 0 ? spec->input_base + b.sensory[i] : -1;
             norig[i] = b.orig_n.empty() ? uint32_t(i) : b.orig_n[i];
         }
@@ -718,7 +722,7 @@ void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &mov
             if (i < nc) {
                 const ns_neuron_gene &g = b->genes[i];
                 h_nflags[n0 + i] = uint32_t(g.type) | (g.active ? NF_ACTIVE : 0) |
-                                   (b->receptor[i] ? NF_RECEPTOR : 0) | (spec->world ? NF_WORLD : 0);
+                                   (b->receptor[i] ? NF_RECEPTOR : 0) | (spec->world ? world_flags(b->max_delay) : 0);
             } else {
                 h_nflags[n0 + i] = 0;   // inactive tail
             }

@@ -10,6 +10,7 @@
 #include <zlib.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <filesystem>
 #include <memory>
@@ -24,7 +25,7 @@ namespace {
 // The world build's constants (src/main.cu:33-47, physiology.cuh:17) and
 // struct sizes (main.cu:66-104, 164-177, 1845-1849).
 constexpr uint64_t kP = 32768, kNeuronCap = 4194304, kSynapseCap = 8388608;
-constexpr uint64_t kS = 19, kM = 5, kW = 128, kH = 128, kDendrites = 4, kHistory = 9, kTicks = 16;
+constexpr uint64_t kS = 19, kM = 5, kW = 128, kH = 128, kDendrites = 4, kTicks = 16;
 constexpr uint64_t kAnimalBytes = 472;          // Animal, Behavior 276 B at offset 196
 constexpr uint64_t kTotalsBytes = 528;          // BehaviorTotals
 constexpr uint64_t kLifeBytes = 504;            // LifeRecord
@@ -37,6 +38,15 @@ constexpr uint64_t kCohortTag = 0x4e53504c41535431ull;   // "NSPLAST1"
 static_assert(sizeof(ns_checkpoint_header) == 608, "CheckpointHeader layout");
 static_assert(sizeof(ns_checkpoint_neuron) == 344 && sizeof(ns_checkpoint_synapse) == 136,
               "checkpoint record layout");
+static_assert(sizeof(ns_checkpoint_neuron17) == 352 &&
+                  offsetof(ns_checkpoint_neuron17, spike_history) ==
+                      offsetof(ns_checkpoint_neuron, spike_history),
+              "17-slot checkpoint record layout");
+
+// The ring length of a neuron record type (main.cu:46 HISTORY).
+template <class R> constexpr uint64_t history_of();
+template <> constexpr uint64_t history_of<ns_checkpoint_neuron>() { return 9; }
+template <> constexpr uint64_t history_of<ns_checkpoint_neuron17>() { return 17; }
 
 // Bytes of the fixed world arrays after the observer section, in file order
 // (checkpoint_impl.cuh:173-190): animals, food, kind, senses, motors, five
@@ -45,11 +55,13 @@ constexpr uint64_t kArrayBytes = kP * kAnimalBytes + 2 * kW * kH * 4 + kP * kS *
                                  5 * 4 + kM * 4 + 4 + 6 * kM * 4;
 constexpr uint64_t kExtensionBytes = 4 * 8 + kP * kCohortBytes;
 
-uint64_t layout_hash() {
-    const uint64_t values[] = {kP, kNeuronCap, kSynapseCap, kS, kM, kW, kH, kDendrites, kHistory,
+// checkpoint_impl.cuh:21-32 over a build with `history` ring slots (9 or 17)
+uint64_t layout_hash(uint64_t history) {
+    const uint64_t record = history == 17 ? sizeof(ns_checkpoint_neuron17) : sizeof(ns_checkpoint_neuron);
+    const uint64_t values[] = {kP, kNeuronCap, kSynapseCap, kS, kM, kW, kH, kDendrites, history,
                                kTicks, kAnimalBytes, sizeof(ns_neuron_gene), sizeof(ns_neuron_state),
                                sizeof(ns_synapse_gene), sizeof(ns_synapse_state),
-                               sizeof(ns_checkpoint_neuron), sizeof(ns_checkpoint_synapse),
+                               record, sizeof(ns_checkpoint_synapse),
                                kTotalsBytes, kLifeBytes, kBiteBytes};
     uint64_t h = 1469598103934665603ull;            // FNV-1a over whole values
     for (uint64_t v : values) {
@@ -59,10 +71,15 @@ uint64_t layout_hash() {
     return h;
 }
 
-// checkpoint_impl.cuh:368-383
-void check_header(const ns_checkpoint_header &h) {
+// The layout version a header names: 9 or 17 ring slots, 0 for neither.
+uint64_t header_history(const ns_checkpoint_header &h) {
+    return h.layout == layout_hash(9) ? 9 : h.layout == layout_hash(17) ? 17 : 0;
+}
+
+// checkpoint_impl.cuh:368-383, the layout being that of `history` slots
+void check_header(const ns_checkpoint_header &h, uint64_t history) {
     if (std::memcmp(h.magic, "NSCHKP01", 8) != 0 || h.version != 1 || h.endian != 0x01020304u ||
-        h.layout != layout_hash() || h.step < 0 || h.step > INT64_MAX / int64_t(kTicks) ||
+        h.layout != layout_hash(history) || h.step < 0 || h.step > INT64_MAX / int64_t(kTicks) ||
         h.checkpoint_every <= 0 || h.initial < 1 || uint64_t(h.initial) > kP ||
         h.trace_animal < -1 || h.trace_animal >= h.initial || (h.flags & ~127u) != 0 ||
         !std::isfinite(h.food_renewal) || h.food_renewal < 0 || h.food_renewal > 1 ||
@@ -139,7 +156,8 @@ void check_animals(const unsigned char *animals, uint64_t neurons, uint64_t syna
 }
 
 // checkpoint_impl.cuh:247-256
-void check_neurons(const ns_checkpoint_neuron *neurons, uint64_t nn, const std::vector<uint8_t> &alive) {
+template <class R>
+void check_neurons(const R *neurons, uint64_t nn, const std::vector<uint8_t> &alive) {
     int prev = -1;
     for (uint64_t i = 0; i < nn; ++i) {
         const auto &r = neurons[i];
@@ -166,8 +184,10 @@ void check_synapses(const ns_checkpoint_synapse *synapses, uint64_t ns, const st
 
 struct Checkpoint {
     ns_checkpoint_header header{};
+    uint64_t history = 9;
     std::vector<unsigned char> world;
-    std::vector<ns_checkpoint_neuron> neurons;
+    std::vector<ns_checkpoint_neuron> neurons;      // 9-slot files
+    std::vector<ns_checkpoint_neuron17> neurons17;  // 17-slot files
     std::vector<ns_checkpoint_synapse> synapses;
 };
 
@@ -181,7 +201,8 @@ Checkpoint *read_checkpoint(const char *path) {
     gzbuffer(in.f, unsigned(kChunk));
     auto ck = std::make_unique<Checkpoint>();
     in.read(&ck->header, sizeof(ck->header));
-    check_header(ck->header);
+    ck->history = header_history(ck->header);
+    check_header(ck->header, ck->history ? ck->history : 9);
     // observer section, read field by field for its limits
     std::vector<unsigned char> &w = ck->world;
     auto take = [&](uint64_t bytes) {
@@ -204,10 +225,16 @@ Checkpoint *read_checkpoint(const char *path) {
     std::vector<uint8_t> alive;
     check_animals(w.data() + arrays, ck->header.neuron_count, ck->header.synapse_count, &alive,
                   "Checkpoint brain allocation counts disagree");
-    ck->neurons.resize(ck->header.neuron_count);
+    if (ck->history == 17) {
+        ck->neurons17.resize(ck->header.neuron_count);
+        in.read(ck->neurons17.data(), ck->neurons17.size() * sizeof(ns_checkpoint_neuron17));
+        check_neurons(ck->neurons17.data(), ck->neurons17.size(), alive);
+    } else {
+        ck->neurons.resize(ck->header.neuron_count);
+        in.read(ck->neurons.data(), ck->neurons.size() * sizeof(ns_checkpoint_neuron));
+        check_neurons(ck->neurons.data(), ck->neurons.size(), alive);
+    }
     ck->synapses.resize(ck->header.synapse_count);
-    in.read(ck->neurons.data(), ck->neurons.size() * sizeof(ns_checkpoint_neuron));
-    check_neurons(ck->neurons.data(), ck->neurons.size(), alive);
     in.read(ck->synapses.data(), ck->synapses.size() * sizeof(ns_checkpoint_synapse));
     check_synapses(ck->synapses.data(), ck->synapses.size(), alive);
     if (ck->header.flags & 64u) {   // plasticity-cohort extension
@@ -225,8 +252,9 @@ Checkpoint *read_checkpoint(const char *path) {
 
 // checkpoint_impl.cuh:135-199 (file order) and 385-413 (temporary file,
 // gzip level 1, fsync, rename, directory fsync).
+template <class R>
 void write_checkpoint(const char *path, const ns_checkpoint_header &given, const unsigned char *world,
-                      uint64_t world_bytes, const ns_checkpoint_neuron *neurons, uint64_t nn,
+                      uint64_t world_bytes, const R *neurons, uint64_t nn,
                       const ns_checkpoint_synapse *synapses, uint64_t ns) {
     if (!path || (world_bytes && !world) || (nn && !neurons) || (ns && !synapses))
         throw std::runtime_error("Invalid checkpoint arguments");
@@ -234,7 +262,7 @@ void write_checkpoint(const char *path, const ns_checkpoint_header &given, const
     h.neuron_count = uint32_t(nn);
     h.synapse_count = uint32_t(ns);
     if (nn > kNeuronCap || ns > kSynapseCap) throw std::runtime_error("Incompatible or invalid checkpoint header");
-    check_header(h);
+    check_header(h, history_of<R>());
     const uint64_t obs = observer_bytes(world, world_bytes);
     const uint64_t want = obs + kArrayBytes + ((h.flags & 64u) ? kExtensionBytes : 0);
     if (!obs || world_bytes != want) throw std::runtime_error("Invalid checkpoint world sections");
@@ -249,30 +277,36 @@ void write_checkpoint(const char *path, const ns_checkpoint_header &given, const
             throw std::runtime_error("Incompatible plasticity intervention extension");
     }
     const std::string temporary = std::string(path) + ".tmp";
-    {
-        GzFile out;
-        out.f = gzopen(temporary.c_str(), "wb1");
-        if (!out.f) throw std::runtime_error("Cannot open checkpoint for writing: " + temporary);
-        gzbuffer(out.f, unsigned(kChunk));
-        out.write(&h, sizeof(h));
-        out.write(world, obs + kArrayBytes);
-        out.write(neurons, nn * sizeof(ns_checkpoint_neuron));
-        out.write(synapses, ns * sizeof(ns_checkpoint_synapse));
-        if (h.flags & 64u) out.write(world + obs + kArrayBytes, kExtensionBytes);
-        const int rc = gzclose(out.f);
-        out.f = nullptr;
-        if (rc != Z_OK) throw std::runtime_error("Cannot finish checkpoint: " + temporary);
+    try {
+        {
+            GzFile out;
+            out.f = gzopen(temporary.c_str(), "wb1");
+            if (!out.f) throw std::runtime_error("Cannot open checkpoint for writing: " + temporary);
+            gzbuffer(out.f, unsigned(kChunk));
+            out.write(&h, sizeof(h));
+            out.write(world, obs + kArrayBytes);
+            out.write(neurons, nn * sizeof(R));
+            out.write(synapses, ns * sizeof(ns_checkpoint_synapse));
+            if (h.flags & 64u) out.write(world + obs + kArrayBytes, kExtensionBytes);
+            const int rc = gzclose(out.f);
+            out.f = nullptr;
+            if (rc != Z_OK) throw std::runtime_error("Cannot finish checkpoint: " + temporary);
+        }
+        int fd = open(temporary.c_str(), O_RDONLY);
+        if (fd < 0 || fsync(fd) != 0) {
+            if (fd >= 0) close(fd);
+            throw std::runtime_error("Cannot sync checkpoint: " + temporary);
+        }
+        close(fd);
+        std::filesystem::rename(temporary, path);
+    } catch (...) {
+        std::error_code ec;                   // no partial file is left behind
+        std::filesystem::remove(temporary, ec);
+        throw;
     }
-    int fd = open(temporary.c_str(), O_RDONLY);
-    if (fd < 0 || fsync(fd) != 0) {
-        if (fd >= 0) close(fd);
-        throw std::runtime_error("Cannot sync checkpoint: " + temporary);
-    }
-    close(fd);
-    std::filesystem::rename(temporary, path);
     std::filesystem::path parent = std::filesystem::path(path).parent_path();
     if (parent.empty()) parent = ".";
-    fd = open(parent.c_str(), O_RDONLY | O_DIRECTORY);
+    const int fd = open(parent.c_str(), O_RDONLY | O_DIRECTORY);
     if (fd >= 0) {
         (void)fsync(fd);
         close(fd);
@@ -286,7 +320,11 @@ using namespace nsa;
 
 extern "C" {
 
-uint64_t ns_checkpoint_layout(void) { return layout_hash(); }
+uint64_t ns_checkpoint_layout(void) { return layout_hash(9); }
+
+uint64_t ns_checkpoint_layout_ex(int history) {
+    return history == 9 || history == 17 ? layout_hash(uint64_t(history)) : 0;
+}
 
 void *ns_checkpoint_open(const char *path) {
     try {
@@ -297,15 +335,28 @@ void *ns_checkpoint_open(const char *path) {
     }
 }
 
-int ns_checkpoint_get(void *ckpt, ns_checkpoint_header *header, const void **world, size_t *world_bytes,
-                      const ns_checkpoint_neuron **neurons, const ns_checkpoint_synapse **synapses) {
+int ns_checkpoint_history(void *ckpt) {
+    return ckpt ? int(static_cast<Checkpoint *>(ckpt)->history) : -1;
+}
+
+}  // extern "C"
+
+template <class R>
+static int get_checkpoint(void *ckpt, ns_checkpoint_header *header, const void **world, size_t *world_bytes,
+                          const R **neurons, const ns_checkpoint_synapse **synapses) {
     try {
         if (!ckpt) throw std::runtime_error("Invalid checkpoint handle");
         const auto &c = *static_cast<Checkpoint *>(ckpt);
+        if (c.history != history_of<R>())
+            throw std::runtime_error(c.history == 17 ? "Checkpoint has 17-slot neuron records"
+                                                     : "Checkpoint has 9-slot neuron records");
         if (header) *header = c.header;
         if (world) *world = c.world.data();
         if (world_bytes) *world_bytes = c.world.size();
-        if (neurons) *neurons = c.neurons.data();
+        if (neurons) {
+            if constexpr (history_of<R>() == 17) *neurons = c.neurons17.data();
+            else *neurons = c.neurons.data();
+        }
         if (synapses) *synapses = c.synapses.data();
         return 0;
     } catch (const std::exception &e) {
@@ -314,11 +365,26 @@ int ns_checkpoint_get(void *ckpt, ns_checkpoint_header *header, const void **wor
     }
 }
 
+extern "C" {
+
+int ns_checkpoint_get(void *ckpt, ns_checkpoint_header *header, const void **world, size_t *world_bytes,
+                      const ns_checkpoint_neuron **neurons, const ns_checkpoint_synapse **synapses) {
+    return get_checkpoint(ckpt, header, world, world_bytes, neurons, synapses);
+}
+
+int ns_checkpoint_get17(void *ckpt, ns_checkpoint_header *header, const void **world, size_t *world_bytes,
+                        const ns_checkpoint_neuron17 **neurons, const ns_checkpoint_synapse **synapses) {
+    return get_checkpoint(ckpt, header, world, world_bytes, neurons, synapses);
+}
+
 void ns_checkpoint_close(void *ckpt) { delete static_cast<Checkpoint *>(ckpt); }
 
-int ns_checkpoint_save(const char *path, const ns_checkpoint_header *header, const void *world,
-                       size_t world_bytes, const ns_checkpoint_neuron *neurons, int neuron_count,
-                       const ns_checkpoint_synapse *synapses, int synapse_count) {
+}  // extern "C"
+
+template <class R>
+static int save_checkpoint(const char *path, const ns_checkpoint_header *header, const void *world,
+                           size_t world_bytes, const R *neurons, int neuron_count,
+                           const ns_checkpoint_synapse *synapses, int synapse_count) {
     try {
         if (!header || neuron_count < 0 || synapse_count < 0)
             throw std::runtime_error("Invalid checkpoint arguments");
@@ -331,18 +397,42 @@ int ns_checkpoint_save(const char *path, const ns_checkpoint_header *header, con
     }
 }
 
+extern "C" {
+
+int ns_checkpoint_save(const char *path, const ns_checkpoint_header *header, const void *world,
+                       size_t world_bytes, const ns_checkpoint_neuron *neurons, int neuron_count,
+                       const ns_checkpoint_synapse *synapses, int synapse_count) {
+    return save_checkpoint(path, header, world, world_bytes, neurons, neuron_count, synapses, synapse_count);
+}
+
+int ns_checkpoint_save17(const char *path, const ns_checkpoint_header *header, const void *world,
+                         size_t world_bytes, const ns_checkpoint_neuron17 *neurons, int neuron_count,
+                         const ns_checkpoint_synapse *synapses, int synapse_count) {
+    return save_checkpoint(path, header, world, world_bytes, neurons, neuron_count, synapses, synapse_count);
+}
+
 int ns_pool_save_checkpoint(void *pool, const char *path, const ns_checkpoint_header *header,
                             const void *world, size_t world_bytes) {
     try {
         if (!pool || !header) throw std::runtime_error("Invalid checkpoint arguments");
         ns_pool_stats st{};
         if (ns_pool_info(pool, &st)) return -1;
-        std::vector<ns_checkpoint_neuron> n(size_t(std::max<int64_t>(st.neuron_records, 1)));
         std::vector<ns_checkpoint_synapse> s(size_t(std::max<int64_t>(st.synapse_records, 1)));
         int nn = 0, ns = 0;
-        if (ns_pool_export_records(pool, n.data(), int(n.size()), s.data(), int(s.size()), &nn, &ns)) return -1;
-        write_checkpoint(path, *header, static_cast<const unsigned char *>(world), world_bytes, n.data(),
-                         uint64_t(nn), s.data(), uint64_t(ns));
+        const size_t ncap = size_t(std::max<int64_t>(st.neuron_records, 1));
+        if (ns_pool_history(pool) == 17) {
+            std::vector<ns_checkpoint_neuron17> n(ncap);
+            if (ns_pool_export_records17(pool, n.data(), int(n.size()), s.data(), int(s.size()), &nn, &ns))
+                return -1;
+            write_checkpoint(path, *header, static_cast<const unsigned char *>(world), world_bytes, n.data(),
+                             uint64_t(nn), s.data(), uint64_t(ns));
+        } else {
+            std::vector<ns_checkpoint_neuron> n(ncap);
+            if (ns_pool_export_records(pool, n.data(), int(n.size()), s.data(), int(s.size()), &nn, &ns))
+                return -1;
+            write_checkpoint(path, *header, static_cast<const unsigned char *>(world), world_bytes, n.data(),
+                             uint64_t(nn), s.data(), uint64_t(ns));
+        }
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -354,10 +444,15 @@ int ns_pool_load_checkpoint(void *pool, const char *path, ns_checkpoint_header *
                             size_t world_capacity, size_t *world_bytes) {
     try {
         if (!pool) throw std::runtime_error("Invalid checkpoint arguments");
+        const int history = ns_pool_history(pool);
+        if (history < 0) return -1;
         std::unique_ptr<Checkpoint> c(read_checkpoint(path));
         if (world && world_capacity < c->world.size()) throw std::runtime_error("World buffer too small");
-        if (ns_pool_import_records(pool, c->neurons.data(), int(c->neurons.size()), c->synapses.data(),
-                                   int(c->synapses.size())))
+        if (int(c->history) != history) throw std::runtime_error("Checkpoint ring length differs from the pool's");
+        if (history == 17 ? ns_pool_import_records17(pool, c->neurons17.data(), int(c->neurons17.size()),
+                                                     c->synapses.data(), int(c->synapses.size()))
+                          : ns_pool_import_records(pool, c->neurons.data(), int(c->neurons.size()),
+                                                   c->synapses.data(), int(c->synapses.size())))
             return -1;
         if (header) *header = c->header;
         if (world) std::memcpy(world, c->world.data(), c->world.size());

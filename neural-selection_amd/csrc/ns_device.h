@@ -39,6 +39,8 @@ constexpr uint32_t NF_TYPE = 0xffu;
 constexpr uint32_t NF_ACTIVE = 1u << 8;
 constexpr uint32_t NF_RECEPTOR = 1u << 9;   // sensory input through receptor events
 constexpr uint32_t NF_WORLD = 1u << 10;     // inactive neurons keep their ring slot
+constexpr int NF_RING_SHIFT = 11;           // bits 11-15: world ring length - 1
+constexpr uint32_t NF_RING = 31u << NF_RING_SHIFT;
 
 // Class of a neuron slot inside a pack: cortical AdEx (0), receptor-driven
 // (1), other AdEx (2), LFS (3), inactive (4).  Packs order their slots by
@@ -67,8 +69,21 @@ constexpr uint32_t R_VALID = 1u << 5, R_STP = 1u << 6, R_PAIR = 1u << 7;
 constexpr unsigned kModNE = 1, kMod5HT = 2, kModDA = 4;
 constexpr float kDepolGateMv = 2.3f;   // profile-2 LTP gate on the post's recent depolarization
 
-// World-rule ring length (src/main.cu:45-46 HISTORY = 9).
+// World-rule ring length (src/main.cu:45-46 HISTORY = D + 1): 9 slots for the
+// reference's delay bound D = 8, D + 1 for a bound of 9..16 (the 1-16 ms
+// delays of the 256 x 256 world).  An inactive neuron keeps its ring slot
+// (main.cu:718), so its shift register rotates: the bit of HISTORY ticks ago
+// becomes the bit of this tick.
 constexpr int kWorldHistory = 9;
+__host__ __device__ inline int world_ring(int max_delay) {
+    return max_delay + 1 > kWorldHistory ? max_delay + 1 : kWorldHistory;
+}
+__host__ __device__ inline uint32_t world_flags(int max_delay) {
+    return NF_WORLD | (uint32_t(world_ring(max_delay) - 1) << NF_RING_SHIFT);
+}
+__host__ __device__ inline uint32_t world_rotate(uint32_t h, uint32_t flags) {
+    return (h << 1) | ((h >> ((flags & NF_RING) >> NF_RING_SHIFT)) & 1u);
+}
 
 // ---------------------------------------------------------------------------
 // Initial states (physiology.cuh:179-202, 393-404), shared by the device reset

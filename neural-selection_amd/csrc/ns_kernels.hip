@@ -278,7 +278,7 @@ __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, co
     if (!(flags & NF_ACTIVE)) {
         uint32_t h = p.hist[i];
         if (flags & NF_WORLD) {
-            p.hist[i] = (h << 1) | ((h >> (kWorldHistory - 1)) & 1u);
+            p.hist[i] = world_rotate(h, flags);
         } else {
             p.hist[i] = h << 1;
             p.spike[i] = 0;
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
             const uint32_t flags = nc[k].flags;
             if (!(flags & NF_ACTIVE)) {
                 const uint32_t h = hist_l[n];
-                hist_l[n] = (flags & NF_WORLD) ? (h << 1) | ((h >> (kWorldHistory - 1)) & 1u) : h << 1;
+                hist_l[n] = (flags & NF_WORLD) ? world_rotate(h, flags) : h << 1;
                 continue;
             }
             const int64_t tick = btick_l[nbr[k]] + t;
@@ -1149,7 +1149,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             const uint32_t flags = nc[k].flags;
             if (!(flags & NF_ACTIVE)) {
                 const uint32_t h = hist_l[n];
-                hist_l[n] = (flags & NF_WORLD) ? (h << 1) | ((h >> (kWorldHistory - 1)) & 1u) : h << 1;
+                hist_l[n] = (flags & NF_WORLD) ? world_rotate(h, flags) : h << 1;
                 continue;
             }
             const int64_t tick = btick_l[nbr[k]] + t;

@@ -14,6 +14,7 @@
 // nondeterministic); here the sum runs in ascending synapse-slot order per
 // post, a fixed order.
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -27,13 +28,36 @@ namespace nsa {
 namespace {
 constexpr int kSenses = 19, kMotors = 5;   // src/main.cu:40-41 (S, M)
 constexpr int kTicksPerStep = 16;          // src/main.cu:47
+constexpr int kMaxHistory = 17;            // ring slots of the 1-16 ms delay layout
+
+// Records are held in the 17-slot form (the 9-slot record is the same up to
+// its ring, ns_amd.h); a pool's own ring length is `history`.
+using NRec = ns_checkpoint_neuron17;
+constexpr size_t kRingOffset = offsetof(ns_checkpoint_neuron, spike_history);
+
+NRec widen(const ns_checkpoint_neuron &r) {
+    NRec o;
+    std::memset(&o, 0, sizeof o);
+    std::memcpy(&o, &r, kRingOffset);
+    std::memcpy(o.spike_history, r.spike_history, sizeof r.spike_history);
+    return o;
+}
+NRec widen(const NRec &r) { return r; }
+void narrow(const NRec &r, ns_checkpoint_neuron *o) {
+    std::memset(o, 0, sizeof *o);
+    std::memcpy(o, &r, kRingOffset);
+    std::memcpy(o->spike_history, r.spike_history, sizeof o->spike_history);
+}
+void narrow(const NRec &r, NRec *o) { *o = r; }
+template <class R> constexpr int history_of() { return sizeof(R::spike_history); }
 
 struct WorldPool {
     int P, ncap, scap;
+    int history = kWorldHistory;              // ring slots: 9, or 17 for 1-16 ms delays
     // Per animal: its records in ascending slot order (genes as given; state
     // as imported, refreshed from the device by pull()), its brain and its
     // pool spec (stable addresses: slot placements read them).
-    std::vector<std::vector<ns_checkpoint_neuron>> nrec;
+    std::vector<std::vector<NRec>> nrec;
     std::vector<std::vector<ns_checkpoint_synapse>> srec;
     std::vector<HostBrain> brains;
     std::vector<BrainSpec> spec;
@@ -65,25 +89,27 @@ struct WorldPool {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // ring slot k (= tick % 9) <-> shift-register bit (T-1-tick)
-    static uint32_t ring_to_bits(const uint8_t *ring, int64_t T) {
+    // ring slot k (= tick % history) <-> shift-register bit (T-1-tick)
+    uint32_t ring_to_bits(const uint8_t *ring, int64_t T) const {
+        const int H = history;
         uint32_t bits = 0;
-        for (int j = 0; j < kWorldHistory; ++j) {
+        for (int j = 0; j < H; ++j) {
             const int64_t tau = T - 1 - j;
-            if (ring[((tau % kWorldHistory) + kWorldHistory) % kWorldHistory]) bits |= 1u << j;
+            if (ring[((tau % H) + H) % H]) bits |= 1u << j;
         }
         return bits;
     }
-    static void bits_to_ring(uint32_t bits, int64_t T, uint8_t *ring) {
-        for (int k = 0; k < kWorldHistory; ++k) {
+    void bits_to_ring(uint32_t bits, int64_t T, uint8_t *ring) const {
+        const int H = history;
+        for (int k = 0; k < H; ++k) {
             const int64_t last = T - 1;
-            const int64_t back = ((last - k) % kWorldHistory + kWorldHistory) % kWorldHistory;
+            const int64_t back = ((last - k) % H + H) % H;
             ring[k] = uint8_t((bits >> back) & 1u);
         }
     }
 
     // checkpoint_impl.cuh:272-311 record checks
-    bool valid(const ns_checkpoint_neuron &r) const {
+    bool valid(const NRec &r) const {
         return r.slot >= 0 && r.slot < ncap && r.owner >= 0 && r.owner < P && r.next >= -1 &&
                r.next < ncap && r.port >= -1 && r.port < kSenses + kMotors && r.gene.type <= 6 &&
                r.gene.modulator <= 3;
@@ -91,18 +117,17 @@ struct WorldPool {
     bool valid(const ns_checkpoint_synapse &r) const {
         return r.slot >= 0 && r.slot < scap && r.owner >= 0 && r.owner < P && r.next >= -1 &&
                r.next < scap && r.gene.pre >= 0 && r.gene.pre < ncap && r.gene.post >= 0 &&
-               r.gene.post < ncap && r.gene.delay >= 1 && r.gene.delay <= kWorldHistory - 1 &&
+               r.gene.post < ncap && r.gene.delay >= 1 && r.gene.delay <= history - 1 &&
                r.gene.branch <= 3;
     }
 
     // Animal a's brain from its records.  main.cu:702-704: a synapse whose
     // endpoints are not neurons of its own animal never delivers or learns;
     // it keeps its state unchanged (pre = post = 0, inactive).
-    static HostBrain make_brain(const std::vector<ns_checkpoint_neuron> &nr,
-                                const std::vector<ns_checkpoint_synapse> &sr) {
+    HostBrain make_brain(const std::vector<NRec> &nr, const std::vector<ns_checkpoint_synapse> &sr) const {
         HostBrain hb;
         hb.channels = kSenses;
-        hb.max_delay = kWorldHistory - 1;
+        hb.max_delay = history - 1;               // world_flags: a ring of `history` slots
         hb.motors.assign(kMotors, -1);
         for (size_t k = 0; k < nr.size(); ++k) {
             const auto &r = nr[k];
@@ -111,11 +136,17 @@ struct WorldPool {
             const bool sensed = ch >= 0 && ch < kSenses;                   // main.cu:729
             hb.sensory.push_back(sensed ? ch : -1);
             hb.receptor.push_back(sensed && r.gene.type == 5 && ch < 6);   // main.cu:735
-            if (r.port >= kSenses && r.port < kSenses + kMotors) hb.motors[r.port - kSenses] = int(k);
+            if (r.port >= kSenses && r.port < kSenses + kMotors) {
+                // one neuron per motor port, as the animal's port table holds it
+                // (main.cu:82 ports[S + M])
+                if (hb.motors[r.port - kSenses] >= 0)
+                    throw std::runtime_error("Two neurons of one animal on a motor port");
+                hb.motors[r.port - kSenses] = int(k);
+            }
         }
         auto local = [&](int slot) {
             auto it = std::lower_bound(nr.begin(), nr.end(), slot,
-                                       [](const ns_checkpoint_neuron &r, int s) { return r.slot < s; });
+                                       [](const NRec &r, int s) { return r.slot < s; });
             return it != nr.end() && it->slot == slot ? int(it - nr.begin()) : -1;
         };
         for (const auto &r : sr) {
@@ -132,7 +163,7 @@ struct WorldPool {
         return hb;
     }
 
-    void set_animal(int a, std::vector<ns_checkpoint_neuron> &&nr,
+    void set_animal(int a, std::vector<NRec> &&nr,
                     std::vector<ns_checkpoint_synapse> &&sr, HostBrain &&hb) {
         for (const auto &r : nrec[a]) nused[r.slot] = 0;
         for (const auto &r : srec[a]) sused[r.slot] = 0;
@@ -224,7 +255,7 @@ struct WorldPool {
             std::vector<int64_t> ticks;
             std::vector<uint64_t> st;
             pool.download_state(n, s, h, ticks, &st);
-            uint8_t ring[kWorldHistory];
+            uint8_t ring[kMaxHistory];
             for (auto &bits : h) {
                 bits_to_ring(bits, next_tick, ring);
                 bits = ring_to_bits(ring, T);
@@ -262,14 +293,16 @@ struct WorldPool {
         flags_generation = pool.generation;
     }
 
-    void replace(int count, const int *animals, const ns_checkpoint_neuron *neurons, int neuron_count,
+    void replace(int count, const int *animals, const NRec *neurons, int neuron_count,
                  const ns_checkpoint_synapse *synapses, int synapse_count);
+    void import_records(const NRec *neurons, int neuron_count, const ns_checkpoint_synapse *synapses,
+                        int synapse_count);
 };
 
 // Animals animals[0..count) lose their records; the given records (each
 // owned by one of them) become their new brains.  Checks everything before
 // changing anything.
-void WorldPool::replace(int count, const int *animals, const ns_checkpoint_neuron *neurons,
+void WorldPool::replace(int count, const int *animals, const NRec *neurons,
                         int neuron_count, const ns_checkpoint_synapse *synapses, int synapse_count) {
     if (count < 0 || neuron_count < 0 || synapse_count < 0 || (count && !animals) ||
         (neuron_count && !neurons) || (synapse_count && !synapses))
@@ -311,7 +344,7 @@ void WorldPool::replace(int count, const int *animals, const ns_checkpoint_neuro
         for (const auto &r : srec[animals[j]]) sused[r.slot] = 1;
     }
     if (clash) throw std::runtime_error("Record slot in use");
-    std::vector<std::vector<ns_checkpoint_neuron>> nr(count);
+    std::vector<std::vector<NRec>> nr(count);
     std::vector<std::vector<ns_checkpoint_synapse>> sr(count);
     for (int i = 0; i < neuron_count; ++i) nr[listed[neurons[i].owner]].push_back(neurons[i]);
     for (int i = 0; i < synapse_count; ++i) {
@@ -402,6 +435,121 @@ void WorldPool::replace(int count, const int *animals, const ns_checkpoint_neuro
     sync_all();
     for (int j = 0; j < count; ++j) placements += new_slot[j] >= 0;
 }
+
+// Every animal's records at once (checks first, then the pool is rebuilt).
+void WorldPool::import_records(const NRec *neurons, int neuron_count, const ns_checkpoint_synapse *synapses,
+                               int synapse_count) {
+    if (neuron_count < 0 || synapse_count < 0 || (neuron_count && !neurons) || (synapse_count && !synapses))
+        throw std::runtime_error("Invalid record count");
+    int prev = -1;
+    for (int i = 0; i < neuron_count; ++i) {
+        if (!valid(neurons[i]) || neurons[i].slot <= prev)
+            throw std::runtime_error("Invalid checkpoint neuron record");
+        prev = neurons[i].slot;
+    }
+    prev = -1;
+    for (int i = 0; i < synapse_count; ++i) {
+        if (!valid(synapses[i]) || synapses[i].slot <= prev)
+            throw std::runtime_error("Invalid checkpoint synapse record");
+        prev = synapses[i].slot;
+    }
+    // group by owner (records stay in ascending slot order per animal)
+    std::vector<std::vector<NRec>> nr(P);
+    std::vector<std::vector<ns_checkpoint_synapse>> sr(P);
+    for (int i = 0; i < neuron_count; ++i) nr[neurons[i].owner].push_back(neurons[i]);
+    for (int i = 0; i < synapse_count; ++i) {
+        if (nr[synapses[i].owner].empty()) throw std::runtime_error("Synapse owner has no neurons");
+        sr[synapses[i].owner].push_back(synapses[i]);
+    }
+    sync_all();
+    for (int a = 0; a < P; ++a) {
+        HostBrain hb = make_brain(nr[a], sr[a]);
+        set_animal(a, std::move(nr[a]), std::move(sr[a]), std::move(hb));
+    }
+    rebuild();
+}
+
+// The record calls for either record layout R, which must be the pool's.
+template <class R> WorldPool &pool_for(void *p) {
+    if (!p) throw std::runtime_error("Invalid pool handle");
+    auto &w = *static_cast<WorldPool *>(p);
+    if (w.history != history_of<R>())
+        throw std::runtime_error(w.history == 17 ? "Pool holds 17-slot neuron records"
+                                                 : "Pool holds 9-slot neuron records");
+    return w;
+}
+
+template <class R> std::vector<NRec> widen_all(const R *neurons, int count) {
+    std::vector<NRec> out;
+    if (count > 0 && neurons) {
+        out.reserve(size_t(count));
+        for (int i = 0; i < count; ++i) out.push_back(widen(neurons[i]));
+    }
+    return out;
+}
+
+template <class R>
+int import_records(void *p, const R *neurons, int neuron_count, const ns_checkpoint_synapse *synapses,
+                   int synapse_count) {
+    try {
+        auto &w = pool_for<R>(p);
+        if (neuron_count < 0 || (neuron_count && !neurons)) throw std::runtime_error("Invalid record count");
+        const std::vector<NRec> n = widen_all(neurons, neuron_count);
+        w.import_records(n.data(), neuron_count, synapses, synapse_count);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+template <class R>
+int replace_records(void *p, int animal_count, const int *animals, const R *neurons, int neuron_count,
+                    const ns_checkpoint_synapse *synapses, int synapse_count) {
+    try {
+        auto &w = pool_for<R>(p);
+        if (neuron_count < 0 || (neuron_count && !neurons)) throw std::runtime_error("Invalid replacement");
+        const std::vector<NRec> n = widen_all(neurons, neuron_count);
+        w.replace(animal_count, animals, n.data(), neuron_count, synapses, synapse_count);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+template <class R>
+int export_records(void *p, R *neurons, int neuron_capacity, ns_checkpoint_synapse *synapses,
+                   int synapse_capacity, int *neuron_count, int *synapse_count) {
+    try {
+        auto &w = pool_for<R>(p);
+        if (!neuron_count || !synapse_count) throw std::runtime_error("Invalid count pointer");
+        if (neuron_capacity < w.nrecords || synapse_capacity < w.srecords ||
+            (w.nrecords && !neurons) || (w.srecords && !synapses))
+            throw std::runtime_error("Record buffer too small");
+        w.sync_all();
+        w.pull();
+        // ascending slot order over every animal's records
+        std::vector<std::pair<int, const NRec *>> on;
+        std::vector<std::pair<int, const ns_checkpoint_synapse *>> os;
+        on.reserve(size_t(w.nrecords));
+        os.reserve(size_t(w.srecords));
+        for (int a = 0; a < w.P; ++a) {
+            for (const auto &r : w.nrec[a]) on.push_back({r.slot, &r});
+            for (const auto &r : w.srec[a]) os.push_back({r.slot, &r});
+        }
+        std::sort(on.begin(), on.end());
+        std::sort(os.begin(), os.end());
+        for (size_t i = 0; i < on.size(); ++i) narrow(*on[i].second, &neurons[i]);
+        for (size_t i = 0; i < os.size(); ++i) std::memcpy(&synapses[i], os[i].second, sizeof(synapses[i]));
+        *neuron_count = int(on.size());
+        *synapse_count = int(os.size());
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
 }  // namespace
 
 }  // namespace nsa
@@ -410,11 +558,14 @@ using namespace nsa;
 
 extern "C" {
 
-void *ns_pool_create(int animal_capacity, int neuron_capacity, int synapse_capacity) {
+void *ns_pool_create_ex(int animal_capacity, int neuron_capacity, int synapse_capacity, int history) {
     try {
         if (animal_capacity < 1 || neuron_capacity < 1 || synapse_capacity < 0)
             throw std::runtime_error("Invalid pool capacity");
+        if (history != kWorldHistory && history != kMaxHistory)
+            throw std::runtime_error("Invalid ring length (9 or 17 slots)");
         auto *w = new WorldPool;
+        w->history = history;
         w->P = animal_capacity;
         w->ncap = neuron_capacity;
         w->scap = synapse_capacity;
@@ -440,46 +591,22 @@ void *ns_pool_create(int animal_capacity, int neuron_capacity, int synapse_capac
     }
 }
 
+void *ns_pool_create(int animal_capacity, int neuron_capacity, int synapse_capacity) {
+    return ns_pool_create_ex(animal_capacity, neuron_capacity, synapse_capacity, kWorldHistory);
+}
+
+int ns_pool_history(void *p) { return p ? static_cast<WorldPool *>(p)->history : -1; }
+
 void ns_pool_destroy(void *p) { delete static_cast<WorldPool *>(p); }
 
 int ns_pool_import_records(void *p, const ns_checkpoint_neuron *neurons, int neuron_count,
                            const ns_checkpoint_synapse *synapses, int synapse_count) {
-    try {
-        auto &w = *static_cast<WorldPool *>(p);
-        if (neuron_count < 0 || synapse_count < 0 || (neuron_count && !neurons) ||
-            (synapse_count && !synapses))
-            throw std::runtime_error("Invalid record count");
-        int prev = -1;
-        for (int i = 0; i < neuron_count; ++i) {
-            if (!w.valid(neurons[i]) || neurons[i].slot <= prev)
-                throw std::runtime_error("Invalid checkpoint neuron record");
-            prev = neurons[i].slot;
-        }
-        prev = -1;
-        for (int i = 0; i < synapse_count; ++i) {
-            if (!w.valid(synapses[i]) || synapses[i].slot <= prev)
-                throw std::runtime_error("Invalid checkpoint synapse record");
-            prev = synapses[i].slot;
-        }
-        // group by owner (records stay in ascending slot order per animal)
-        std::vector<std::vector<ns_checkpoint_neuron>> nr(w.P);
-        std::vector<std::vector<ns_checkpoint_synapse>> sr(w.P);
-        for (int i = 0; i < neuron_count; ++i) nr[neurons[i].owner].push_back(neurons[i]);
-        for (int i = 0; i < synapse_count; ++i) {
-            if (nr[synapses[i].owner].empty()) throw std::runtime_error("Synapse owner has no neurons");
-            sr[synapses[i].owner].push_back(synapses[i]);
-        }
-        w.sync_all();
-        for (int a = 0; a < w.P; ++a) {
-            HostBrain hb = WorldPool::make_brain(nr[a], sr[a]);
-            w.set_animal(a, std::move(nr[a]), std::move(sr[a]), std::move(hb));
-        }
-        w.rebuild();
-        return 0;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return -1;
-    }
+    return import_records(p, neurons, neuron_count, synapses, synapse_count);
+}
+
+int ns_pool_import_records17(void *p, const ns_checkpoint_neuron17 *neurons, int neuron_count,
+                             const ns_checkpoint_synapse *synapses, int synapse_count) {
+    return import_records(p, neurons, neuron_count, synapses, synapse_count);
 }
 
 int ns_pool_reserve(void *p, float slack, float spare) {
@@ -502,14 +629,13 @@ int ns_pool_reserve(void *p, float slack, float spare) {
 int ns_pool_replace_records(void *p, int animal_count, const int *animals,
                             const ns_checkpoint_neuron *neurons, int neuron_count,
                             const ns_checkpoint_synapse *synapses, int synapse_count) {
-    try {
-        auto &w = *static_cast<WorldPool *>(p);
-        w.replace(animal_count, animals, neurons, neuron_count, synapses, synapse_count);
-        return 0;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return -1;
-    }
+    return replace_records(p, animal_count, animals, neurons, neuron_count, synapses, synapse_count);
+}
+
+int ns_pool_replace_records17(void *p, int animal_count, const int *animals,
+                              const ns_checkpoint_neuron17 *neurons, int neuron_count,
+                              const ns_checkpoint_synapse *synapses, int synapse_count) {
+    return replace_records(p, animal_count, animals, neurons, neuron_count, synapses, synapse_count);
 }
 
 int ns_pool_info(void *p, ns_pool_stats *info) {
@@ -535,34 +661,15 @@ int ns_pool_info(void *p, ns_pool_stats *info) {
 int ns_pool_export_records(void *p, ns_checkpoint_neuron *neurons, int neuron_capacity,
                            ns_checkpoint_synapse *synapses, int synapse_capacity, int *neuron_count,
                            int *synapse_count) {
-    try {
-        auto &w = *static_cast<WorldPool *>(p);
-        if (!neuron_count || !synapse_count) throw std::runtime_error("Invalid count pointer");
-        if (neuron_capacity < w.nrecords || synapse_capacity < w.srecords ||
-            (w.nrecords && !neurons) || (w.srecords && !synapses))
-            throw std::runtime_error("Record buffer too small");
-        w.sync_all();
-        w.pull();
-        // ascending slot order over every animal's records
-        std::vector<std::pair<int, const ns_checkpoint_neuron *>> on;
-        std::vector<std::pair<int, const ns_checkpoint_synapse *>> os;
-        on.reserve(size_t(w.nrecords));
-        os.reserve(size_t(w.srecords));
-        for (int a = 0; a < w.P; ++a) {
-            for (const auto &r : w.nrec[a]) on.push_back({r.slot, &r});
-            for (const auto &r : w.srec[a]) os.push_back({r.slot, &r});
-        }
-        std::sort(on.begin(), on.end());
-        std::sort(os.begin(), os.end());
-        for (size_t i = 0; i < on.size(); ++i) std::memcpy(&neurons[i], on[i].second, sizeof(neurons[i]));
-        for (size_t i = 0; i < os.size(); ++i) std::memcpy(&synapses[i], os[i].second, sizeof(synapses[i]));
-        *neuron_count = int(on.size());
-        *synapse_count = int(os.size());
-        return 0;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return -1;
-    }
+    return export_records(p, neurons, neuron_capacity, synapses, synapse_capacity, neuron_count,
+                          synapse_count);
+}
+
+int ns_pool_export_records17(void *p, ns_checkpoint_neuron17 *neurons, int neuron_capacity,
+                             ns_checkpoint_synapse *synapses, int synapse_capacity, int *neuron_count,
+                             int *synapse_count) {
+    return export_records(p, neurons, neuron_capacity, synapses, synapse_capacity, neuron_count,
+                          synapse_count);
 }
 
 int ns_world_neural_step(void *p, int64_t first_tick, const float *senses, int *motors,

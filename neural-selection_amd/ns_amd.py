@@ -128,6 +128,11 @@ SIGNATURES = {
     'ns_device_math_digest': ([ct.c_int, ct.c_uint64, ct.c_uint64, ct.c_int64,
                                ct.POINTER(ct.c_uint64)], ct.c_int),
     'ns_pool_create': ([ct.c_int, ct.c_int, ct.c_int], _vp),
+    'ns_pool_create_ex': ([ct.c_int, ct.c_int, ct.c_int, ct.c_int], _vp),
+    'ns_pool_history': ([_vp], ct.c_int),
+    'ns_pool_import_records17': ([_vp, _vp, ct.c_int, _vp, ct.c_int], ct.c_int),
+    'ns_pool_export_records17': ([_vp, _vp, ct.c_int, _vp, ct.c_int, _i32, _i32], ct.c_int),
+    'ns_pool_replace_records17': ([_vp, ct.c_int, _i32, _vp, ct.c_int, _vp, ct.c_int], ct.c_int),
     'ns_pool_destroy': ([_vp], None),
     'ns_pool_import_records': ([_vp, _vp, ct.c_int, _vp, ct.c_int], ct.c_int),
     'ns_pool_export_records': ([_vp, _vp, ct.c_int, _vp, ct.c_int, _i32, _i32], ct.c_int),
@@ -145,12 +150,18 @@ SIGNATURES = {
     'ns_serve_drain': ([_vp], ct.c_int),
     'ns_serve_stats_get': ([_vp, ct.POINTER(ServeStats)], ct.c_int),
     'ns_checkpoint_layout': ([], ct.c_uint64),
+    'ns_checkpoint_layout_ex': ([ct.c_int], ct.c_uint64),
     'ns_checkpoint_open': ([ct.c_char_p], _vp),
+    'ns_checkpoint_history': ([_vp], ct.c_int),
     'ns_checkpoint_get': ([_vp, ct.POINTER(CheckpointHeader), ct.POINTER(_vp), ct.POINTER(ct.c_size_t),
                            ct.POINTER(_vp), ct.POINTER(_vp)], ct.c_int),
+    'ns_checkpoint_get17': ([_vp, ct.POINTER(CheckpointHeader), ct.POINTER(_vp), ct.POINTER(ct.c_size_t),
+                             ct.POINTER(_vp), ct.POINTER(_vp)], ct.c_int),
     'ns_checkpoint_close': ([_vp], None),
     'ns_checkpoint_save': ([ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t, _vp, ct.c_int, _vp,
                             ct.c_int], ct.c_int),
+    'ns_checkpoint_save17': ([ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t, _vp, ct.c_int, _vp,
+                              ct.c_int], ct.c_int),
     'ns_pool_save_checkpoint': ([_vp, ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t], ct.c_int),
     'ns_pool_load_checkpoint': ([_vp, ct.c_char_p, ct.POINTER(CheckpointHeader), _vp, ct.c_size_t,
                                  ct.POINTER(ct.c_size_t)], ct.c_int),

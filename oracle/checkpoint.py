@@ -35,6 +35,15 @@ NEURON_RECORD = np.dtype({
     'formats': ['<i4', '<i4', '<i4', '<i4', ('u1', 124), ('u1', 120), ('<f4', 4), '<f4',
                 ('<u8', 6), ('u1', 9), ('u1', 7)],
     'offsets': [0, 4, 8, 12, 16, 140, 260, 276, 280, 328, 337], 'itemsize': 344})
+# The record of a build with 1-16 ms delays: main.cu:46 HISTORY = D + 1 = 17
+# ring slots, 345 bytes of fields padded to the 8-byte alignment of the
+# mod_event_tick array: 352.
+HISTORY17 = 17
+NEURON_RECORD17 = np.dtype({
+    'names': list(NEURON_RECORD.names),
+    'formats': ['<i4', '<i4', '<i4', '<i4', ('u1', 124), ('u1', 120), ('<f4', 4), '<f4',
+                ('<u8', 6), ('u1', 17), ('u1', 7)],
+    'offsets': [0, 4, 8, 12, 16, 140, 260, 276, 280, 328, 345], 'itemsize': 352})
 SYNAPSE_RECORD = np.dtype({
     'names': ['slot', 'owner', 'next', 'gene', 'pre', 'post', 'pad', 'state'],
     'formats': ['<i4', '<i4', '<i4', ('u1', 40), '<i4', '<i4', '<u4', ('u1', 80)],
@@ -52,8 +61,8 @@ def layout_hash(values):
 
 
 def layout_values(animal, neuron_gene, neuron_state, synapse_gene, synapse_state, ck_neuron, ck_synapse,
-                  totals, life, bite):
-    return [P, NEURON_CAPACITY, SYNAPSE_CAPACITY, S, M, W, H, DENDRITES, HISTORY, TICKS, animal,
+                  totals, life, bite, history=HISTORY):
+    return [P, NEURON_CAPACITY, SYNAPSE_CAPACITY, S, M, W, H, DENDRITES, history, TICKS, animal,
             neuron_gene, neuron_state, synapse_gene, synapse_state, ck_neuron, ck_synapse, totals, life,
             bite]
 
@@ -113,17 +122,18 @@ def write(path, head, world, neurons, synapses):
         f.write(compose(head, world, neurons, synapses))
 
 
-def parse(raw):
+def parse(raw, neuron_record=NEURON_RECORD):
     """Split an uncompressed file into (header tuple, world block, neuron
-    records, synapse records) in the reader's order."""
+    records, synapse records) in the reader's order; neuron_record is the
+    layout version's record (NEURON_RECORD or NEURON_RECORD17)."""
     h = HEADER.unpack_from(raw, 0)
     off = HEADER.size
     obs = observer_bytes(raw[off:])
     world = raw[off:off + obs + ARRAY_BYTES]
     off += obs + ARRAY_BYTES
     nn, ns = h[11], h[12]
-    neurons = np.frombuffer(raw, NEURON_RECORD, nn, off).copy()
-    off += nn * NEURON_RECORD.itemsize
+    neurons = np.frombuffer(raw, neuron_record, nn, off).copy()
+    off += nn * neuron_record.itemsize
     synapses = np.frombuffer(raw, SYNAPSE_RECORD, ns, off).copy()
     off += ns * SYNAPSE_RECORD.itemsize
     if h[10] & 64:

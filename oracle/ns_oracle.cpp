@@ -1457,4 +1457,23 @@ void ora_world_layout(uint64_t *out) {
     out[9] = sizeof(Header);
 }
 
+// CheckpointNeuron (main.cu:164-171) laid out for a ring of HISTORY slots
+// (main.cu:46 HISTORY = D + 1): out = {sizeof at 9 slots, sizeof at 17,
+// offsetof spike_history}.
+}  // extern "C"
+template <int HISTORY> struct CheckpointNeuronOf {
+    int slot, owner, next, port;
+    ns_neuron_gene gene;
+    ns_neuron_state state;
+    float pending_exc[4], pending_inh;
+    unsigned long long mod_event_tick[6];
+    unsigned char spike_history[HISTORY];
+};
+extern "C" {
+void ora_checkpoint_neuron_layout(uint64_t *out) {
+    out[0] = sizeof(CheckpointNeuronOf<9>);
+    out[1] = sizeof(CheckpointNeuronOf<17>);
+    out[2] = offsetof(CheckpointNeuronOf<17>, spike_history);
+}
+
 }  // extern "C"

@@ -114,6 +114,7 @@ def oracle_lib():
         lib.ora_maze_request.restype = ct.c_int
         lib.ora_maze_layout.argtypes = [ct.c_uint32, ct.c_uint32, _u32, _u32]
         lib.ora_world_layout.argtypes = [_u64]
+        lib.ora_checkpoint_neuron_layout.argtypes = [_u64]
         for name in ('ora_fn_expf', 'ora_fn_logf', 'ora_fn_branch_io'):
             getattr(lib, name).argtypes = [ct.c_float]
             getattr(lib, name).restype = ct.c_float

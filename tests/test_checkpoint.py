@@ -45,6 +45,25 @@ def test_layout_hash_matches_the_restatement():
     assert C.layout_hash(values[::-1]) != h
 
 
+def test_17_slot_record_layout_and_hash():
+    """The layout version of 1-16 ms delays: the compiler's CheckpointNeuron
+    with HISTORY 17 is 352 bytes with the ring at the same offset; its hash
+    is checkpoint_layout over HISTORY 17 and that size; other ring lengths
+    have none."""
+    sizes = np.zeros(3, np.uint64)
+    O.oracle_lib().ora_checkpoint_neuron_layout(sizes.ctypes.data_as(O._u64))
+    assert [int(x) for x in sizes] == [C.NEURON_RECORD.itemsize, C.NEURON_RECORD17.itemsize, 328]
+    animal, _, _, _, totals, life, bite, _, _, _ = _layout()
+    base = (animal, O.NEURON_GENE.itemsize, O.NEURON_STATE.itemsize, O.SYNAPSE_GENE.itemsize,
+            O.SYNAPSE_STATE.itemsize)
+    L = ns_amd.lib()
+    h9 = C.layout_hash(C.layout_values(*base, 344, 136, totals, life, bite, history=9))
+    h17 = C.layout_hash(C.layout_values(*base, 352, 136, totals, life, bite, history=17))
+    assert L.ns_checkpoint_layout_ex(9) == L.ns_checkpoint_layout() == h9
+    assert L.ns_checkpoint_layout_ex(17) == h17 != h9
+    assert L.ns_checkpoint_layout_ex(8) == 0 and L.ns_checkpoint_layout_ex(33) == 0
+
+
 def records(rng, counts):
     """Neuron and synapse records for animals with counts[a] = (n, s):
     slots interleaved across animals, ascending; random genes and state."""
@@ -206,3 +225,68 @@ def test_failed_save_writes_nothing(tmp_path):
     assert lib_save(path, head, world, n3, s) == -1
     assert 'Invalid checkpoint neuron record' in ns_amd.error()
     assert path.read_bytes() == before and not (tmp_path / 'keep.gz.tmp').exists()
+
+
+def records17(rng, counts):
+    n9, s = records(rng, counts)
+    n = np.zeros(len(n9), C.NEURON_RECORD17)
+    raw9 = n9.view(np.uint8).reshape(len(n9), -1)
+    raw = n.view(np.uint8).reshape(len(n), -1)
+    raw[:, :328] = raw9[:, :328]
+    raw[:, 328:345] = np.frombuffer(rng.bytes(len(n) * 17), np.uint8).reshape(len(n), 17)
+    return n, s
+
+
+def test_17_slot_files_in_both_readers(tmp_path):
+    """A file of the 17-slot layout: written by the library as the restated
+    container lays it out, read back by either reader; the 9-slot calls
+    refuse it and a header of the other layout is refused on save."""
+    L = ns_amd.lib()
+    rng = np.random.default_rng(9)
+    counts = {3: (5, 9), 17: (8, 20), 32767: (3, 4), 900: (6, 0)}
+    n, s = records17(rng, counts)
+    world = C.world_block(rng, counts)
+    head = C.header(L.ns_checkpoint_layout_ex(17), neurons=len(n), synapses=len(s), initial=1000)
+    path = tmp_path / 'w17.gz'
+    h = ns_amd.CheckpointHeader.from_buffer_copy(head)
+    assert L.ns_checkpoint_save17(str(path).encode(), ct.byref(h), world, len(world), n.ctypes.data, len(n),
+                                  s.ctypes.data, len(s)) == 0, ns_amd.error()
+    raw = gzip.decompress(path.read_bytes())
+    assert raw == C.compose(head, world, n, s)
+    _, w2, n2, s2 = C.parse(raw, C.NEURON_RECORD17)
+    assert w2 == world and n2.tobytes() == n.tobytes() and s2.tobytes() == s.tobytes()
+    ck = L.ns_checkpoint_open(str(path).encode())
+    assert ck, ns_amd.error()
+    try:
+        assert L.ns_checkpoint_history(ck) == 17
+        got = ns_amd.CheckpointHeader()
+        w, wb, pn, ps = ct.c_void_p(), ct.c_size_t(), ct.c_void_p(), ct.c_void_p()
+        assert L.ns_checkpoint_get(ck, ct.byref(got), ct.byref(w), ct.byref(wb), ct.byref(pn), ct.byref(ps)) == -1
+        assert '17-slot' in ns_amd.error()
+        assert L.ns_checkpoint_get17(ck, ct.byref(got), ct.byref(w), ct.byref(wb), ct.byref(pn),
+                                     ct.byref(ps)) == 0
+        assert bytes(got) == head and ct.string_at(w, wb.value) == world
+        assert ct.string_at(pn, len(n) * 352) == n.tobytes() and ct.string_at(ps, len(s) * 136) == s.tobytes()
+    finally:
+        L.ns_checkpoint_close(ck)
+    # the 9-slot calls on a 9-slot file, and each layout's header on the other's records
+    head9 = C.header(L.ns_checkpoint_layout(), neurons=len(n), synapses=len(s), initial=1000)
+    h9 = ns_amd.CheckpointHeader.from_buffer_copy(head9)
+    assert L.ns_checkpoint_save17(str(tmp_path / 'x.gz').encode(), ct.byref(h9), world, len(world),
+                                  n.ctypes.data, len(n), s.ctypes.data, len(s)) == -1
+    assert 'Incompatible or invalid checkpoint header' in ns_amd.error()
+    assert lib_save(tmp_path / 'y.gz', head, world, n, s) == -1          # 17-slot layout, 9-slot call
+    assert 'Incompatible or invalid checkpoint header' in ns_amd.error()
+    assert not (tmp_path / 'x.gz').exists() and not (tmp_path / 'y.gz').exists()
+
+
+def test_failed_write_leaves_no_temporary_file(tmp_path):
+    """A save whose final rename fails (the path is a directory) reports the
+    error and removes path.tmp."""
+    head, world, n, s = case(seed=10)
+    target = tmp_path / 'busy'
+    target.mkdir()
+    (target / 'inside').write_text('x')
+    assert lib_save(target, head, world, n, s) == -1
+    assert not (tmp_path / 'busy.tmp').exists()
+    assert (target / 'inside').read_text() == 'x'

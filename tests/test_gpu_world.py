@@ -18,18 +18,26 @@ NEURON_RECORD = np.dtype({
     'formats': ['<i4', '<i4', '<i4', '<i4', O.NEURON_GENE, O.NEURON_STATE, ('<f4', 4), '<f4',
                 ('<u8', 6), ('u1', 9)],
     'offsets': [0, 4, 8, 12, 16, 140, 260, 276, 280, 328], 'itemsize': 344})
+# the 17-slot layout (delays 1-16): the same fields, a longer ring
+NEURON_RECORD17 = np.dtype({
+    'names': list(NEURON_RECORD.names),
+    'formats': ['<i4', '<i4', '<i4', '<i4', O.NEURON_GENE, O.NEURON_STATE, ('<f4', 4), '<f4',
+                ('<u8', 6), ('u1', 17)],
+    'offsets': [0, 4, 8, 12, 16, 140, 260, 276, 280, 328], 'itemsize': 352})
 SYNAPSE_RECORD = np.dtype({
     'names': ['slot', 'owner', 'next', 'gene', 'state'],
     'formats': ['<i4', '<i4', '<i4', O.SYNAPSE_GENE, O.SYNAPSE_STATE],
     'offsets': [0, 4, 8, 12, 56], 'itemsize': 136})
 
 
-def build_world(seed=7, animals=10, lo=30, hi=60, types=(0, 1, 4, 5, 6)):
+def build_world(seed=7, animals=10, lo=30, hi=60, types=(0, 1, 4, 5, 6), max_delay=8,
+                record=NEURON_RECORD):
     """Records for `animals` animals with interleaved slots; genes from the
-    small mixed workload, ports 0..18 sensory, 19..23 motors."""
+    small mixed workload, ports 0..18 sensory, 19..23 motors; with the
+    17-slot record, random ring contents."""
     lib = O.oracle_lib()
     w = W.small_mix(seed=seed, brains=animals, types=types, lo=lo, hi=hi,
-                    modulator_p=0.15)
+                    modulator_p=0.15, max_delay=max_delay)
     rng = np.random.default_rng(seed)
     neurons, synapses = [], []
     slot_n, slot_s = 0, 0
@@ -54,9 +62,11 @@ def build_world(seed=7, animals=10, lo=30, hi=60, types=(0, 1, 4, 5, 6)):
         for i in range(n):
             st = np.zeros(1, O.NEURON_STATE)
             lib.ora_fn_initial_neuron(genes[i:i + 1].ctypes.data, int(seed * 1000 + a * 97 + i), st.ctypes.data)
-            r = np.zeros(1, NEURON_RECORD)
+            r = np.zeros(1, record)
             r['slot'] = slots[i]; r['owner'] = a; r['next'] = -1; r['port'] = port[i]
             r['gene'] = genes[i]; r['state'] = st[0]
+            if record is NEURON_RECORD17:
+                r['spike_history'] = (rng.random(17) < 0.2).astype(np.uint8)
             neurons.append(r)
         sg = np.zeros(len(srow), O.SYNAPSE_GENE)
         sg['pre'] = slots[srow[:, 0].astype(int)]
@@ -79,12 +89,12 @@ def build_world(seed=7, animals=10, lo=30, hi=60, types=(0, 1, 4, 5, 6)):
             synapses.append(r)
         slot_s += len(srow)
         brains.append((genes, sg, slots, ch, port))
-    nr = O.cat_records(neurons, NEURON_RECORD)
-    nr = O.cat_records([nr[np.argsort(nr['slot'])]], NEURON_RECORD)
+    nr = O.cat_records(neurons, record)
+    nr = O.cat_records([nr[np.argsort(nr['slot'])]], record)
     return nr, O.cat_records(synapses, SYNAPSE_RECORD), brains
 
 
-def oracle_world(nr, sr, brains):
+def oracle_world(nr, sr, brains, max_delay=8):
     lib = O.oracle_lib()
     handles = []
     for a, (genes, sg, slots, ch, port) in enumerate(brains):
@@ -98,10 +108,11 @@ def oracle_world(nr, sr, brains):
         sgl['post'] = [local[int(x)] for x in syn['gene']['post']]
         ports = np.ascontiguousarray(mine['port'], np.int32)
         h = lib.ora_create_world(len(g), len(sgl), g.ctypes.data, sgl.ctypes.data,
-                                 ports.ctypes.data_as(O._i32), 8)
+                                 ports.ctypes.data_as(O._i32), max_delay)
         ns = np.ascontiguousarray(mine['state'])
         ss = np.ascontiguousarray(syn['state'])
-        lib.ora_import(h, ns.ctypes.data, ss.ctypes.data, None, 0)
+        ring = (np.ascontiguousarray(mine['spike_history']) if max_delay > 8 else None)
+        lib.ora_import(h, ns.ctypes.data, ss.ctypes.data, O.ptr(ring, O._u8), 0)
         handles.append((h, mine['slot'], syn['slot']))
     return handles
 
@@ -518,3 +529,112 @@ def test_device_step_equals_host_step():
         hip.h.hipFree(d_senses)
         hip.h.hipFree(d_motors)
         hip.h.hipStreamDestroy(stream)
+
+
+def test_17_slot_world_layout_steps_rings_and_checkpoint(tmp_path):
+    """The 17-slot record layout (delays 1-16, ring slot = tick % 17):
+    world steps of a ns_pool_create_ex(..., 17) pool equal the oracle's
+    world brains with a 17-slot ring, random rings in, inactive neurons
+    keeping their slots, rings out byte for byte; the 9-slot calls refuse the
+    pool; a checkpoint of the 17-slot layout resumes into a fresh pool as the
+    continuous run goes on."""
+    import checkpoint as C
+    L = ns_amd.lib()
+    lib = O.oracle_lib()
+    nr, sr, brains = build_world(seed=41, animals=10, max_delay=16, record=NEURON_RECORD17)
+    assert sr['gene']['delay'].max() > 8 and (nr['gene']['active'] == 0).any()
+    P = C.P
+    print(f'[world17] animals=10 of {P} neurons={len(nr)} synapses={len(sr)}', flush=True)
+    A = L.ns_pool_create_ex(P, C.NEURON_CAPACITY, C.SYNAPSE_CAPACITY, 17)
+    R = L.ns_pool_create_ex(P, C.NEURON_CAPACITY, C.SYNAPSE_CAPACITY, 17)
+    assert A and R and L.ns_pool_history(A) == 17, ns_amd.error()
+    try:
+        assert L.ns_pool_import_records(A, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == -1
+        assert '17-slot' in ns_amd.error()
+        twice = nr.copy()                              # two neurons of animal 0 on motor port 19
+        mine = np.flatnonzero(twice['owner'] == 0)
+        twice['port'][mine[0]] = 19
+        assert L.ns_pool_import_records17(A, twice.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == -1
+        assert 'motor port' in ns_amd.error()
+        assert L.ns_pool_import_records17(A, nr.ctypes.data, len(nr), sr.ctypes.data, len(sr)) == 0, \
+            ns_amd.error()
+        ora = oracle_world(nr, sr, brains, max_delay=16)
+        frozen = np.zeros(P, np.uint8)
+        frozen[4] = 1
+        first = 7003                                   # not a multiple of 17: slots wrap mid-step
+        ora_motors = np.zeros(P * 5, np.float32)
+
+        def step(pl, k):
+            senses = (np.random.default_rng(300 + k).random(P * 19) < 0.5).astype(np.float32)
+            m = np.zeros(P * 5, np.int32)
+            assert L.ns_world_neural_step(pl, first + 16 * k, senses.ctypes.data_as(O._f),
+                                          m.ctypes.data_as(O._i32), frozen.ctypes.data_as(O._u8)) == 0, \
+                ns_amd.error()
+            age = (np.arange(P) * 3 + k).astype(np.int32)
+            assert L.ns_world_homeostasis(pl, age.ctypes.data_as(O._i32), frozen.ctypes.data_as(O._u8)) == 0
+            return m, senses, age
+
+        def export17(pl):
+            n = np.zeros(len(nr), NEURON_RECORD17)
+            s_ = np.zeros(len(sr), SYNAPSE_RECORD)
+            cn, cs = ct.c_int(), ct.c_int()
+            assert L.ns_pool_export_records17(pl, n.ctypes.data, len(nr), s_.ctypes.data, len(sr),
+                                              ct.byref(cn), ct.byref(cs)) == 0, ns_amd.error()
+            assert cn.value == len(nr) and cs.value == len(sr)
+            return n, s_
+
+        motors = np.zeros(P * 5, np.int32)
+        for k in range(8):
+            m, senses, age = step(A, k)
+            motors += m
+            for a, (h, _, _) in enumerate(ora):
+                lib.ora_set_tick(h, first + 16 * k)
+                lib.ora_set_frozen(h, int(frozen[a]))
+                out = np.zeros(5, np.float32)
+                mine = np.ascontiguousarray(senses[a * 19:(a + 1) * 19])
+                lib.ora_step(h, mine.ctypes.data_as(O._f), 16, 230.0, out.ctypes.data_as(O._f), None)
+                ora_motors[a * 5:(a + 1) * 5] += out
+                lib.ora_world_homeostasis(h, int(age[a]))
+        assert np.array_equal(motors, ora_motors.astype(np.int32))
+        out_n, out_s = export17(A)
+        for a, (h, nslots, sslots) in enumerate(ora):
+            n = len(nslots)
+            ns = np.zeros(n, O.NEURON_STATE)
+            ss = np.zeros(len(sslots), O.SYNAPSE_STATE)
+            ring = np.zeros(n * 17, np.uint8)
+            stamps = np.zeros(n * 6, np.uint64)
+            lib.ora_export(h, ns.ctypes.data, ss.ctypes.data, None, ring.ctypes.data_as(O._u8), None,
+                           stamps.ctypes.data_as(O._u64))
+            dn = out_n[np.isin(out_n['slot'], nslots)]
+            ds = out_s[np.isin(out_s['slot'], sslots)]
+            assert np.array_equal(O.state_bytes(np.ascontiguousarray(dn['state']), O.NEURON_STATE_BYTES),
+                                  O.state_bytes(ns, O.NEURON_STATE_BYTES)), f'animal {a} neurons'
+            assert np.array_equal(O.state_bytes(np.ascontiguousarray(ds['state']), O.SYNAPSE_STATE_BYTES),
+                                  O.state_bytes(ss, O.SYNAPSE_STATE_BYTES)), f'animal {a} synapses'
+            assert np.array_equal(dn['spike_history'].reshape(-1), ring), f'animal {a} 17-slot rings'
+            assert np.array_equal(dn['mod_event_tick'].reshape(-1), stamps), f'animal {a} stamps'
+        # checkpoint of the 17-slot layout, resumed into a fresh pool
+        counts = {a: (int((nr['owner'] == a).sum()), int((sr['owner'] == a).sum())) for a in range(10)}
+        world = C.world_block(np.random.default_rng(41), counts)
+        layout17 = L.ns_checkpoint_layout_ex(17)
+        assert layout17 not in (0, L.ns_checkpoint_layout())
+        path = tmp_path / 'world17.checkpoint.gz'
+        head9 = ns_amd.CheckpointHeader.from_buffer_copy(C.header(L.ns_checkpoint_layout(), step=8, initial=12))
+        assert L.ns_pool_save_checkpoint(A, str(path).encode(), ct.byref(head9), world, len(world)) == -1
+        head = ns_amd.CheckpointHeader.from_buffer_copy(C.header(layout17, step=8, initial=12))
+        assert L.ns_pool_save_checkpoint(A, str(path).encode(), ct.byref(head), world, len(world)) == 0, \
+            ns_amd.error()
+        nine = L.ns_pool_create(P, C.NEURON_CAPACITY, C.SYNAPSE_CAPACITY)
+        try:
+            assert L.ns_pool_load_checkpoint(nine, str(path).encode(), None, None, 0, None) == -1
+            assert 'ring length' in ns_amd.error()
+        finally:
+            L.ns_pool_destroy(nine)
+        assert L.ns_pool_load_checkpoint(R, str(path).encode(), None, None, 0, None) == 0, ns_amd.error()
+        for k in range(8, 12):
+            assert np.array_equal(step(A, k)[0], step(R, k)[0]), f'motors at step {k}'
+        xa, xr = export17(A), export17(R)
+        assert _same_records(xa[0], xr[0]) and _same_records(xa[1], xr[1])
+    finally:
+        L.ns_pool_destroy(A)
+        L.ns_pool_destroy(R)
