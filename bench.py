@@ -261,6 +261,48 @@ def serve_run(eps, rate, episode_steps):
             'device_busy_s': stats.busy_s}
 
 
+def world_run(steps, warmup):
+    """The full C2 world step: 2048 lognormal-size brains under the world rules
+    on the 256 x 256 torus with food A/B/C at 0.05 and 1e-3 regrowth, each
+    step the senses, 16 neural ticks, motor forces, feeding, bites, deaths,
+    world homeostasis and regrowth, all on the device (ns_ecology_step)."""
+    w = W.c2()
+    batch = ns_amd.GpuBatch(workload=w, world_rules=True)
+    eco = None
+    try:
+        batch.reset(w.seeds())
+        eco = ns_amd.Ecology(batch, 256, 256, True, 0.05, 1e-3, seed=1)
+        for s in range(1, warmup + 1):
+            eco.step(s)
+        batch.set_timing(True)
+        i0 = batch.info()
+        step_ms, neurons = [], 0
+        for s in range(warmup + 1, warmup + steps + 1):
+            alive = eco.export()[0]['alive'] == 1      # the animals this step runs
+            neurons += int(w.neuron_counts[alive].sum())
+            t = time.perf_counter()
+            eco.step(s)                                # returns once the step is done on the device
+            step_ms.append((time.perf_counter() - t) * 1e3)
+        i1 = batch.info()
+        an = eco.export()[0]
+        wall = sum(step_ms) / 1e3
+        neural_ms = (i1['resident_ms'] - i0['resident_ms']) / steps
+        return {'desc': 'C2 world: 2048 animals, lognormal brains (median 96, 16..1024 neurons, '
+                        '20 synapses per neuron, delays 1-16 ms), 256x256 torus, food A/B/C 0.05, '
+                        'regrowth 1e-3; senses, 16 ticks, body, feeding, bites, homeostasis, regrowth',
+                'steps': steps, 'warmup': warmup,
+                'ms_per_world_step': wall * 1e3 / steps,
+                'step_ms': {'p50': float(np.percentile(step_ms, 50)),
+                            'p99': float(np.percentile(step_ms, 99)), 'max': float(max(step_ms))},
+                'neural_launch_ms_per_step': neural_ms,
+                'neuron_updates_per_s': neurons * w.ticks / wall,
+                'alive_at_end': int((an['alive'] == 1).sum())}
+    finally:
+        if eco is not None:
+            eco.close()
+        batch.close()
+
+
 def bench_serving(args):
     """Continuous batching under Poisson arrivals: per-episode latency (submit
     to finish) and whole-run throughput, one run per arrival rate."""
@@ -414,6 +456,8 @@ def main():
     ap.add_argument('--mazes', type=int, default=10, help='c5: mazes per request')
     ap.add_argument('--maze-horizon', type=int, default=100, help='c5: steps per maze at most')
     ap.add_argument('--no-serving', action='store_true', help='leave out the C4 serving leg')
+    ap.add_argument('--no-world', action='store_true', help='leave out the C2 world-step leg')
+    ap.add_argument('--world-steps', type=int, default=100, help='world leg: timed world steps')
     ap.add_argument('--serving-rate', type=float, default=1000.0, help='serving leg: arrivals per s')
     ap.add_argument('--serving-episodes', type=int, default=200, help='serving leg: episodes')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -626,6 +670,15 @@ def main():
                            f'{args.episode_steps} steps x 16 ticks), Poisson arrivals')
         log(f'  p50 {serving["latency_ms"]["p50"]:.1f} ms, p99 {serving["latency_ms"]["p99"]:.1f} ms')
 
+    world = None
+    if not args.no_world:
+        if args.no_serving:
+            batch.close()
+        log(f'world leg: {args.world_steps} C2 world steps on the device')
+        world = world_run(args.world_steps, 10)
+        log(f'  {world["ms_per_world_step"]:.3f} ms per world step '
+            f'(neural launch {world["neural_launch_ms_per_step"]:.3f} ms)')
+
     cpu = None
     if not args.no_cpu:
         log('timing the CPU oracle on a bounded sample (one thread, then the host share)')
@@ -705,6 +758,7 @@ def main():
                       'synapses_per_step': su / w.ticks / args.steps,
                       'plastic_pairs_per_step': su_pairs / w.ticks / args.steps},
         'serving_c4': serving,
+        'world_c2': world,
         'cpu_baseline': cpu,
     }
     if births is not None:
@@ -715,7 +769,7 @@ def main():
                          'slots': i1['slots'], 'free_slots': i1['free_slots'],
                          'slot_neurons': i1['slot_neurons'], 'slot_synapses': i1['slot_synapses']}
     print(json.dumps(line), flush=True)
-    if args.no_serving:
+    if args.no_serving and args.no_world:
         batch.close()
     if births is not None:
         births.close()
