@@ -905,8 +905,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __shared__ uint16_t cnt_l[kPackMotors];
     __shared__ int qlen;
     __shared__ unsigned events_l;
-    __shared__ float crel_l[kRelCap];          // releases of the tick's queued STP arrivals
-    __shared__ uint16_t cq_l[kRelCap];         // their synapses
+    __shared__ float crel_l[kRelCap];          // releases of the tick's short-term arrivals
 
     if (int(blockIdx.x) < nbig) {
         // the pack's synapse arrays are free in a big brain's workgroup
@@ -1063,11 +1062,12 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
 
     unsigned my_events = 0;
     for (int t = 0; t < ticks; ++t) {
-        // Phase A: plasticity of tick t-1, arrivals of tick t (uniform trip
-        // count so every lane reaches the ballots).
-        for (int base = 0; base < ns; base += WG) {
+        // Phase A: plasticity of tick t-1, arrivals of tick t; the arrivals
+        // that take a short-term release are noted (one bit per round) and
+        // released by the same thread after the sweep.
+        uint32_t stp_rounds = 0;
+        for (int base = 0, round = 0; base < ns; base += WG, ++round) {
             const int q = base + tid;
-            bool arrive_stp = false;
             if (q < ns) {
                 const uint32_t f = fl_l[q];
                 const uint32_t route = f >> SF_ROUTE_SHIFT;
@@ -1104,39 +1104,29 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                             }
                         }
                     }
-                    // queued STP arrivals take their code below
+                    // short-term arrivals take their code below
                     ai_l[q] = uint16_t((pend << kPendShift) | (arr ? kArrDirect : kArrNone));
                     if (arr) mod_l[post] |= kArrivalMark;
                     my_events += arr;
-                    arrive_stp = arr && (route & R_STP);
-                }
-            }
-            const uint64_t m = __ballot(arrive_stp);
-            if (m) {
-                const int lane = tid & 63;
-                int qb = 0;
-                if (lane == 0) qb = atomicAdd(&qlen, __popcll(m));
-                qb = __shfl(qb, 0);
-                if (arrive_stp) {
-                    const int pos = qb + __popcll(m & ((uint64_t(1) << lane) - 1));
-                    const uint32_t keep = ai_l[q] & ~kArrMask;   // deferred steps
-                    if (pos < kRelCap) {
-                        cq_l[pos] = uint16_t(q);
-                        ai_l[q] = uint16_t(keep | uint32_t(pos + 1));
-                    } else {
-                        ai_l[q] = uint16_t(keep | kArrSpill);   // released by its post neuron's thread in phase B
-                    }
+                    if (arr && (route & R_STP)) stp_rounds |= 1u << round;
                 }
             }
         }
-        __syncthreads();
-        // Phase A2: short-term release over the compacted arrivals.
-        {
-            const int n_q = min(qlen, kRelCap);
-            for (int j = tid; j < n_q; j += WG) {
-                const int q = cq_l[j];
+        // Short-term releases (physiology.cuh:425-454) of the arrivals this
+        // thread found: a few per pack and tick, so each is released where it
+        // was found (its weight is final: plasticity of tick t-1 ran above, in
+        // this thread) rather than after a barrier of its own.  A release
+        // past the kRelCap entries is left to its post neuron's thread.
+        for (uint32_t rounds = stp_rounds; rounds; rounds &= rounds - 1) {
+            const int q = __builtin_ctz(rounds) * WG + tid;
+            const uint32_t keep = ai_l[q] & ~kArrMask;   // deferred steps
+            const int pos = atomicAdd(&qlen, 1);
+            if (pos < kRelCap) {
                 const int64_t tick = btick_l[pp_l[q] >> kPpBrain] + t;
-                crel_l[j] = release_at(p, s0 + q, fl_l[q], w_l[q], tick, etab);
+                crel_l[pos] = release_at(p, s0 + q, fl_l[q], w_l[q], tick, etab);
+                ai_l[q] = uint16_t(keep | uint32_t(pos + 1));
+            } else {
+                ai_l[q] = uint16_t(keep | kArrSpill);
             }
         }
         __syncthreads();
@@ -1417,6 +1407,14 @@ NS_INSTANTIATE_LDS(256, 2, 1)
 #define NS_TILE_LDS(WG, KN, MINW, SP, SS)                                                      \
     {WG, KN, 0, MINW, SP, SS, nullptr, &launch_lds_t<WG, KN, MINW, true>,                      \
      &launch_lds_t<WG, KN, MINW, false>}
+// phase A of k_resident_lds notes short-term arrivals in one bit per round
+// of WG synapses: at most 32 rounds per pack
+constexpr bool lds_rounds_fit(int wg, int sp, int ss) { return (sp + wg - 1) / wg <= 32 && (ss + wg - 1) / wg <= 32; }
+static_assert(lds_rounds_fit(1024, 4448, 10784) && lds_rounds_fit(1024, 4224, 10272) &&
+              lds_rounds_fit(1024, 3936, 9536) && lds_rounds_fit(1024, 2048, 3072) &&
+              lds_rounds_fit(512, 1792, 2944) && lds_rounds_fit(512, 4448, 10784) &&
+              lds_rounds_fit(256, 896, 1408) && lds_rounds_fit(256, 1792, 2944),
+              "LDS tile with more than 32 synapse rounds");
 const TileShape kTiles[] = {
     NS_TILE(256, 1, 4, 1),
     NS_TILE(256, 1, 4, 2),

@@ -251,9 +251,9 @@ class GpuBatch:
 
     def __init__(self, genomes=None, channels=None, max_delay=8, world_rules=False,
                  workload=None, handles=False):
+        self.ptr = None
         L = lib()
         self.lib = L
-        self.ptr = None
         if workload is not None:
             counts = workload.neuron_counts
             edges = workload.edge_counts
