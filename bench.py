@@ -720,7 +720,10 @@ def main():
         'synapse_updates_per_s': su / wall,
         'step_latency_ms': {'p50': float(np.percentile(step_ms, 50)),
                             'p99': float(np.percentile(step_ms, 99)),
-                            'max': float(step_ms.max())},
+                            'max': float(step_ms.max()),
+                            # timed steps over 1.5x the median: (index, ms)
+                            'slow_steps': [[int(k), round(float(step_ms[k]), 3)] for k in
+                                           np.flatnonzero(step_ms > 1.5 * np.percentile(step_ms, 50))]},
         'plain_per_tick': {'value': plain_value, 'speedup': value / plain_value,
                            'steps': args.plain_steps,
                            'ms_per_step': plain_wall * 1e3 / max(args.plain_steps, 1),
@@ -765,6 +768,7 @@ def main():
         line['churn'] = {'fraction_per_step': churn, 'brains_per_step': len(plans[0][0]),
                          'slack': args.slack, 'spare': args.spare,
                          'placements': i1['placements'], 'relayouts': i1['relayouts'],
+                         'appended_slots': i1['appended_slots'],
                          'staging_grows': i1['staging_grows'],
                          'slots': i1['slots'], 'free_slots': i1['free_slots'],
                          'slot_neurons': i1['slot_neurons'], 'slot_synapses': i1['slot_synapses']}

@@ -321,6 +321,7 @@ typedef struct ns_batch_info {
     int64_t placements, relayouts;         /* replacements placed in free slots / by rebuild */
     int64_t graph_captures, graph_launches;  /* ns_gpu_set_graph */
     int64_t staging_grows;      /* replacement staging buffers enlarged (host and device) */
+    int64_t appended_slots;     /* slots added past the last one for newborns no free slot fit */
 } ns_batch_info;
 int ns_gpu_info(void *batch, ns_batch_info *info);
 
@@ -336,8 +337,10 @@ int ns_gpu_info(void *batch, ns_batch_info *info);
  * ns_gpu_replace: brains indices[0..count) take the genomes `handles` (made by
  * ns_create; copied, the caller keeps them) with fresh state seeded like
  * ns_gpu_reset with seeds[j].  Each newborn goes to the best-fitting free slot
- * (spares, vacancies, or the slot it replaces); the pools are rebuilt (state
- * kept) only when none fits, or for LP/Prinz cells.  Enqueued on `stream`
+ * (spares, vacancies, or the slot it replaces); one that fits none gets a new
+ * slot past the last one, in room the pool keeps for that (about 1/16 of its
+ * neurons and synapses); the pools are rebuilt (state kept) only when that
+ * room is short, or for LP/Prinz cells.  Enqueued on `stream`
  * (a hipStream_t, NULL = the batch's own stream) after earlier work there;
  * later ns_gpu_step_device calls on that stream see the new brains.  Results
  * are those of a batch created with the new genomes and reset with seeds. */

@@ -378,8 +378,22 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     if (nn >= (int64_t(1) << 30) || ns >= (int64_t(1) << 30))
         throw std::runtime_error("Pool exceeds 2^30 elements");
 
-    // 5. Host structure-of-arrays images, one slot at a time.
-    const int64_t N = std::max<int64_t>(nn, 1), S = std::max<int64_t>(ns, 1);
+    // 5. Host structure-of-arrays images, one slot at a time, in arrays with
+    // the headroom of a growable pool past the last slot (inactive neurons,
+    // invalid synapses until append_slots uses it).
+    int64_t head_n = 0, head_s = 0;
+    int head_slots = 0;
+    if (growable) {
+        int max_n = 1, max_s = 1;
+        for (int pb = 0; pb < nb; ++pb) {
+            max_n = std::max(max_n, bnc[pb]);
+            max_s = std::max(max_s, bsc[pb]);
+        }
+        head_n = std::max<int64_t>(nn / 16, 8 * int64_t(max_n));
+        head_s = std::max<int64_t>(ns / 16, 8 * int64_t(max_s));
+        head_slots = std::max(64, nb / 16);
+    }
+    const int64_t N = std::max<int64_t>(nn + head_n, 1), S = std::max<int64_t>(ns + head_s, 1);
     std::vector<float> ng(NG_FIELDS * N, 0.0f), nx(NX_FIELDS * N, 0.0f);
     std::vector<uint32_t> nflags(N, 0), norig(N, 0);
     std::vector<int32_t> ninput(N, -1), nin(N + 1, 0), nbrain(N, 0), slow_pos;
@@ -456,7 +470,8 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     p.srng = arena.alloc<uint32_t>(S);
     p.arrived = arena.alloc<uint8_t>(S);
     p.released = arena.alloc<float>(S);
-    const int B = std::max(nb, 1);
+    const int B = std::max(nb + head_slots, 1);
+    nb_cap = B;
     int32_t *d_bn0 = arena.alloc<int32_t>(B); up(d_bn0, bn0); p.bn0 = d_bn0;
     int32_t *d_bnc = arena.alloc<int32_t>(B); up(d_bnc, bnc); p.bnc = d_bnc;
     int32_t *d_bs0 = arena.alloc<int32_t>(B); up(d_bs0, bs0); p.bs0 = d_bs0;
@@ -493,10 +508,12 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     // 7. Large slots, packs (with type-sorted neuron slots) and per-tick ranges.
     std::vector<int32_t> big_list(std::max(nbig, 1), 0);
     for (int k = 0; k < nbig; ++k) big_list[k] = fallback_brains + k;
-    d_big = arena.alloc<int32_t>(std::max(nbig, 1));
+    big_cap = std::max(nbig + head_slots, 1);
+    d_big = arena.alloc<int32_t>(big_cap);
     up(d_big, big_list);
     std::vector<uint16_t> h_perm(N, 0);
     packs.clear();
+    open_pack = -1;
     int pb = fallback_brains + nbig;
     for (const Bin &bin : bins) {
         Pack pk{};
@@ -514,10 +531,11 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     }
     d_perm = arena.alloc<uint16_t>(N);
     up(d_perm, h_perm);
-    d_touched = arena.alloc<int32_t>(std::max<size_t>(packs.size(), 1));
+    packs_cap = std::max(int(packs.size()) + head_slots, 1);
+    d_touched = arena.alloc<int32_t>(packs_cap);
     touched.clear();
-    d_reset = arena.alloc<int32_t>(2 * int64_t(std::max(nb, 1)));
-    d_packs = arena.alloc<Pack>(std::max<size_t>(packs.size(), 1));
+    d_reset = arena.alloc<int32_t>(2 * int64_t(B));
+    d_packs = arena.alloc<Pack>(packs_cap);
     if (!packs.empty()) up(d_packs, packs);
     const int64_t fb_n = fallback_brains ? int64_t(bn0[fallback_brains - 1]) + bnc[fallback_brains - 1] : 0;
     const int64_t fb_s = fallback_brains ? int64_t(bs0[fallback_brains - 1]) + bsc[fallback_brains - 1] : 0;
@@ -528,6 +546,13 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
 // ---------------------------------------------------------------------------
 // Slot placement
 // ---------------------------------------------------------------------------
+
+// A kept one-word buffer for a staged copy.
+static const void *keep_i32_once(std::vector<std::vector<unsigned char>> &keep, int32_t v) {
+    keep.emplace_back(size_t(4));
+    std::memcpy(keep.back().data(), &v, 4);
+    return keep.back().data();
+}
 
 // Write one newborn's slot image into its reserved chunks (thread-safe across
 // different slots: every write goes to the slot's own ranges).
@@ -736,6 +761,111 @@ void DevicePool::place(const std::vector<std::pair<int, const BrainSpec *>> &mov
     std::sort(touched.begin(), touched.end());
     touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
     if (!touched.empty()) sg.add(d_touched, touched.data(), 4u * touched.size());
+}
+
+bool DevicePool::append_slots(const std::vector<std::pair<int, int>> &caps,
+                              const std::vector<std::pair<int, int>> &occ, Stager &sg,
+                              std::vector<std::vector<unsigned char>> &keep, std::vector<int> &slots) {
+    slots.clear();
+    if (!growable || caps.empty()) return caps.empty();
+    const int64_t N = view.ncap, S = view.scap;
+    const int nmax = tile.nmax(), smax = tile.ks > 0 ? tile.threads * tile.ks : smax_lds;
+    const int brains_per_pack = std::min(kPackBrains, std::max(1, kPackMotors / std::max(1, outputs)));
+    // 1. Room: every appended slot may need a pack or an own-workgroup entry.
+    int64_t need_n = 0, need_s = 0;
+    for (const auto &c : caps) { need_n += c.first; need_s += c.second; }
+    const int k = int(caps.size());
+    if (nn + need_n > N || ns + need_s > S || nb + k > nb_cap || int(packs.size()) + k > packs_cap ||
+        nbig + k > big_cap)
+        return false;
+    // 2. Slot tables; packable slots join the last appended pack while it has
+    // room and still ends the pool (no own-workgroup slot appended after it).
+    int open = open_pack >= 0 && open_pack == int(packs.size()) - 1 &&
+                       packs[open_pack].b0 + packs[open_pack].nb == nb
+                   ? open_pack : -1;
+    std::vector<int> new_packs, new_big;
+    if (open >= 0) new_packs.push_back(open);  // re-staged with its new extent
+    for (int j = 0; j < k; ++j) {
+        const int cn = caps[j].first, cs = caps[j].second;
+        const int pb = nb++;
+        const int64_t n0 = nn, s0 = ns;
+        nn += cn;
+        ns += cs;
+        bn0.push_back(int32_t(n0)); bnc.push_back(cn);
+        bs0.push_back(int32_t(s0)); bsc.push_back(cs);
+        batch_of.push_back(-1); fallback.push_back(0);
+        occ_n.push_back(0); occ_s.push_back(0);
+        h_bout0.push_back(0);
+        const bool packable = cn <= nmax && cs <= smax && outputs <= kPackMotors;
+        slot_kind.push_back(uint8_t(packable ? 2 : 1));
+        pack_of.push_back(-1);
+        if (packable) {
+            if (open >= 0 && packs[open].b0 + packs[open].nb == pb && packs[open].nn + cn <= nmax &&
+                packs[open].ns + cs <= smax && packs[open].nb < brains_per_pack) {
+                Pack &pk = packs[open];
+                pk.nn += cn; pk.ns += cs; pk.nb += 1; pk.nm = pk.nb * outputs;
+            } else {
+                Pack pk{};
+                pk.b0 = pb; pk.nb = 1; pk.n0 = int32_t(n0); pk.s0 = int32_t(s0);
+                pk.nn = cn; pk.ns = cs; pk.nm = outputs;
+                open = int(packs.size());
+                packs.push_back(pk);
+                new_packs.push_back(open);
+            }
+            pack_of[pb] = open;
+        } else {
+            new_big.push_back(pb);
+        }
+        for (int i = 0; i < cn; ++i) h_nflags[n0 + i] = 0;
+        slots.push_back(pb);
+        // 3. The image parts place() does not write: the slot's brain index on
+        // every neuron and synapse, the neuron runs and endpoints past the
+        // newborn (an empty run at the slot's end, the slot's first neuron),
+        // an empty input and origin past it.
+        const int nc = std::min(occ[j].first, cn), sc = std::min(occ[j].second, cs);
+        auto keep_i32 = [&](int64_t count, int32_t v) {
+            keep.emplace_back(size_t(4) * size_t(std::max<int64_t>(count, 0)));
+            int32_t *x = reinterpret_cast<int32_t *>(keep.back().data());
+            for (int64_t i = 0; i < count; ++i) x[i] = v;
+            return static_cast<const void *>(x);
+        };
+        sg.add(view.nbrain + n0, keep_i32(cn, pb), 4u * cn);
+        sg.add(view.sbrain + s0, keep_i32(cs, pb), 4u * cs);
+        if (cn > nc + 1) sg.add(view.nin + n0 + nc + 1, keep_i32(cn - nc - 1, int32_t(s0 + sc)), 4u * (cn - nc - 1));
+        if (cn > nc) {
+            sg.add(view.ninput + n0 + nc, keep_i32(cn - nc, -1), 4u * (cn - nc));
+            sg.add(d_norig + n0 + nc, keep_i32(cn - nc, 0), 4u * (cn - nc));
+        }
+        if (cs > sc) {
+            sg.add(view.spre + s0 + sc, keep_i32(cs - sc, int32_t(n0)), 4u * (cs - sc));
+            sg.add(view.spost + s0 + sc, keep_i32(cs - sc, int32_t(n0)), 4u * (cs - sc));
+            sg.add(d_sorig + s0 + sc, keep_i32(cs - sc, 0), 4u * (cs - sc));
+        }
+        const int32_t tab[4] = {int32_t(n0), cn, int32_t(s0), cs};
+        const void *const dst[4] = {view.bn0 + pb, view.bnc + pb, view.bs0 + pb, view.bsc + pb};
+        for (int f = 0; f < 4; ++f) sg.add(dst[f], keep_i32(1, tab[f]), 4);
+        keep.emplace_back(size_t(1), (unsigned char)0);
+        sg.add(view.bfrozen + pb, keep.back().data(), 1);
+        for (int64_t q = s0; q < s0 + cs; ++q) syn_perm[q] = -1;
+    }
+    // 4. Pack and own-workgroup entries (packs opened here, in full).
+    for (int idx : new_packs) {
+        keep.emplace_back(sizeof(Pack));
+        std::memcpy(keep.back().data(), &packs[idx], sizeof(Pack));
+        sg.add(d_packs + idx, keep.back().data(), sizeof(Pack));
+    }
+    for (int pb : new_big) {
+        sg.add(d_big + nbig, keep_i32_once(keep, pb), 4);
+        ++nbig;
+    }
+    open_pack = open;
+    all_range.n_hi = nn;
+    all_range.s_hi = ns;
+    all_range.brains = nb;
+    all_range.slots = nb * outputs;
+    appended += k;
+    ++generation;                              // launch sizes changed: graphs capture again
+    return true;
 }
 
 void DevicePool::order_touched(hipStream_t st) {
@@ -1032,6 +1162,7 @@ void Batch::create(std::vector<HostBrain> &&owned, int) {
     std::vector<const HostBrain *> views;
     for (const auto &b : brains) views.push_back(&b);
     const Tile t0 = default_tile(views);
+    pool.growable = true;                    // room for slots appended by replacements
     pool.build(slot_specs(t0), outputs, t0, stream);
     d_input = io.alloc<float>(int64_t(count) * channels);
     d_output = io.alloc<float>(int64_t(count) * outputs);
@@ -1113,6 +1244,7 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
     std::vector<int> old_slot(k), new_slot(k, -1);
     for (int j = 0; j < k; ++j) old_slot[j] = pool.pool_of[idx[j]];
     stage.clear();
+    stage_keep.clear();
     bool rebuild = false;
     for (int j = 0; j < k && !rebuild; ++j)
         if (has_slow_cell(fresh[j]) || (!pool.plastic && has_plastic_pair(fresh[j]))) rebuild = true;
@@ -1126,13 +1258,36 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
         for (int j = 0; j < k; ++j)
             if (pool.slot_kind[old_slot[j]] != 0)
                 freeset.insert({pool.bnc[old_slot[j]], pool.bsc[old_slot[j]], old_slot[j]});
+        std::vector<int> homeless;
         for (int j = 0; j < k; ++j) {
             const int n = int(fresh[j].genes.size()), e = int(fresh[j].edges.size());
             auto it = freeset.lower_bound({n, e, -1});
             while (it != freeset.end() && std::get<1>(*it) < e) ++it;
-            if (it == freeset.end()) { rebuild = true; break; }
+            if (it == freeset.end()) { homeless.push_back(j); continue; }
             new_slot[j] = std::get<2>(*it);
             freeset.erase(it);
+        }
+        // A newborn that fits no free slot gets a new slot past the last one,
+        // in the pool's headroom (rung capacity as the reserve gives), which
+        // changes no other slot; the pool is rebuilt only when that room is
+        // short.
+        if (!homeless.empty()) {
+            const int nmax = pool.tile.nmax();
+            const int smax = pool.tile.ks > 0 ? pool.tile.threads * pool.tile.ks : pool.smax_lds;
+            std::vector<std::pair<int, int>> caps, occ;
+            for (int j : homeless) {
+                const int n = int(fresh[j].genes.size()), e = int(fresh[j].edges.size());
+                int cn = ladder(n, slack), cs = ladder(e, slack);
+                if (n <= nmax && e <= smax) { cn = std::min(cn, nmax); cs = std::min(cs, smax); }
+                caps.push_back({std::max(cn, 1), cs});
+                occ.push_back({n, e});
+            }
+            std::vector<int> added;
+            if (pool.append_slots(caps, occ, stage, stage_keep, added)) {
+                for (size_t h = 0; h < homeless.size(); ++h) new_slot[homeless[h]] = added[h];
+            } else {
+                rebuild = true;
+            }
         }
     }
     if (rebuild) {
@@ -1806,6 +1961,7 @@ int ns_gpu_info(void *p, ns_batch_info *info) {
         info->graph_captures = b.graph_captures;
         info->graph_launches = b.graph_launches;
         info->staging_grows = b.pool.staging.grows + b.pool.dstage_grows;
+        info->appended_slots = b.pool.appended;
         int64_t big_n = 0;
         for (int k = 0; k < b.pool.nbig; ++k) big_n += b.pool.bnc[b.pool.fallback_brains + k];
         info->big_neurons = big_n;

@@ -289,7 +289,25 @@ public:
     uint32_t *d_norig = nullptr, *d_sorig = nullptr, *d_bseed = nullptr;
     int32_t *d_reset = nullptr;      // 2 x nb: slots to reset, then their seeds
     hipStream_t stream = nullptr;
-    uint64_t generation = 0;         // counts builds (device arrays and packs replaced)
+    uint64_t generation = 0;         // counts layouts (builds, appended slots)
+    // Headroom for slots appended after a build: a growable pool reserves
+    // room past its last slot for neurons, synapses, slots, packs and
+    // own-workgroup entries; append_slots fills it and a build starts over.
+    bool growable = false;
+    int nb_cap = 0, packs_cap = 0, big_cap = 0;
+    int64_t appended = 0;            // slots appended since the pool was created
+    int open_pack = -1;              // the pack appended slots extend while it has room
+    // Append empty slots of capacities caps[j] = (neurons, synapses), each
+    // sized for the newborn of occ[j] = (neurons, synapses) that place() puts
+    // there next: a place in the last appended pack while it has room (its
+    // slots and neurons end the pool), else a new pack, or an own-workgroup
+    // entry when larger than a pack, and the parts of the slot image place()
+    // leaves alone (brain indexes, the tails of the runs and endpoints)
+    // staged into `sg` (buffers kept in `keep`).  The new slot indices go to
+    // `slots`.  False, with nothing changed, when the headroom is short.
+    bool append_slots(const std::vector<std::pair<int, int>> &caps,
+                      const std::vector<std::pair<int, int>> &occ, struct Stager &sg,
+                      std::vector<std::vector<unsigned char>> &keep, std::vector<int> &slots);
 };
 
 // One call of `ticks` ticks over every brain whose run flag is set: the
@@ -395,6 +413,7 @@ struct Batch {
     int64_t relayouts = 0, placements = 0;
     std::vector<uint8_t> last_mask;                    // batch order, last run mask
     Stager stage;                                      // reused across replacements
+    std::vector<std::vector<unsigned char>> stage_keep;   // buffers `stage` copies from
 
     void create(std::vector<HostBrain> &&owned, int max_delay_unused);
     void relayout(const Tile &tile);

@@ -41,7 +41,8 @@ class BatchInfo(ct.Structure):
                 ('slots', ct.c_int64), ('free_slots', ct.c_int64), ('slot_neurons', ct.c_int64),
                 ('slot_synapses', ct.c_int64), ('placements', ct.c_int64),
                 ('relayouts', ct.c_int64), ('graph_captures', ct.c_int64),
-                ('graph_launches', ct.c_int64), ('staging_grows', ct.c_int64)]
+                ('graph_launches', ct.c_int64), ('staging_grows', ct.c_int64),
+                ('appended_slots', ct.c_int64)]
 
 
 class ServeConfig(ct.Structure):

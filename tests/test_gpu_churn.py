@@ -2,8 +2,10 @@
 after a replacement equals that of oracle brains where the replaced ones were
 created fresh at the same point and reset with the same seeds.  Covers
 placement into spare slots, into the slot a brain vacated, into slots freed
-by earlier rounds, and the rebuild taken when no free slot fits (a larger
-newborn) or a newborn has LP/Prinz cells."""
+by earlier rounds, new slots appended past the last one when no free slot
+fits (in a pack of their own, or a workgroup of their own when larger than a
+pack), and the rebuild taken when a newborn is larger than the room kept for
+appended slots or has LP/Prinz cells."""
 import numpy as np
 import pytest
 
@@ -61,21 +63,23 @@ def _run(form, tiles, types, rounds):
 ROUNDS = {
     3: (101, [1, 7, 3], 8, 60, (0, 1, 4, 5, 6)),          # small newborns: spares / own slots
     9: (102, [2, 11, 5, 14], 8, 120, (0, 1, 4, 5, 6)),   # mixed sizes, earlier vacancies
-    16: (103, [7], 400, 420, (0, 1, 4, 5)),               # larger than any free slot: rebuild
+    16: (103, [7], 400, 420, (0, 1, 4, 5)),               # larger than any free slot: appended
     24: (104, [0, 9, 12], 8, 100, (0, 1, 4, 5)),
-}
+    30: (107, [4], 2400, 2500, (0, 1, 4, 5)),             # larger than the room left: rebuild
+    34: (108, [13], 3000, 3100, (0, 1, 4, 5)),            # past the new largest slot: appended,
+}                                                         # a workgroup of its own
 
 
 @pytest.mark.parametrize('form', [ns_amd.FORM_RESIDENT, ns_amd.FORM_PER_TICK])
 def test_replacements_match_fresh_brains(form):
     info = _run(form, None, (0, 1, 4, 5, 6), ROUNDS)
-    assert info['placements'] >= 10 and info['relayouts'] >= 1
+    assert info['placements'] >= 10 and info['appended_slots'] >= 2 and info['relayouts'] >= 1
 
 
 @pytest.mark.parametrize('tiles', [(1024, 1, 0, 1, 4448), (1024, 2, 0, 1, 4224), (256, 2, 4)])
 def test_replacements_on_lds_and_register_tiles(tiles):
     info = _run(ns_amd.FORM_RESIDENT, tiles, (0, 1, 4, 5, 6), ROUNDS)
-    assert info['placements'] >= 10
+    assert info['placements'] >= 10 and info['appended_slots'] >= 2 and info['relayouts'] >= 1
 
 
 def test_slow_cell_newborn_rebuilds():
