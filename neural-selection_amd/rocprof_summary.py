@@ -4,6 +4,7 @@ file (the figures bench.py reports beside its own HIP-event timing).
     python neural-selection_amd/rocprof_summary.py --kernel k_resident_lds \
         --trace DIR_OF_KERNEL_TRACE --last 50 \
         [--fetch DIR --write DIR --copy-fetch DIR --copy-write DIR --copy-mib 1024] \
+        [--sq DIR ... --sq-command "..."] \
         --command "..." --pmc-command "..." --workload c3 --tile 1024,1,0,1,2048 \
         --out profiles/k_resident_lds_c3.md
 
@@ -65,6 +66,12 @@ def main():
     ap.add_argument('--pmc-command', default='')
     ap.add_argument('--workload', required=True)
     ap.add_argument('--tile', required=True)
+    ap.add_argument('--sq', nargs='*', default=[],
+                    help='`rocprofv3 --pmc SQ_...` output directories (counters per dispatch)')
+    ap.add_argument('--sq-command', default='')
+    ap.add_argument('--clock-ghz', type=float, default=2.4,
+                    help='shader clock assumed for the SIMD utilisation figure')
+    ap.add_argument('--simds', type=int, default=1024, help='SIMDs of the device (256 CUs x 4)')
     ap.add_argument('--out', required=True)
     a = ap.parse_args()
 
@@ -108,6 +115,35 @@ def main():
                   f'{_copy_reading(a.copy_write):.0f} KiB, i.e. {w_scale:.2f} bytes per counter '
                   f'unit -> {wr / 1e9:.3f} GB written',
                   '', f'bytes_per_launch: {rd + wr:.0f}']
+    if a.sq:
+        sq = {}
+        for d in a.sq:
+            for r in _csv(d, 'counter_collection.csv'):
+                if a.kernel in r['Kernel_Name']:
+                    sq.setdefault(r['Counter_Name'], []).append(float(r['Counter_Value']))
+        mean = {k: statistics.mean(v) for k, v in sq.items()}
+        lines += ['', '## SQ counters (`rocprofv3 --pmc`, per dispatch of this kernel)', '',
+                  f'Command: `{a.sq_command}`', '', '| counter | mean per dispatch | dispatches |',
+                  '|---|---|---|']
+        for k in sorted(mean):
+            lines.append(f'| {k} | {mean[k]:.4g} | {len(sq[k])} |')
+        waves = mean.get('SQ_WAVES')
+        if waves:
+            lines += ['', 'Per wave (counter / SQ_WAVES):', '']
+            for k, label in (('SQ_INSTS_VALU', 'VALU instructions'), ('SQ_INSTS_SALU', 'SALU instructions'),
+                             ('SQ_INSTS_LDS', 'LDS instructions'), ('SQ_INSTS_VMEM_RD', 'vector memory reads'),
+                             ('SQ_INSTS_VMEM_WR', 'vector memory writes')):
+                if k in mean:
+                    lines.append(f'- {label}: {mean[k] / waves:.0f}')
+        if 'SQ_INSTS_VALU' in mean and timed:
+            cycles = statistics.mean(timed) * 1e-3 * a.clock_ghz * 1e9
+            util = mean['SQ_INSTS_VALU'] * 4 / (a.simds * cycles)
+            lines += ['', f'- VALU issue share: {util:.2f} of the SIMD cycles of the average timed launch '
+                          f'(each wave64 VALU instruction holds a SIMD for 4 cycles; {a.simds} SIMDs at '
+                          f'{a.clock_ghz} GHz over {statistics.mean(timed):.3f} ms)']
+        if 'SQ_LDS_BANK_CONFLICT' in mean and 'SQ_LDS_IDX_ACTIVE' in mean:
+            lines.append(f'- LDS bank conflict cycles / LDS active cycles: '
+                         f'{mean["SQ_LDS_BANK_CONFLICT"] / mean["SQ_LDS_IDX_ACTIVE"]:.3f}')
     with open(a.out, 'w') as f:
         f.write('\n'.join(lines) + '\n')
     print('\n'.join(lines))
