@@ -322,6 +322,10 @@ typedef struct ns_batch_info {
     int64_t graph_captures, graph_launches;  /* ns_gpu_set_graph */
     int64_t staging_grows;      /* replacement staging buffers enlarged (host and device) */
     int64_t appended_slots;     /* slots added past the last one for newborns no free slot fit */
+    /* neuron ticks the resident launch skipped: an AdEx cell without sensory
+     * input whose previous tick changed no state, with no arrival (the tick
+     * is then the identity) */
+    int64_t skipped_neuron_ticks;
 } ns_batch_info;
 int ns_gpu_info(void *batch, ns_batch_info *info);
 

@@ -486,12 +486,12 @@ void DevicePool::build(const std::vector<BrainSpec> &specs, int outs, const Tile
     up(d_motor, motor);
     p.motor = d_motor;
     p.outputs = outs;
-    p.events = arena.alloc<unsigned long long>(2);
+    p.events = arena.alloc<unsigned long long>(3);
     NS_HIP(hipMemset(p.btick, 0, sizeof(int64_t) * B));
     NS_HIP(hipMemset(p.brun, 0, B));
     NS_HIP(hipMemset(p.bfrozen, 0, B));
     NS_HIP(hipMemset(p.bstatus, 0, sizeof(int32_t) * B));
-    NS_HIP(hipMemset(p.events, 0, 2 * sizeof(unsigned long long)));
+    NS_HIP(hipMemset(p.events, 0, 3 * sizeof(unsigned long long)));
     NS_HIP(hipMemset(p.hist, 0, sizeof(uint32_t) * N));
     NS_HIP(hipMemset(p.stamps, 0, sizeof(uint64_t) * 6 * N));
     NS_HIP(hipMemset(p.released, 0, sizeof(float) * S));
@@ -1764,7 +1764,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
         std::vector<int64_t> tk;
         std::vector<uint64_t> stamps;
         b.pool.download_state(n, s, h, tk, &stamps);
-        unsigned long long events0[2] = {0, 0};
+        unsigned long long events0[3] = {0, 0, 0};
         NS_HIP(hipMemcpy(events0, b.pool.view.events, sizeof events0, hipMemcpyDeviceToHost));
         // A fixed sensory pattern (30% of channels at level 1) at the gym gain,
         // so every tile is timed on the same, driven dynamics.
@@ -1937,7 +1937,7 @@ int ns_gpu_import_spike_bits(void *p, const uint32_t *bits) {
 int ns_gpu_info(void *p, ns_batch_info *info) {
     try {
         auto &b = *static_cast<Batch *>(p);
-        unsigned long long events[2] = {0, 0};
+        unsigned long long events[3] = {0, 0, 0};
         NS_HIP(hipStreamSynchronize(b.stream));
         NS_HIP(hipMemcpy(events, b.pool.view.events, sizeof events, hipMemcpyDeviceToHost));
         info->brains = b.count;
@@ -1962,6 +1962,7 @@ int ns_gpu_info(void *p, ns_batch_info *info) {
         info->graph_launches = b.graph_launches;
         info->staging_grows = b.pool.staging.grows + b.pool.dstage_grows;
         info->appended_slots = b.pool.appended;
+        info->skipped_neuron_ticks = int64_t(events[2]);
         int64_t big_n = 0;
         for (int k = 0; k < b.pool.nbig; ++k) big_n += b.pool.bnc[b.pool.fallback_brains + k];
         info->big_neurons = big_n;

@@ -165,7 +165,8 @@ struct Pool {
     int32_t *bstatus;
     const int32_t *motor;     // absolute neuron position per output slot
     int outputs;              // outputs per brain
-    unsigned long long *events;   // [0] resident launches, [1] per-tick launches
+    unsigned long long *events;   // arrivals: [0] resident launches, [1] per-tick
+                                  // launches; [2] neuron ticks the resident launch skipped
 };
 
 // ---------------------------------------------------------------------------
@@ -308,15 +309,26 @@ __device__ inline float branch_io(float c, const uint64_t *etab) {
 // and c + offset rounds to offset.  Such a dendrite adds +0 to a sum that is
 // never -0, so it is skipped.  With adapt_ns == 0, w == +0 and a finite
 // v - rest, the adaptation update adds a signed zero to +0, an identity.
+//
+// still (when not null) receives whether the tick left v and w unchanged in
+// every substep with the conductances, the inhibition and a decaying
+// afterdepolarization all +0 throughout (checked on entry: they then stay
+// +0, and they are not read in any other state).
 __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float current,
-                                const uint64_t *etab) {
+                                const uint64_t *etab, bool *still = nullptr) {
     const float h = 0.25f;
     const bool cortical = (g.flags & NF_TYPE) == 0;
     const bool adp_on = !(g.flags & (1u << 31));
     int spike = 0;
+    bool same = true;
+    if (still)
+        same = (__float_as_uint(s.d0) | __float_as_uint(s.d1) | __float_as_uint(s.d2) |
+                __float_as_uint(s.d3) | __float_as_uint(s.inh) |
+                (adp_on ? __float_as_uint(s.adp) : 0u)) == 0u;
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
         float v = s.v;
+        const uint32_t w_in = __float_as_uint(s.w);
         if (!(g.a == 0.0f && __float_as_uint(s.w) == 0u && __builtin_isfinite(v - g.rest)))
             s.w += h * (g.a * (v - g.rest) - s.w) / g.tauw;
         float x = clamp_range((v - g.thr) / g.slope, -20.0f, 10.0f);
@@ -340,11 +352,13 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
             s.w += g.b;
             s.adp += g.adp_spike;
         }
+        if (still) same = same && __float_as_uint(s.v) == __float_as_uint(v) && __float_as_uint(s.w) == w_in;
         if (adp_on) s.adp *= g.adp_decay;
         s.d0 *= kDecayExcQuarter; s.d1 *= kDecayExcQuarter;
         s.d2 *= kDecayExcQuarter; s.d3 *= kDecayExcQuarter;
         s.inh *= kDecayInhQuarter;
     }
+    if (still) *still = same && !spike;
     return spike;
 }
 
@@ -386,11 +400,17 @@ __device__ inline int lfs_tick(NeuronRegs &s, const NeuronConst &g, float *cell,
 // physiology.cuh:290-302 prelude and the per-type dispatch of neuron_advance
 // for the register-resident types (0, 1, 4, 5, 6).  Types 2 and 3 run in
 // slow_neuron_tick.  Returns the spike.
+//
+// still (when not null): whether the tick changed no state (AdEx types only;
+// an LFS cell reports false).  With the same inputs, a tick that changed no
+// state changes none the next time either: the tick is a function of the
+// state and its inputs.
 __device__ inline int neuron_tick(NeuronRegs &s, const NeuronConst &g, float current, float e0,
                                   float e1, float e2, float e3, float inh, float *cell, int64_t N,
-                                  int64_t i, const uint64_t *etab) {
+                                  int64_t i, const uint64_t *etab, bool *still = nullptr) {
     s.d0 += e0; s.d1 += e1; s.d2 += e2; s.d3 += e3;
     float sub = fmaxf(0.0f, fminf(s.v, -50.0f) - g.rest);
+    const uint32_t depol_in = __float_as_uint(s.depol);
     s.depol += 0.1f * (sub - s.depol);
     s.inh += inh;
     if ((g.flags & NF_TYPE) == 6) {
@@ -399,9 +419,12 @@ __device__ inline int neuron_tick(NeuronRegs &s, const NeuronConst &g, float cur
         int spike = lfs_tick(s, g, cell, N, i, current, e, s.inh);
         s.d0 *= kDecayExcMs; s.d1 *= kDecayExcMs; s.d2 *= kDecayExcMs; s.d3 *= kDecayExcMs;
         s.inh *= kDecayInhMs;
+        if (still) *still = false;
         return spike;
     }
-    return adex_tick(s, g, current, etab);
+    const int spike = adex_tick(s, g, current, etab, still);
+    if (still) *still = *still && __float_as_uint(s.depol) == depol_in;
+    return spike;
 }
 
 // homeostatic_advance, physiology.cuh:243-253.

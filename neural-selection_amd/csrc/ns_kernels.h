@@ -30,8 +30,9 @@ struct PlainRange {
 using ResidentLauncher = hipError_t (*)(const Pool &, const Pack *, int, const int32_t *, int,
                                         const uint16_t *, int, const float *, float, float *, int,
                                         hipStream_t);
+// (LDS tiles rewrite the slot order of the packs they run, see k_resident_lds)
 using LdsLauncher = hipError_t (*)(const Pool &, const Pack *, int, const int32_t *, int,
-                                   const uint16_t *, int, const float *, float, float *, int, int,
+                                   uint16_t *, int, const float *, float, float *, int, int,
                                    hipStream_t);
 // A compiled tile.  Register-slot tiles (synapses_per_thread > 0) keep
 // synapse state in registers; LDS tiles keep it in LDS with room for

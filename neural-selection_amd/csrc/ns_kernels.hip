@@ -884,12 +884,15 @@ __device__ inline void replay_quiet_steps(PlastRegs &r, int profile, float w0, u
 // depolarization passes the profile-2 gate: plasticity reads the
 // depolarization only through that comparison.
 constexpr unsigned kModBits = kModNE | kMod5HT | kModDA, kArrivalMark = 0x40u, kDepolGate = 0x80u;
+// NeuronConst flag bit (k_resident_lds): an AdEx cell without sensory input,
+// whose tick may be skipped when its last one changed no state.
+constexpr uint32_t kSkipCell = 1u << 29;
 __device__ inline float gate_depol(unsigned m) { return (m & kDepolGate) ? kDepolGateMv : 0.0f; }
 
 template <int WG, int KN, int MINW, bool PLASTIC>
 __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *packs,
                                                          const int32_t *big, int nbig,
-                                                         const uint16_t *perm, int ticks,
+                                                         uint16_t *perm, int ticks,
                                                          const float *input, float gain,
                                                          float *out, int homeostasis, int smax) {
     constexpr int NMAX = WG * KN;
@@ -904,7 +907,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __shared__ uint16_t mot_l[kPackMotors];
     __shared__ uint16_t cnt_l[kPackMotors];
     __shared__ int qlen;
-    __shared__ unsigned events_l;
+    __shared__ unsigned events_l, skipped_l;
     __shared__ float crel_l[kRelCap];          // releases of the tick's short-term arrivals
 
     if (int(blockIdx.x) < nbig) {
@@ -962,6 +965,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         in_l[pk.nn] = uint16_t(ns);
         qlen = 0;
         events_l = 0;
+        skipped_l = 0;
     }
     for (int m = tid; m < pk.nm; m += WG) {
         const int32_t mn = p.motor[int64_t(pk.b0) * p.outputs + m];
@@ -1056,11 +1060,15 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             nscale[k] = ns_expf(nscale[k], g_exp_tab);
             ncur[k] = nidx[k] >= 0 ? input[nidx[k]] : -1.0f;
             if (nidx[k] < 0) nc[k].flags |= 1u << 30;
+            // an AdEx cell without sensory input: its only inputs are synaptic
+            if (nidx[k] < 0 && (nc[k].flags & NF_TYPE) != 6) nc[k].flags |= kSkipCell;
         }
     }
     __syncthreads();
 
-    unsigned my_events = 0;
+    unsigned my_events = 0, my_skipped = 0;
+    // bit k: neuron slot k's last tick changed no state (see phase B)
+    uint32_t still = 0;
     for (int t = 0; t < ticks; ++t) {
         // Phase A: plasticity of tick t-1, arrivals of tick t; the arrivals
         // that take a short-term release are noted (one bit per round) and
@@ -1142,10 +1150,21 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                 hist_l[n] = (flags & NF_WORLD) ? world_rotate(h, flags) : h << 1;
                 continue;
             }
+            const bool arrived = (mod_l[n] & kArrivalMark) != 0;
+            if (((still >> k) & 1u) && !arrived) {
+                // Its last tick changed no state and no synapse onto it
+                // arrives: its inputs are those of the last tick (+0 sums, no
+                // sensory current), so this tick is the same identity: no
+                // spike, no modulator bits.
+                hist_l[n] <<= 1;
+                mod_l[n] = uint8_t(mod_l[n] & kDepolGate);
+                ++my_skipped;
+                continue;
+            }
             const int64_t tick = btick_l[nbr[k]] + t;
             float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
             unsigned mod = 0;
-            const int q1 = (mod_l[n] & kArrivalMark) ? int(in_l[n + 1]) : 0;
+            const int q1 = arrived ? int(in_l[n + 1]) : 0;
             for (int q = in_l[n]; q < q1; ++q) {
                 // this tick's release: +0 without an arrival, the weight
                 // without short-term plasticity, else the queued release
@@ -1174,8 +1193,10 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             if (!(flags & (1u << 30)))
                 current = (flags & NF_RECEPTOR) ? receptor_drive(nr[k], nc[k], ncur[k])
                                                 : gain * ncur[k];
+            bool same = false;
             const int spike = neuron_tick(nr[k], nc[k], current, e0, e1, e2, e3, inh, p.cell, N,
-                                          n0 + n, etab);
+                                          n0 + n, etab, &same);
+            still = (still & ~(1u << k)) | (uint32_t(same && (flags & kSkipCell)) << k);
             hist_l[n] = (hist_l[n] << 1) | uint32_t(spike);
             mod_l[n] = uint8_t(mod | (nr[k].depol >= kDepolGateMv ? kDepolGate : 0u));
             if (!isfinite(nr[k].v)) p.bstatus[pk.b0 + nbr[k]] = -1;
@@ -1241,10 +1262,61 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     }
     for (int b = tid; b < pk.nb; b += WG)
         if (brun_l[b]) p.btick[pk.b0 + b] = btick_l[b] + ticks;
-    for (int off = 32; off > 0; off >>= 1) my_events += __shfl_xor(my_events, off);
+    for (int off = 32; off > 0; off >>= 1) {
+        my_events += __shfl_xor(my_events, off);
+        my_skipped += __shfl_xor(my_skipped, off);
+    }
     if ((tid & 63) == 0 && my_events) atomicAdd(&events_l, my_events);
+    if ((tid & 63) == 0 && my_skipped) atomicAdd(&skipped_l, my_skipped);
+    // Slot order for the next launch: by cell class as the build orders it,
+    // with the slots whose last tick changed no state (and those of brains
+    // that sat out) after all the others, so that they fill whole
+    // wavefronts, whose ticks then skip together.  The order only assigns
+    // neurons to threads.
+    {
+        constexpr int W = WG / 64, K = 2 * kSlotClasses;
+        const int lane = tid & 63, wave = tid >> 6;
+        int *cnt = reinterpret_cast<int *>(lds_syn);   // [KN][W][K], then totals and bases
+        int *tot = cnt + KN * W * K, *base = tot + K;
+        int key[KN], rank[KN];
+        __syncthreads();   // the synapse arrays are no longer read
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+            key[k] = -1;
+            rank[k] = 0;
+            if (nid[k] < pk.nn)
+                key[k] = slot_class(nc[k].flags) + ((((still >> k) & 1u) || !nlive[k]) ? kSlotClasses : 0);
+            for (int c = 0; c < K; ++c) {
+                const uint64_t m = __ballot(key[k] == c);
+                if (lane == 0) cnt[(k * W + wave) * K + c] = __popcll(m);
+                if (key[k] == c) rank[k] = __popcll(m & ((uint64_t(1) << lane) - 1));
+            }
+        }
+        __syncthreads();
+        if (tid < K) {
+            int s = 0;
+            for (int j = 0; j < KN * W; ++j) {
+                const int x = cnt[j * K + tid];
+                cnt[j * K + tid] = s;
+                s += x;
+            }
+            tot[tid] = s;
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int c = 0, s = 0; c < K; ++c) {
+                base[c] = s;
+                s += tot[c];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KN; ++k)
+            if (key[k] >= 0)
+                perm[n0 + base[key[k]] + cnt[(k * W + wave) * K + key[k]] + rank[k]] = uint16_t(nid[k]);
+    }
     __syncthreads();
     if (tid == 0 && events_l) atomicAdd(&p.events[0], (unsigned long long)events_l);
+    if (tid == 0 && skipped_l) atomicAdd(&p.events[2], (unsigned long long)skipped_l);
 }
 
 // ---------------------------------------------------------------------------
@@ -1343,7 +1415,7 @@ static hipError_t launch_resident_t(const Pool &p, const Pack *packs, int npacks
 
 template <int WG, int KN, int MINW, bool PLASTIC>
 static hipError_t launch_lds_t(const Pool &p, const Pack *packs, int npacks, const int32_t *big,
-                               int nbig, const uint16_t *perm, int ticks, const float *input,
+                               int nbig, uint16_t *perm, int ticks, const float *input,
                                float gain, float *out, int homeostasis, int smax, hipStream_t st) {
     const size_t bytes = size_t(smax) * lds_synapse_bytes<PLASTIC>() + 16;
     // dynamic LDS limit raised so far for this instance, per device (threads
@@ -1375,10 +1447,10 @@ static hipError_t launch_lds_t(const Pool &p, const Pack *packs, int npacks, con
                                                           float, float *, int);
 #define NS_INSTANTIATE_LDS(WG, KN, MINW)                                                        \
     template __global__ void k_resident_lds<WG, KN, MINW, true>(                                 \
-        Pool, const Pack *, const int32_t *, int, const uint16_t *, int, const float *, float,  \
+        Pool, const Pack *, const int32_t *, int, uint16_t *, int, const float *, float,        \
         float *, int, int);                                                                     \
     template __global__ void k_resident_lds<WG, KN, MINW, false>(                                \
-        Pool, const Pack *, const int32_t *, int, const uint16_t *, int, const float *, float,  \
+        Pool, const Pack *, const int32_t *, int, uint16_t *, int, const float *, float,        \
         float *, int, int);
 NS_INSTANTIATE(256, 1, 4, 1)
 NS_INSTANTIATE(256, 1, 4, 2)

@@ -42,7 +42,7 @@ class BatchInfo(ct.Structure):
                 ('slot_synapses', ct.c_int64), ('placements', ct.c_int64),
                 ('relayouts', ct.c_int64), ('graph_captures', ct.c_int64),
                 ('graph_launches', ct.c_int64), ('staging_grows', ct.c_int64),
-                ('appended_slots', ct.c_int64)]
+                ('appended_slots', ct.c_int64), ('skipped_neuron_ticks', ct.c_int64)]
 
 
 class ServeConfig(ct.Structure):
