@@ -627,22 +627,96 @@ static void fill_slot(DevicePool &d, const PlaceChunks &c, unsigned char *base) 
     for (int j = 0; j < d.outputs; ++j) mo[j] = b->motors[j] >= 0 ? int32_t(n0 + b->motors[j]) : -1;
 }
 
+HostWorkers &HostWorkers::get() {
+    static HostWorkers workers;
+    return workers;
+}
+
+HostWorkers::HostWorkers() {
+    // the host share of one GPU is about 16 CPUs; the caller works too
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int count = int(std::min(15u, hw > 1 ? hw - 1 : 1u));
+    for (int i = 0; i < count; ++i) threads_.emplace_back([this] { loop(); });
+}
+
+HostWorkers::~HostWorkers() {
+    {
+        std::lock_guard<std::mutex> g(m_);
+        stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto &th : threads_) th.join();
+}
+
+void HostWorkers::loop() {
+    uint64_t seen = 0;
+    for (;;) {
+        const std::function<void(int)> *fn;
+        int n;
+        {
+            std::unique_lock<std::mutex> g(m_);
+            wake_.wait(g, [&] { return stop_ || round_ != seen; });
+            if (stop_) return;
+            seen = round_;
+            fn = fn_;
+            n = n_;
+            ++busy_;
+        }
+        std::exception_ptr err;
+        try {
+            for (int i = next_.fetch_add(1); i < n; i = next_.fetch_add(1)) (*fn)(i);
+        } catch (...) {
+            err = std::current_exception();
+            next_.store(n);   // the others stop taking indices
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            if (err && !error_) error_ = err;
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+}
+
+void HostWorkers::run(int n, const std::function<void(int)> &fn) {
+    if (n <= 0) return;
+    if (n < 8 || threads_.empty()) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::lock_guard<std::mutex> one(run_lock_);
+    {
+        std::lock_guard<std::mutex> g(m_);
+        fn_ = &fn;
+        n_ = n;
+        next_.store(0);
+        ++round_;
+    }
+    wake_.notify_all();
+    std::exception_ptr err;
+    try {
+        for (int i = next_.fetch_add(1); i < n; i = next_.fetch_add(1)) fn(i);
+    } catch (...) {
+        err = std::current_exception();
+        next_.store(n);
+    }
+    // every index is taken; wait until the workers that joined this round
+    // have finished theirs (a worker that wakes late finds none left)
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [&] { return busy_ == 0; });
+    fn_ = nullptr;
+    if (!err) err = error_;
+    error_ = nullptr;
+    g.unlock();
+    if (err) std::rethrow_exception(err);
+}
+
 void DevicePool::upload_stage(const Stager &sg, hipStream_t st) {
     const size_t bytes = sg.bytes();
     PinnedStaging::Buf &buf = staging.take(bytes);
     unsigned char *base = sg.write(buf.p);
     // slot images, in parallel over newborns (disjoint slots)
-    const int nthreads = int(std::min<size_t>(8, std::max<size_t>(1, fill_list.size() / 16)));
-    if (nthreads <= 1) {
-        for (const auto &c : fill_list) fill_slot(*this, c, base);
-    } else {
-        std::vector<std::thread> workers;
-        for (int t = 0; t < nthreads; ++t)
-            workers.emplace_back([&, t] {
-                for (size_t j = t; j < fill_list.size(); j += nthreads) fill_slot(*this, fill_list[j], base);
-            });
-        for (auto &th : workers) th.join();
-    }
+    HostWorkers::get().run(int(fill_list.size()),
+                           [&](int j) { fill_slot(*this, fill_list[size_t(j)], base); });
     fill_list.clear();
     if (!copy_stream) {
         NS_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
@@ -1696,12 +1770,10 @@ int ns_gpu_replace(void *p, int count, const int *indices, void **handles, const
     try {
         auto &b = *static_cast<Batch *>(p);
         if (count < 0 || (count && (!indices || !handles))) throw std::runtime_error("Invalid replacement");
-        std::vector<HostBrain> fresh;
-        fresh.reserve(count);
-        for (int j = 0; j < count; ++j) {
+        for (int j = 0; j < count; ++j)
             if (!handles[j]) throw std::runtime_error("Invalid genome handle");
-            fresh.push_back(handle_brain(handles[j]));
-        }
+        std::vector<HostBrain> fresh(static_cast<size_t>(count));
+        HostWorkers::get().run(count, [&](int j) { fresh[size_t(j)] = handle_brain(handles[j]); });
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b.stream;
         b.replace(count, indices, std::move(fresh), seeds, st);
         return 0;

@@ -7,7 +7,13 @@
 #include <cstring>
 #include <memory>
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <exception>
+#include <functional>
+#include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -158,6 +164,32 @@ constexpr size_t kStageMinBytes = size_t(64) << 20;
 
 // Pinned host staging for slot placements, double buffered so one buffer can
 // be filled while the other's copy is still in flight.
+// Host worker threads for the placement work between launches (genome
+// copies, slot images), created once per process and kept waiting, so a
+// placement of a few hundred newborns does not pay for thread creation.
+// run(n, fn) calls fn(i) for i in [0, n) on the workers and the caller and
+// returns when every call has returned; calls from several host threads
+// take turns.
+class HostWorkers {
+public:
+    static HostWorkers &get();
+    void run(int n, const std::function<void(int)> &fn);
+    ~HostWorkers();
+
+private:
+    HostWorkers();
+    void loop();
+    std::vector<std::thread> threads_;
+    std::mutex run_lock_, m_;
+    std::condition_variable wake_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    std::atomic<int> next_{0};
+    int n_ = 0, busy_ = 0;
+    uint64_t round_ = 0;
+    bool stop_ = false;
+    std::exception_ptr error_;   // the first exception of the round, rethrown by run
+};
+
 struct PinnedStaging {
     struct Buf { unsigned char *p = nullptr; size_t cap = 0; hipEvent_t done = nullptr; };
     Buf bufs[2];
