@@ -272,8 +272,12 @@ __device__ inline float sensory_current(const Pool &p, int64_t i, NeuronRegs &s,
 // `arrived` = false: no incoming synapse of neuron i arrived this tick, so
 // every release it would sum is +0 and its sums are +0 without reading them
 // (sums start at +0; +0 + +0 = +0) and no modulator bit is set.
+//
+// still (when not null) receives whether this tick changed no state of an
+// AdEx cell without sensory input (neuron_tick), so that its next tick with
+// no arrival is the identity.
 __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, const float *input,
-                                  float gain, bool arrived = true) {
+                                  float gain, bool arrived = true, bool *still = nullptr) {
     uint32_t flags = p.nflags[i];
     if (!(flags & NF_ACTIVE)) {
         uint32_t h = p.hist[i];
@@ -293,7 +297,8 @@ __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, co
     float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
     const unsigned mod = arrived ? gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab) : 0u;
     float current = sensory_current(p, i, s, g, input, gain);
-    int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, g_exp_tab);
+    int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, g_exp_tab, still);
+    if (still) *still = *still && p.ninput[i] < 0 && (flags & NF_TYPE) != 6;
     store_neuron(p, i, s);
     p.spike[i] = uint8_t(spike);
     p.hist[i] = (p.hist[i] << 1) | uint32_t(spike);
@@ -439,45 +444,149 @@ __global__ void k_homeostasis_world(Pool p, int64_t n, const int32_t *age) {
 // Resident form
 // ---------------------------------------------------------------------------
 
+// k plasticity steps of a pair synapse for ticks with no arrival, no spike of
+// its post neuron and no modulator at it: plasticity_step with those inputs
+// (physiology.cuh:459-555) decays the two traces, advances the ages by one
+// and bounds the weight, each variable on its own.  So the steps are replayed
+// variable by variable: each trace by its k multiplications in order; an age
+// that is a whole number below 2^24 - 128 by one addition of k (every
+// intermediate sum is then a float exactly, so the k additions of 1 give the
+// same value), any other age by k additions; the weight by one bound, which
+// is idempotent.  Profiles 4 and 5 count their dopamine / structural age up
+// to a reset value instead of decaying a trace.
+// Most quiet steps a synapse defers before it replays them (7 bits).
+constexpr uint32_t kQuietMax = 0x7f;
+
+__device__ inline float add_steps(float age, uint32_t k) {
+    if (age >= 0.0f && age <= 16777088.0f && age == __builtin_truncf(age)) return age + float(k);
+    for (uint32_t i = 0; i < k; ++i) age += 1.0f;
+    return age;
+}
+
+__device__ inline float count_steps(float x, uint32_t k, float reset_above) {
+    for (uint32_t i = 0; i < k && x >= 0.0f; ++i) {
+        x += 1.0f;
+        if (x > reset_above) x = -1.0f;
+    }
+    return x;
+}
+
+__device__ inline void replay_quiet_steps(PlastRegs &r, int profile, float w0, uint32_t k) {
+    if (!k) return;
+    r.pre_age = add_steps(r.pre_age, k);
+    if (profile == 5) {
+        r.x = count_steps(r.x, k, 5000.0f);
+        return;
+    }
+    if (profile == 4) {
+        r.x = count_steps(r.x, k, 20000.0f);
+    } else {
+        const float dx = profile == 3 ? kDecayEligLtp : kDecayLtp;
+        const float dy = profile == 3 ? kDecayEligLtd : kDecayLtd;
+        for (uint32_t i = 0; i < k; ++i) {
+            r.x *= dx;
+            r.y *= dy;
+        }
+        r.post_age = add_steps(r.post_age, k);
+    }
+    r.weight = clamp_range(r.weight, 0.0f, 2.0f * w0);
+}
+
+
+// Plasticity of tick t-1 of big-brain synapse q with its quiet steps
+// deferred (the rule of k_resident_lds phase A): a pair synapse whose step
+// has no input (no arrival at t-1, no spike of its post neuron, no modulator
+// at it) and no arrival at t counts the step in `pend` instead; the next step
+// with an input, the 127th deferred step and the call's last step replay the
+// deferred ones first.
+__device__ inline void plasticity_deferred(const Pool &p, int64_t q, int32_t b, bool arr_now, bool last,
+                                           uint8_t &pend) {
+    if (p.bfrozen[b]) return;
+    const uint32_t flags = p.sflags[q];
+    const uint32_t route = flags >> SF_ROUTE_SHIFT;
+    const int profile = (flags >> SF_PLAST_SHIFT) & 7;
+    if (!(route & R_VALID) || !(route & R_PAIR) || !profile) return;
+    const int32_t post = p.spost[q];
+    const bool arrived = p.arrived[q] != 0, spiked = p.spike[post] != 0;
+    const unsigned pm = p.modbits[post], modrec = (flags >> SF_MODREC_SHIFT) & 7;
+    if (!last && !arrived && !spiked && !(pm & modrec) && !arr_now && pend < kQuietMax) {
+        ++pend;
+        return;
+    }
+    PlastRegs r;
+    load_plast(p, q, profile, r);
+    const float w0 = p.sg[SG_W0 * p.scap + q];
+    replay_quiet_steps(r, profile, w0, pend);
+    plasticity_step(r, profile, modrec, w0, arrived, spiked, p.ns[NS_DEPOL * p.ncap + post], pm, g_exp_tab);
+    store_plast(p, q, profile, r);
+    pend = 0;
+}
+
+// Whether synapse q of brain b arrives at tick t of the call.
+__device__ inline bool arrives(const Pool &p, int64_t q, int32_t b, int t) {
+    const uint32_t flags = p.sflags[q];
+    if (!((flags >> SF_ROUTE_SHIFT) & R_VALID)) return false;
+    const int d = flags & SF_DELAY;
+    return p.btick[b] + t >= d && ((p.hist[p.spre[q]] >> (d - 1)) & 1u);
+}
+
 // A brain too large for a pack: one workgroup runs all its ticks, with the
 // per-tick phases separated by workgroup barriers instead of kernel launches
 // and its state kept in global memory (L2-resident for the call).
+//
+// `lds` (hit_cap bytes; may be null) holds, when the brain fits, a byte per
+// neuron and then a byte per synapse.  The neuron byte: bit 0, a synapse onto
+// it arrived this tick (set by the delivery pass; the neuron pass skips the
+// input gather of neurons without one); bit 1, its last tick changed no state
+// and it is an AdEx cell without sensory input (see neuron_tick): with no
+// arrival its tick is the identity, and the neuron pass only shifts its spike
+// register.  The synapse byte counts its deferred quiet plasticity steps.
 template <int WG>
-// `hit` (LDS, hit_cap bytes; may be null) holds one arrival flag per neuron
-// of the brain when the brain fits: the delivery pass sets the flags of the
-// posts it reaches, the neuron pass reads and clears them and skips the
-// input gather of neurons without one.
 __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float *input,
-                                float gain, float *out, int homeostasis, uint8_t *hit, int64_t hit_cap) {
-    __shared__ unsigned big_events;
+                                float gain, float *out, int homeostasis, uint8_t *lds, int64_t hit_cap) {
+    __shared__ unsigned big_events, big_skipped;
     if (!(p.brun[b] & 1)) return;
     const int tid = threadIdx.x;
     const int64_t n0 = p.bn0[b], n1 = n0 + p.bnc[b];
     const int64_t s0 = p.bs0[b], s1 = s0 + p.bsc[b];
-    if (!hit || n1 - n0 > hit_cap) hit = nullptr;
-    if (hit)
-        for (int64_t i = tid; i < n1 - n0; i += WG) hit[i] = 0;
-    if (tid == 0) big_events = 0;
+    if (!lds || (n1 - n0) + (s1 - s0) > hit_cap) lds = nullptr;
+    uint8_t *hit = lds, *pend = lds ? lds + (n1 - n0) : nullptr;
+    if (lds)
+        for (int64_t i = tid; i < (n1 - n0) + (s1 - s0); i += WG) lds[i] = 0;
+    if (tid == 0) big_events = big_skipped = 0;
     __syncthreads();
-    unsigned arrivals = 0;
+    unsigned arrivals = 0, skipped = 0;
     // Per tick: plasticity of tick t-1 and delivery of tick t in one synapse
     // pass (a synapse's plasticity precedes its next delivery in its own
     // thread, the reference's order), then the neurons of tick t.
     for (int t = 0; t < ticks; ++t) {
         for (int64_t q = s0 + tid; q < s1; q += WG) {
-            if (t > 0) plasticity_one(p, q, b);
+            if (lds) {
+                if (t > 0) plasticity_deferred(p, q, b, arrives(p, q, b, t), false, pend[q - s0]);
+            } else if (t > 0) {
+                plasticity_one(p, q, b);
+            }
             const unsigned arr = deliver_one(p, q, b, t);
-            if (arr && hit) hit[p.spost[q] - n0] = 1;
+            if (arr && hit) hit[p.spost[q] - n0] |= 1;
             arrivals += arr;
         }
         __syncthreads();
         for (int64_t i = n0 + tid; i < n1; i += WG) {
-            bool arrived = true;
-            if (hit) {
-                arrived = hit[i - n0] != 0;
-                hit[i - n0] = 0;
+            if (!hit) {
+                neuron_one(p, i, b, t, input, gain, true);
+                continue;
             }
-            neuron_one(p, i, b, t, input, gain, arrived);
+            const uint8_t h = hit[i - n0];
+            if (h == 2) {
+                p.hist[i] <<= 1;
+                p.spike[i] = 0;
+                p.modbits[i] = 0;
+                ++skipped;
+                continue;
+            }
+            bool still = false;
+            neuron_one(p, i, b, t, input, gain, h & 1, &still);
+            hit[i - n0] = still ? 2 : 0;
         }
         __syncthreads();
         for (int m = tid; m < p.outputs; m += WG) {
@@ -486,7 +595,10 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
         }
     }
     if (ticks > 0)
-        for (int64_t q = s0 + tid; q < s1; q += WG) plasticity_one(p, q, b);
+        for (int64_t q = s0 + tid; q < s1; q += WG) {
+            if (lds) plasticity_deferred(p, q, b, false, true, pend[q - s0]);
+            else plasticity_one(p, q, b);
+        }
     __syncthreads();
     if (homeostasis) {
         const int64_t tick = p.btick[b] + ticks;
@@ -498,10 +610,12 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
         }
     }
     if (arrivals) atomicAdd(&big_events, arrivals);
+    if (skipped) atomicAdd(&big_skipped, skipped);
     __syncthreads();
     if (tid == 0) {
         p.btick[b] += ticks;
         if (big_events) atomicAdd(&p.events[0], (unsigned long long)big_events);
+        if (big_skipped) atomicAdd(&p.events[2], (unsigned long long)big_skipped);
     }
 }
 
@@ -830,52 +944,7 @@ constexpr uint32_t kPpMask = 0x7ffu;
 constexpr uint32_t kArrNone = 0, kArrDirect = 0x101, kArrSpill = 0x102, kArrMask = 0x1ff;
 constexpr int kRelCap = 256;
 constexpr int kPendShift = 9;
-constexpr uint32_t kPendMax = 0x7f;
-
-// k plasticity steps of a pair synapse for ticks with no arrival, no spike of
-// its post neuron and no modulator at it: plasticity_step with those inputs
-// (physiology.cuh:459-555) decays the two traces, advances the ages by one
-// and bounds the weight, each variable on its own.  So the steps are replayed
-// variable by variable: each trace by its k multiplications in order; an age
-// that is a whole number below 2^24 - 128 by one addition of k (every
-// intermediate sum is then a float exactly, so the k additions of 1 give the
-// same value), any other age by k additions; the weight by one bound, which
-// is idempotent.  Profiles 4 and 5 count their dopamine / structural age up
-// to a reset value instead of decaying a trace.
-__device__ inline float add_steps(float age, uint32_t k) {
-    if (age >= 0.0f && age <= 16777088.0f && age == __builtin_truncf(age)) return age + float(k);
-    for (uint32_t i = 0; i < k; ++i) age += 1.0f;
-    return age;
-}
-
-__device__ inline float count_steps(float x, uint32_t k, float reset_above) {
-    for (uint32_t i = 0; i < k && x >= 0.0f; ++i) {
-        x += 1.0f;
-        if (x > reset_above) x = -1.0f;
-    }
-    return x;
-}
-
-__device__ inline void replay_quiet_steps(PlastRegs &r, int profile, float w0, uint32_t k) {
-    if (!k) return;
-    r.pre_age = add_steps(r.pre_age, k);
-    if (profile == 5) {
-        r.x = count_steps(r.x, k, 5000.0f);
-        return;
-    }
-    if (profile == 4) {
-        r.x = count_steps(r.x, k, 20000.0f);
-    } else {
-        const float dx = profile == 3 ? kDecayEligLtp : kDecayLtp;
-        const float dy = profile == 3 ? kDecayEligLtd : kDecayLtd;
-        for (uint32_t i = 0; i < k; ++i) {
-            r.x *= dx;
-            r.y *= dy;
-        }
-        r.post_age = add_steps(r.post_age, k);
-    }
-    r.weight = clamp_range(r.weight, 0.0f, 2.0f * w0);
-}
+constexpr uint32_t kPendMax = kQuietMax;
 
 // A neuron's LDS modulator byte (mod_l) holds the tick's modulator bits
 // (kModBits); in bit 6, whether a synapse onto it arrived this tick (set in
