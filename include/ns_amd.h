@@ -546,9 +546,9 @@ int ns_serve_poll(void *server, ns_serve_result *out, int capacity);
 int ns_serve_drain(void *server);
 
 typedef struct ns_serve_stats {
-    int64_t submitted, admitted, finished, launches;
+    int64_t submitted, admitted, finished, launches;   /* launches: launch rounds */
     int64_t resident_peak;
-    double busy_s;
+    double busy_s;              /* device time of the launch rounds (HIP events) */
 } ns_serve_stats;
 int ns_serve_stats_get(void *server, ns_serve_stats *stats);
 

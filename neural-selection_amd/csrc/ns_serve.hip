@@ -292,7 +292,18 @@ struct Server {
     float *d_input = nullptr, *d_out = nullptr, *d_task = nullptr;
     uint32_t *d_rng = nullptr;
     uint8_t *d_done = nullptr;                           // per slot: request finished
-    std::vector<uint8_t> h_done;
+    // A launch round's outcome (done flags and task rows of every slot),
+    // copied to pinned host memory behind the round on the serving stream;
+    // double buffered, so that round r's copies are read while round r + 1
+    // runs.  begin/end bracket the round's device work.
+    struct RoundOut {
+        uint8_t *done = nullptr;
+        float *task = nullptr;
+        hipEvent_t begin = nullptr, end = nullptr;
+        int64_t round = -1;
+    };
+    RoundOut outs[2];
+    std::vector<int64_t> admit_round;                    // by slot: the round its occupant started in
     int channels = kChannels, outputs = kOutputs;        // of the task
     bool maze = false;
     hipStream_t stream = nullptr;
@@ -341,7 +352,13 @@ struct Server {
         NS_HIP(hipMemset(d_task, 0, sizeof(float) * size_t(rows) * kTaskWords));
         NS_HIP(hipMemset(d_input, 0, sizeof(float) * size_t(rows) * channels));
         NS_HIP(hipMemset(d_done, 0, size_t(rows)));
-        h_done.assign(pool.nb, 0);
+        for (RoundOut &o : outs) {
+            NS_HIP(hipHostMalloc(reinterpret_cast<void **>(&o.done), size_t(rows)));
+            NS_HIP(hipHostMalloc(reinterpret_cast<void **>(&o.task), sizeof(float) * size_t(rows) * kTaskWords));
+            NS_HIP(hipEventCreate(&o.begin));
+            NS_HIP(hipEventCreate(&o.end));
+        }
+        admit_round.assign(pool.nb, 0);
         live.clear();
         live.resize(pool.nb);
         for (int pb = 0; pb < pool.nb; ++pb)
@@ -447,16 +464,19 @@ struct Server {
         NS_HIP(hipGetLastError());
     }
 
-    // Collect finished episodes (done flags read after the last launch
-    // round) and free their slots.  Returns whether any slot was freed (its
-    // run flag must be cleared).
-    bool finish() {
+    // Collect the episodes a completed round finished (slots whose occupant
+    // started in that round or before: a slot admitted since holds a new
+    // episode, whose flag in the round's copy is its predecessor's) and free
+    // their slots.  A finished episode's task row no longer changes, so the
+    // round's copy holds its result even while later rounds still step its
+    // brain.  Returns whether any slot was freed (its run flag must be
+    // cleared).
+    bool finish(const RoundOut &o) {
         std::vector<int> ended;
         for (int pb = 0; pb < pool.nb; ++pb)
-            if (live[pb] && h_done[pb]) ended.push_back(pb);
+            if (live[pb] && admit_round[pb] <= o.round && o.done[pb]) ended.push_back(pb);
         if (ended.empty()) return false;
-        std::vector<float> task(size_t(pool.nb) * kTaskWords);
-        NS_HIP(hipMemcpy(task.data(), d_task, sizeof(float) * task.size(), hipMemcpyDeviceToHost));
+        const float *task = o.task;
         const double t = now();
         std::vector<std::pair<int, const BrainSpec *>> vacate;
         for (int pb : ended) vacate.push_back({pb, nullptr});
@@ -498,34 +518,56 @@ struct Server {
         return true;
     }
 
+    // Launch rounds back to back: while round r runs on the device, the host
+    // collects round r - 1's finished episodes, admits queued requests into
+    // the slots they freed and stages round r + 1's newcomers, so admission
+    // and collection overlap device work.  A finished episode's brain may run
+    // one round more (its run flag clears at the next upload); its result is
+    // already fixed.
     void loop() {
         try {
             bool freed = false;
+            int64_t round = 0;
+            int inflight = -1;   // output buffer of the round in flight, or -1
             while (true) {
                 {
                     std::unique_lock<std::mutex> lock(mu);
                     bool any_live = false;
                     for (const auto &e : live) any_live = any_live || bool(e);
-                    if (!any_live)
+                    if (!any_live && inflight < 0)
                         cv.wait(lock, [&] { return stop || !queue.empty(); });
                     if (stop) break;
                 }
-                const auto busy0 = std::chrono::steady_clock::now();
                 std::vector<Episode *> fresh = admit();
+                for (Episode *e : fresh) admit_round[e->pb] = round;
                 start(fresh, freed);
+                freed = false;
                 int resident = 0;
                 for (const auto &e : live) resident += bool(e);
                 if (resident > stats.resident_peak) stats.resident_peak = resident;
+                int launched = -1;
                 if (resident) {
-                    const int n = cfg.steps_per_launch;
-                    steps(n);
-                    NS_HIP(hipMemcpyAsync(h_done.data(), d_done, size_t(pool.nb),
+                    RoundOut &o = outs[round & 1];
+                    NS_HIP(hipEventRecord(o.begin, stream));
+                    steps(cfg.steps_per_launch);
+                    NS_HIP(hipMemcpyAsync(o.done, d_done, size_t(pool.nb), hipMemcpyDeviceToHost, stream));
+                    NS_HIP(hipMemcpyAsync(o.task, d_task, sizeof(float) * size_t(pool.nb) * kTaskWords,
                                           hipMemcpyDeviceToHost, stream));
-                    NS_HIP(hipStreamSynchronize(stream));
+                    NS_HIP(hipEventRecord(o.end, stream));
+                    o.round = round;
                     stats.launches += 1;
+                    launched = int(round & 1);
+                    ++round;
                 }
-                stats.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy0).count();
-                freed = finish();
+                if (inflight >= 0) {
+                    const RoundOut &o = outs[inflight];
+                    NS_HIP(hipEventSynchronize(o.end));
+                    float ms = 0;
+                    NS_HIP(hipEventElapsedTime(&ms, o.begin, o.end));
+                    stats.busy_s += 1e-3 * ms;
+                    freed = finish(o);
+                }
+                inflight = launched;
             }
         } catch (const std::exception &e) {
             std::lock_guard<std::mutex> lock(mu);
@@ -534,6 +576,15 @@ struct Server {
             cv.notify_all();
         }
         if (stream) (void)hipStreamSynchronize(stream);
+    }
+
+    ~Server() {
+        for (RoundOut &o : outs) {
+            if (o.done) (void)hipHostFree(o.done);
+            if (o.task) (void)hipHostFree(o.task);
+            if (o.begin) (void)hipEventDestroy(o.begin);
+            if (o.end) (void)hipEventDestroy(o.end);
+        }
     }
 };
 }  // namespace

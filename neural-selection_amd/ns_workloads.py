@@ -159,6 +159,22 @@ def c1(seed=2, brains=64, neurons=32, calls=200):
                      8, -1.0, 0.5, None, 0.0, seed, POISSON_20HZ_16MS, 230.0, 16, calls)
 
 
+POISSON_20HZ_1MS = float(1.0 - np.exp(-20.0 * 0.001))
+
+
+def c1_per_tick(seed=2, brains=64, neurons=32, ticks=3200):
+    """C1 with its Poisson 20 Hz sensory input redrawn every 1 ms tick: the
+    same genomes, one tick per call (the batched ABI takes one input per
+    call), each sensor on with probability 1 - e^-0.02 per tick; 3,200
+    ticks are C1's 200 steps of 16.  Homeostasis then runs after every tick,
+    as the reference's gym adapter does after every call."""
+    w = c1(seed, brains, neurons, calls=ticks)
+    w.name = 'c1-per-tick'
+    w.input_p = POISSON_20HZ_1MS
+    w.ticks = 1
+    return w
+
+
 def _stdp_mix(rng, m):
     u = rng.random(m)
     return np.where(u < 0.05, 5, np.where(u < 0.10, 4, np.where(u < 0.20, 3,

@@ -39,6 +39,14 @@ def test_c1_plain_tick_bit_exact(form):
     assert info['events'] > 0
 
 
+def test_c1_per_tick_poisson_bit_exact():
+    """C1's genomes with the sensory input redrawn every 1 ms tick (Poisson
+    20 Hz per sensor), one tick per call: 800 ticks of 16 brains."""
+    w = W.c1_per_tick(brains=16, ticks=800)
+    info = _run_compare(w, ns_amd.FORM_RESIDENT, calls=800, check_every=200)
+    assert info['events'] > 0
+
+
 @pytest.mark.parametrize('form', [ns_amd.FORM_RESIDENT, ns_amd.FORM_PER_TICK])
 @pytest.mark.parametrize('seed,types', [(11, (0, 1, 4, 5, 6)), (12, (0, 1, 2, 3, 4, 5, 6))])
 def test_mixed_plastic_brains_bit_exact(form, seed, types):
