@@ -1136,8 +1136,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __syncthreads();
 
     unsigned my_events = 0, my_skipped = 0;
-    // bit k: neuron slot k's last tick changed no state (see phase B)
-    uint32_t still = 0;
+    // bit k: neuron slot k's last tick changed no state (see phase B); and
+    // whether any tick of this call changed it (else write-back skips it)
+    uint32_t still = 0, dirty = 0;
     for (int t = 0; t < ticks; ++t) {
         // Phase A: plasticity of tick t-1, arrivals of tick t; the arrivals
         // that take a short-term release are noted (one bit per round) and
@@ -1265,7 +1266,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             bool same = false;
             const int spike = neuron_tick(nr[k], nc[k], current, e0, e1, e2, e3, inh, p.cell, N,
                                           n0 + n, etab, &same);
-            still = (still & ~(1u << k)) | (uint32_t(same && (flags & kSkipCell)) << k);
+            const bool unchanged = same && (flags & kSkipCell);
+            still = (still & ~(1u << k)) | (uint32_t(unchanged) << k);
+            dirty |= uint32_t(!unchanged) << k;
             hist_l[n] = (hist_l[n] << 1) | uint32_t(spike);
             mod_l[n] = uint8_t(mod | (nr[k].depol >= kDepolGateMv ? kDepolGate : 0u));
             if (!isfinite(nr[k].v)) p.bstatus[pk.b0 + nbr[k]] = -1;
@@ -1306,7 +1309,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             p.ns[NS_LAS * N + i] = homeostasis_las(p.ns[NS_LAS * N + i], float(ticks), hz);
         }
         if (nc[k].flags & NF_ACTIVE) {
-            store_neuron(p, i, nr[k]);
+            if ((dirty >> k) & 1u) store_neuron(p, i, nr[k]);   // else the loaded state, unchanged
             p.spike[i] = uint8_t(hist_l[n] & 1u);
             p.modbits[i] = mod_l[n] & kModBits;
         } else if (!(nc[k].flags & NF_WORLD)) {
@@ -1318,8 +1321,13 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         if (!brun_l[pp_l[q] >> kPpBrain]) continue;
         const int64_t gq = s0 + q;
         if (PLASTIC) {
-            PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
-            store_plast(p, gq, (fl_l[q] >> SF_PLAST_SHIFT) & 7, r);
+            // only the synapses plasticity runs on change their traces
+            const uint32_t f = fl_l[q], route = f >> SF_ROUTE_SHIFT;
+            const int profile = (f >> SF_PLAST_SHIFT) & 7;
+            if ((route & R_VALID) && (route & R_PAIR) && profile && !bfrozen_l[pp_l[q] >> kPpBrain]) {
+                PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
+                store_plast(p, gq, profile, r);
+            }
         }
         p.arrived[gq] = uint8_t((ai_l[q] & kArrMask) != kArrNone);
     }
