@@ -710,6 +710,45 @@ void HostWorkers::run(int n, const std::function<void(int)> &fn) {
     if (err) std::rethrow_exception(err);
 }
 
+GenomeReaper &GenomeReaper::get() {
+    static GenomeReaper reaper;
+    return reaper;
+}
+
+GenomeReaper::GenomeReaper() : thread_([this] { loop(); }) {}
+
+GenomeReaper::~GenomeReaper() {
+    {
+        std::lock_guard<std::mutex> g(m_);
+        stop_ = true;
+    }
+    wake_.notify_all();
+    thread_.join();
+}
+
+void GenomeReaper::retire(std::vector<std::shared_ptr<const HostBrain>> &&genomes) {
+    if (genomes.empty()) return;
+    {
+        std::lock_guard<std::mutex> g(m_);
+        for (auto &p : genomes) pending_.push_back(std::move(p));
+    }
+    genomes.clear();
+    wake_.notify_one();
+}
+
+void GenomeReaper::loop() {
+    std::vector<std::shared_ptr<const HostBrain>> batch;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> g(m_);
+            wake_.wait(g, [&] { return stop_ || !pending_.empty(); });
+            if (pending_.empty()) return;   // stopping, nothing left
+            batch.swap(pending_);
+        }
+        batch.clear();   // the last references: the genomes are freed here
+    }
+}
+
 void DevicePool::upload_stage(const Stager &sg, hipStream_t st) {
     const size_t bytes = sg.bytes();
     PinnedStaging::Buf &buf = staging.take(bytes);
@@ -1221,20 +1260,20 @@ Tile default_tile(const std::vector<const HostBrain *> &brains) {
     return Tile{1024, 1, 8, 1, 0};
 }
 
-void Batch::create(std::vector<HostBrain> &&owned, int) {
-    brains = std::move(owned);
+void Batch::create(std::vector<std::shared_ptr<const HostBrain>> &&genomes, int) {
+    brains = std::move(genomes);
     count = int(brains.size());
-    channels = brains[0].channels;
-    outputs = int(brains[0].motors.size());
+    channels = brains[0]->channels;
+    outputs = int(brains[0]->motors.size());
     for (const auto &b : brains)
-        if (b.channels != channels || int(b.motors.size()) != outputs)
+        if (b->channels != channels || int(b->motors.size()) != outputs)
             throw std::runtime_error("Batch ports differ");
     NS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     specs.clear();
     for (int i = 0; i < count; ++i)
-        specs.push_back(BrainSpec{&brains[i], i, i * channels, i * outputs, world});
+        specs.push_back(BrainSpec{brains[i].get(), i, i * channels, i * outputs, world});
     std::vector<const HostBrain *> views;
-    for (const auto &b : brains) views.push_back(&b);
+    for (const auto &b : brains) views.push_back(b.get());
     const Tile t0 = default_tile(views);
     pool.growable = true;                    // room for slots appended by replacements
     pool.build(slot_specs(t0), outputs, t0, stream);
@@ -1304,15 +1343,15 @@ std::vector<BrainSpec> Batch::slot_specs(const Tile &t) const {
     return reserve_slots(specs, slack, spare, t, world);
 }
 
-void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const uint32_t *seeds,
-                    hipStream_t st) {
+void Batch::replace(int k, const int *idx, std::vector<std::shared_ptr<const HostBrain>> &&fresh,
+                    const uint32_t *seeds, hipStream_t st) {
     if (k <= 0) return;
     if (int(fresh.size()) != k) throw std::runtime_error("Replacement count differs");
     std::vector<uint8_t> seen(count, 0);
     for (int j = 0; j < k; ++j) {
         if (idx[j] < 0 || idx[j] >= count || seen[idx[j]]) throw std::runtime_error("Invalid brain index");
         seen[idx[j]] = 1;
-        if (fresh[j].channels != channels || int(fresh[j].motors.size()) != outputs)
+        if (fresh[j]->channels != channels || int(fresh[j]->motors.size()) != outputs)
             throw std::runtime_error("Batch ports differ");
     }
     std::vector<int> old_slot(k), new_slot(k, -1);
@@ -1320,21 +1359,30 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
     stage.clear();
     stage_keep.clear();
     bool rebuild = false;
-    for (int j = 0; j < k && !rebuild; ++j)
-        if (has_slow_cell(fresh[j]) || (!pool.plastic && has_plastic_pair(fresh[j]))) rebuild = true;
+    for (int j = 0; j < k && !rebuild; ++j) {
+        const HostBrain &f = *fresh[j];
+        const bool slow = f.slow_cell >= 0 ? f.slow_cell != 0 : has_slow_cell(f);
+        if (slow || (!pool.plastic && has_plastic_pair(f))) rebuild = true;
+    }
+    // the free set is valid again only once this call has placed its brains
+    const bool kept_free = free_gen == pool.generation;
+    free_gen = ~uint64_t(0);
     if (!rebuild) {
         // best fit by neuron capacity among free slots (spares, earlier
         // vacancies and the replaced brains' own slots)
-        std::set<std::tuple<int, int, int>> freeset;
-        for (int pb = 0; pb < pool.nb; ++pb)
-            if (pool.batch_of[pb] < 0 && pool.slot_kind[pb] != 0)
-                freeset.insert({pool.bnc[pb], pool.bsc[pb], pb});
+        std::set<std::tuple<int, int, int>> &freeset = free_fit;
+        if (!kept_free) {
+            freeset.clear();
+            for (int pb = 0; pb < pool.nb; ++pb)
+                if (pool.batch_of[pb] < 0 && pool.slot_kind[pb] != 0)
+                    freeset.insert({pool.bnc[pb], pool.bsc[pb], pb});
+        }
         for (int j = 0; j < k; ++j)
             if (pool.slot_kind[old_slot[j]] != 0)
                 freeset.insert({pool.bnc[old_slot[j]], pool.bsc[old_slot[j]], old_slot[j]});
         std::vector<int> homeless;
         for (int j = 0; j < k; ++j) {
-            const int n = int(fresh[j].genes.size()), e = int(fresh[j].edges.size());
+            const int n = int(fresh[j]->genes.size()), e = int(fresh[j]->edges.size());
             auto it = freeset.lower_bound({n, e, -1});
             while (it != freeset.end() && std::get<1>(*it) < e) ++it;
             if (it == freeset.end()) { homeless.push_back(j); continue; }
@@ -1350,7 +1398,7 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
             const int smax = pool.tile.ks > 0 ? pool.tile.threads * pool.tile.ks : pool.smax_lds;
             std::vector<std::pair<int, int>> caps, occ;
             for (int j : homeless) {
-                const int n = int(fresh[j].genes.size()), e = int(fresh[j].edges.size());
+                const int n = int(fresh[j]->genes.size()), e = int(fresh[j]->edges.size());
                 int cn = ladder(n, slack), cs = ladder(e, slack);
                 if (n <= nmax && e <= smax) { cn = std::min(cn, nmax); cs = std::min(cs, smax); }
                 caps.push_back({std::max(cn, 1), cs});
@@ -1375,12 +1423,15 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
         pool.download_state(on, os, oh, ot, &ost);
         std::vector<int64_t> noff(count + 1, 0), soff(count + 1, 0);
         for (int i = 0; i < count; ++i) {
-            noff[i + 1] = noff[i] + int64_t(brains[i].genes.size());
-            soff[i + 1] = soff[i] + int64_t(brains[i].edges.size());
+            noff[i + 1] = noff[i] + int64_t(brains[i]->genes.size());
+            soff[i + 1] = soff[i] + int64_t(brains[i]->edges.size());
         }
-        for (int j = 0; j < k; ++j) brains[idx[j]] = std::move(fresh[j]);
+        for (int j = 0; j < k; ++j) {
+            brains[idx[j]] = std::move(fresh[j]);
+            specs[idx[j]].brain = brains[idx[j]].get();
+        }
         for (int i = 0; i < count; ++i) {
-            const int64_t n = int64_t(brains[i].genes.size()), e = int64_t(brains[i].edges.size());
+            const int64_t n = int64_t(brains[i]->genes.size()), e = int64_t(brains[i]->edges.size());
             if (seen[i]) {
                 nn_.insert(nn_.end(), size_t(n), ns_neuron_state{});
                 ns_.insert(ns_.end(), size_t(e), ns_synapse_state{});
@@ -1399,7 +1450,14 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
         NS_HIP(hipStreamSynchronize(stream));
         ++relayouts;
     } else {
-        for (int j = 0; j < k; ++j) brains[idx[j]] = std::move(fresh[j]);
+        // the replaced genomes are released on the reaper's thread
+        std::vector<std::shared_ptr<const HostBrain>> retired(static_cast<size_t>(k));
+        for (int j = 0; j < k; ++j) {
+            retired[size_t(j)] = std::move(brains[idx[j]]);
+            brains[idx[j]] = std::move(fresh[j]);
+            specs[idx[j]].brain = brains[idx[j]].get();
+        }
+        GenomeReaper::get().retire(std::move(retired));
         std::vector<std::pair<int, const BrainSpec *>> moves;
         std::vector<uint8_t> reused(pool.nb, 0);
         for (int j = 0; j < k; ++j) {
@@ -1410,6 +1468,7 @@ void Batch::replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const
             if (!reused[old_slot[j]]) moves.push_back({old_slot[j], nullptr});
         pool.place(moves, stage);
         placements += k;
+        free_gen = pool.generation;   // appended slots, if any, are all taken
     }
     // Fresh state for the newborns' slots (initial states seeded as
     // ns_gpu_reset) and the run flags of the last mask under the new slot
@@ -1514,8 +1573,7 @@ void Batch::run(int ticks, const float *d_in, float gain, float *d_out, int home
 void run_pool(DevicePool &pool, int form, int ticks, const float *d_in, float gain, float *d_out,
               int64_t out_count, int homeostasis, hipStream_t st, KernelTimer *timer) {
     hipEvent_t ev;
-    launch_fill(d_out, out_count, 0.0f, st);
-    launch_fill_i32(pool.view.bstatus, pool.nb, 0, st);
+    launch_clear_step(d_out, out_count, pool.view.bstatus, pool.nb, st);
     if (form == NS_FORM_PER_TICK) {
         if (timer) timer->begin(st, 1, ev);
         launch_per_tick(pool.view, pool.all_range, ticks, 1, d_in, gain, d_out, homeostasis, st);
@@ -1583,14 +1641,17 @@ void *ns_create_ex(int count, int edges, int channels, int outputs, const float 
 }
 
 void ns_original_indices(void *p, const uint32_t *neurons, const uint32_t *synapses) {
-    auto &b = handle_brain(p);
+    auto &h = *static_cast<GenomeHandle *>(p);
+    // a genome a batch shares stays as the batch took it
+    if (h.genome.use_count() > 1) h.genome = std::make_shared<HostBrain>(*h.genome);
+    HostBrain &b = *h.genome;
     b.orig_n.assign(neurons, neurons + b.genes.size());
     if (!b.edges.empty()) b.orig_s.assign(synapses, synapses + b.edges.size());
 }
 
 void ns_destroy(void *p) { delete static_cast<GenomeHandle *>(p); }
 
-static void *make_batch(std::vector<HostBrain> &&brains, bool world) {
+static void *make_batch(std::vector<std::shared_ptr<const HostBrain>> &&brains, bool world) {
     auto *batch = new Batch;
     try {
         batch->world = world;
@@ -1605,9 +1666,12 @@ static void *make_batch(std::vector<HostBrain> &&brains, bool world) {
 void *ns_gpu_create(void **handles, int count) {
     try {
         if (count < 1) throw std::runtime_error("Empty GPU batch");
-        std::vector<HostBrain> brains;
+        std::vector<std::shared_ptr<const HostBrain>> brains;
         brains.reserve(count);
-        for (int i = 0; i < count; ++i) brains.push_back(handle_brain(handles[i]));
+        for (int i = 0; i < count; ++i) {
+            if (!handles[i]) throw std::runtime_error("Invalid genome handle");
+            brains.push_back(handle_genome(handles[i]));
+        }
         return make_batch(std::move(brains), false);
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -1620,7 +1684,7 @@ void *ns_gpu_create_bulk(int count, const int *neuron_counts, const int *edge_co
                          int max_delay, int world_rules) {
     try {
         if (count < 1) throw std::runtime_error("Empty GPU batch");
-        std::vector<HostBrain> brains;
+        std::vector<std::shared_ptr<const HostBrain>> brains;
         brains.reserve(count);
         int64_t n0 = 0, s0 = 0;
         for (int i = 0; i < count; ++i) {
@@ -1635,7 +1699,7 @@ void *ns_gpu_create_bulk(int count, const int *neuron_counts, const int *edge_co
                     b->receptor[k] = b->genes[k].type == 5 && b->sensory[k] >= 0 && b->sensory[k] < 6;
                 }
             }
-            brains.push_back(std::move(*b));
+            brains.push_back(std::shared_ptr<const HostBrain>(std::move(b)));
             n0 += neuron_counts[i];
             s0 += edge_counts[i];
         }
@@ -1772,8 +1836,8 @@ int ns_gpu_replace(void *p, int count, const int *indices, void **handles, const
         if (count < 0 || (count && (!indices || !handles))) throw std::runtime_error("Invalid replacement");
         for (int j = 0; j < count; ++j)
             if (!handles[j]) throw std::runtime_error("Invalid genome handle");
-        std::vector<HostBrain> fresh(static_cast<size_t>(count));
-        HostWorkers::get().run(count, [&](int j) { fresh[size_t(j)] = handle_brain(handles[j]); });
+        std::vector<std::shared_ptr<const HostBrain>> fresh(static_cast<size_t>(count));
+        for (int j = 0; j < count; ++j) fresh[size_t(j)] = handle_genome(handles[j]);
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b.stream;
         b.replace(count, indices, std::move(fresh), seeds, st);
         return 0;
@@ -1892,7 +1956,7 @@ int ns_gpu_autotune(void *p, int ticks, int *threads, int *kn, int *ks, int *min
 // for batches whose delays are at most 8 ticks (gym_adapter/core.cuh:18).
 static void check_history9(const Batch &b) {
     int md = 0;
-    for (const auto &hb : b.brains) md = std::max(md, hb.max_delay);
+    for (const auto &hb : b.brains) md = std::max(md, hb->max_delay);
     if (md > 8)
         throw std::runtime_error("Delay bound " + std::to_string(md) +
                                  " exceeds the 9-slot history; use the spike bit registers");

@@ -358,7 +358,7 @@ void *ns_ecology_create(void *batch, const ns_ecology_config *cfg) {
             y[a] = uniform01(rng) * span_y;
             hd[a] = uniform01(rng) * 2.0f * kPi;
             int n = 0;
-            for (const auto &g : b.brains[a].genes) n += g.active != 0;
+            for (const auto &g : b.brains[a]->genes) n += g.active != 0;
             active[a] = n;
         }
         // main.cu:458-462 with the configured density per kind

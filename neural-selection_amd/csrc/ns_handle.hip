@@ -17,11 +17,12 @@ void set_error(const std::string &msg);
 GenomeHandle *make_handle(HostBrain *parsed) {
     std::unique_ptr<HostBrain> b(parsed);
     auto *h = new GenomeHandle;
-    h->brain = std::move(*b);
+    b->slow_cell = int8_t(has_slow_cell(*b));
+    h->genome = std::shared_ptr<HostBrain>(std::move(b));
     // native.cpp:46-82 sizes the state vectors value-initialised (zeros)
-    h->neurons.assign(h->brain.genes.size(), ns_neuron_state{});
-    h->synapses.assign(h->brain.edges.size(), ns_synapse_state{});
-    h->hist.assign(h->brain.genes.size(), 0);
+    h->neurons.assign(h->brain().genes.size(), ns_neuron_state{});
+    h->synapses.assign(h->brain().edges.size(), ns_synapse_state{});
+    h->hist.assign(h->brain().genes.size(), 0);
     return h;
 }
 
@@ -29,7 +30,7 @@ GenomeHandle *make_handle(HostBrain *parsed) {
 // neuron and synapse (physiology.cuh:179-202, 393-404) seeded seed +
 // original * 7919 / + original * 104729, an empty ring.
 static void reset_handle(GenomeHandle &h, uint32_t seed) {
-    const HostBrain &b = h.brain;
+    const HostBrain &b = h.brain();
     h.tick = 0;
     for (size_t i = 0; i < b.genes.size(); ++i) {
         const ns_neuron_gene &g = b.genes[i];
@@ -84,7 +85,7 @@ void build_one_brain(DevicePool &pool, const HostBrain &b, float *&d_in, float *
 // device batch does (gym_adapter/gpu.cu), keeps that state and reports -1.
 void step_one_brain(DevicePool &pool, GenomeHandle &h, float *d_in, float *d_out,
                     const float *input, int ticks, float gain, float *output) {
-    const HostBrain &b = h.brain;
+    const HostBrain &b = h.brain();
     pool.upload_state(h.neurons, h.synapses, h.hist, std::vector<int64_t>{h.tick});
     NS_HIP(hipMemcpy(d_in, input, sizeof(float) * b.channels, hipMemcpyHostToDevice));
     const uint8_t run = uint8_t(1 | (pool.fallback[0] ? 2 : 0));
@@ -114,10 +115,10 @@ void ns_reset(void *p, uint32_t seed) {
 int ns_step(void *p, const float *input, int ticks, float gain, float *output) {
     try {
         auto &h = *static_cast<GenomeHandle *>(p);
-        check_step(h.brain, input, ticks, gain);
+        check_step(h.brain(), input, ticks, gain);
         if (!h.device) {
             auto dev = std::make_unique<HandleDevice>();
-            build_one_brain(dev->pool, h.brain, dev->d_in, dev->d_out);
+            build_one_brain(dev->pool, h.brain(), dev->d_in, dev->d_out);
             h.device = std::move(dev);
         }
         step_one_brain(h.device->pool, h, h.device->d_in, h.device->d_out, input, ticks, gain, output);

@@ -12,8 +12,10 @@
 #include <exception>
 #include <functional>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <thread>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -41,6 +43,7 @@ struct HostBrain {
     std::vector<uint32_t> orig_n, orig_s;
     int channels = 1;
     int max_delay = 8;
+    int8_t slow_cell = -1;   // has_slow_cell, once known (-1: not yet)
 };
 
 ns_neuron_gene profile_gene(int type);
@@ -188,6 +191,25 @@ private:
     uint64_t round_ = 0;
     bool stop_ = false;
     std::exception_ptr error_;   // the first exception of the round, rethrown by run
+};
+
+// Releases retired genomes on a host thread of its own: freeing a replaced
+// brain's genome (a few heap blocks per brain, cold in cache) stays off the
+// thread that places the newborns.
+class GenomeReaper {
+public:
+    static GenomeReaper &get();
+    void retire(std::vector<std::shared_ptr<const HostBrain>> &&genomes);
+    ~GenomeReaper();
+
+private:
+    GenomeReaper();
+    void loop();
+    std::thread thread_;
+    std::mutex m_;
+    std::condition_variable wake_;
+    std::vector<std::shared_ptr<const HostBrain>> pending_;
+    bool stop_ = false;
 };
 
 struct PinnedStaging {
@@ -409,7 +431,10 @@ struct KernelTimer {
 struct Batch {
     DevicePool pool;
     std::vector<BrainSpec> specs;
-    std::vector<HostBrain> brains;   // owned copies of the genome handles
+    // Genomes of the brains in batch order, shared with the genome handles they
+    // came from: a genome is immutable once a batch holds it (a handle that
+    // changes its genome while shared takes a copy first).
+    std::vector<std::shared_ptr<const HostBrain>> brains;
     int count = 0, channels = 0, outputs = 0;
     int form = NS_FORM_RESIDENT;
     bool world = false;
@@ -445,16 +470,20 @@ struct Batch {
     int64_t relayouts = 0, placements = 0;
     std::vector<uint8_t> last_mask;                    // batch order, last run mask
     Stager stage;                                      // reused across replacements
+    // Free slots by (neuron capacity, synapse capacity, slot), kept across
+    // replacements while the layout generation is free_gen.
+    std::set<std::tuple<int, int, int>> free_fit;
+    uint64_t free_gen = ~uint64_t(0);
     std::vector<std::vector<unsigned char>> stage_keep;   // buffers `stage` copies from
 
-    void create(std::vector<HostBrain> &&owned, int max_delay_unused);
+    void create(std::vector<std::shared_ptr<const HostBrain>> &&genomes, int max_delay_unused);
     void relayout(const Tile &tile);
     void set_run(const uint8_t *mask);                 // host mask, batch order
     // Replace brains idx[0..k) by new genomes with fresh state (seeds as
     // ns_gpu_reset): each goes to the best-fitting free slot and the pool is
     // rebuilt only when none fits.  Stream-ordered on `st`.
-    void replace(int k, const int *idx, std::vector<HostBrain> &&fresh, const uint32_t *seeds,
-                 hipStream_t st);
+    void replace(int k, const int *idx, std::vector<std::shared_ptr<const HostBrain>> &&fresh,
+                 const uint32_t *seeds, hipStream_t st);
     // Pool slots for the current brains under the reserve policy.
     std::vector<BrainSpec> slot_specs(const Tile &tile) const;
     void run(int ticks, const float *d_in, float gain, float *d_out, int homeostasis,
@@ -477,14 +506,18 @@ struct HandleDevice {
 // neuron and synapse states, the delay ring as spike shift registers, and the
 // tick.  ns_step's one-brain device pool is made on its first call.
 struct GenomeHandle {
-    HostBrain brain;
+    std::shared_ptr<HostBrain> genome;   // shared with the batches made from the handle
+    const HostBrain &brain() const { return *genome; }
     std::vector<ns_neuron_state> neurons;
     std::vector<ns_synapse_state> synapses;
     std::vector<uint32_t> hist;
     int64_t tick = 0;
     std::unique_ptr<HandleDevice> device;
 };
-inline HostBrain &handle_brain(void *h) { return static_cast<GenomeHandle *>(h)->brain; }
+inline const HostBrain &handle_brain(void *h) { return static_cast<GenomeHandle *>(h)->brain(); }
+inline std::shared_ptr<const HostBrain> handle_genome(void *h) {
+    return static_cast<GenomeHandle *>(h)->genome;
+}
 GenomeHandle *make_handle(HostBrain *parsed);
 // ns_step's argument checks (native.cpp:97-98), the one-brain pool layout and
 // one step on it, shared by the allocating and the static-pool entry points.

@@ -64,6 +64,7 @@ void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *bas
 void launch_pack_order(const uint32_t *nflags, const Pack *packs, const int32_t *list, int count,
                        uint16_t *perm, hipStream_t st);
 void launch_fill(float *x, int64_t n, float v, hipStream_t st);
+void launch_clear_step(float *out, int64_t nout, int32_t *status, int64_t nstatus, hipStream_t st);
 void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st);
 void launch_add_counts(int32_t *total, const float *counts, int64_t n, hipStream_t st);
 void launch_per_tick(const Pool &p, const PlainRange &r, int ticks, uint8_t need,

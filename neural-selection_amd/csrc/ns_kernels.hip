@@ -84,17 +84,20 @@ __global__ void k_reset_synapses(Pool p, int64_t n, const uint32_t *bseed, const
 }
 
 // Reset of the listed slots only: list[0..count) slot indices, list[count..2
-// count) their seeds.  One workgroup per slot, over its whole capacity, and
-// its tick counter back to zero.
+// count) their seeds.  gridDim.y workgroups per slot share its whole capacity
+// (a slot of a few thousand neurons is not left to one workgroup), and the
+// first sets its tick counter back to zero.
 __global__ void k_reset_slots(Pool p, const int32_t *list, int count, const uint32_t *norig,
                               const uint32_t *sorig) {
     const int b = list[blockIdx.x];
     const uint32_t seed = uint32_t(list[count + blockIdx.x]);
     const int64_t n0 = p.bn0[b], n1 = n0 + p.bnc[b];
     const int64_t s0 = p.bs0[b], s1 = s0 + p.bsc[b];
-    for (int64_t i = n0 + threadIdx.x; i < n1; i += blockDim.x) reset_neuron(p, i, seed, norig[i]);
-    for (int64_t q = s0 + threadIdx.x; q < s1; q += blockDim.x) reset_synapse(p, q, seed, sorig[q]);
-    if (threadIdx.x == 0) p.btick[b] = 0;
+    const int64_t first = int64_t(blockIdx.y) * blockDim.x + threadIdx.x;
+    const int64_t stride = int64_t(gridDim.y) * blockDim.x;
+    for (int64_t i = n0 + first; i < n1; i += stride) reset_neuron(p, i, seed, norig[i]);
+    for (int64_t q = s0 + first; q < s1; q += stride) reset_synapse(p, q, seed, sorig[q]);
+    if (blockIdx.y == 0 && threadIdx.x == 0) p.btick[b] = 0;
 }
 
 // Slot placement: each entry copies `bytes` from the staged image (device
@@ -109,7 +112,17 @@ __global__ void k_scatter(const ScatterEntry *entries, int n, const unsigned cha
     if (((en.dst | en.src | en.bytes) & 3u) == 0) {
         uint32_t *d = reinterpret_cast<uint32_t *>(dst);
         const uint32_t *s4 = reinterpret_cast<const uint32_t *>(src);
-        for (uint64_t i = threadIdx.x; i < en.bytes / 4; i += blockDim.x) d[i] = s4[i];
+        const uint64_t words = en.bytes / 4, step = blockDim.x;
+        uint64_t i = threadIdx.x;
+        // four independent loads in flight per thread before their stores
+        for (; i + 3 * step < words; i += 4 * step) {
+            const uint32_t a = s4[i], b = s4[i + step], c = s4[i + 2 * step], e4 = s4[i + 3 * step];
+            d[i] = a;
+            d[i + step] = b;
+            d[i + 2 * step] = c;
+            d[i + 3 * step] = e4;
+        }
+        for (; i < words; i += step) d[i] = s4[i];
     } else {
         for (uint64_t i = threadIdx.x; i < en.bytes; i += blockDim.x) dst[i] = src[i];
     }
@@ -174,6 +187,12 @@ __global__ void k_fill_f32(float *x, int64_t n, float v) {
 __global__ void k_fill_i32(int32_t *x, int64_t n, int32_t v) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) x[i] = v;
+}
+// The clears before a step in one launch: outputs to +0, statuses to 0.
+__global__ void k_clear_step(float *out, int64_t nout, int32_t *status, int64_t nstatus) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nout) out[i] = 0.0f;
+    if (i < nstatus) status[i] = 0;
 }
 
 // Motor spike counts of a world step added to the caller's totals
@@ -1433,7 +1452,7 @@ void launch_reset(const Pool &p, int64_t nn, int64_t ns, int nb, const uint32_t 
 void launch_reset_slots(const Pool &p, const int32_t *list, int count, const uint32_t *norig,
                         const uint32_t *sorig, hipStream_t st) {
     if (count > 0)
-        hipLaunchKernelGGL(k_reset_slots, dim3(count), dim3(256), 0, st, p, list, count, norig, sorig);
+        hipLaunchKernelGGL(k_reset_slots, dim3(count, 4), dim3(256), 0, st, p, list, count, norig, sorig);
 }
 
 void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st) {
@@ -1451,6 +1470,10 @@ void launch_fill(float *x, int64_t n, float v, hipStream_t st) {
 }
 void launch_fill_i32(int32_t *x, int64_t n, int32_t v, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, v);
+}
+void launch_clear_step(float *out, int64_t nout, int32_t *status, int64_t nstatus, hipStream_t st) {
+    const int64_t n = std::max(nout, nstatus);
+    if (n) hipLaunchKernelGGL(k_clear_step, dim3(blocks_for(n, 256)), dim3(256), 0, st, out, nout, status, nstatus);
 }
 
 void launch_add_counts(int32_t *total, const float *counts, int64_t n, hipStream_t st) {

@@ -35,7 +35,7 @@ static unsigned char *scratch_on(int dev) {
 }
 
 static void step_static(GenomeHandle &h, const float *input, int ticks, float gain, float *output) {
-    check_step(h.brain, input, ticks, gain);
+    check_step(h.brain(), input, ticks, gain);
     int dev = 0;
     NS_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= kStaticDevices) throw std::runtime_error("Device index past the static pools");
@@ -45,10 +45,10 @@ static void step_static(GenomeHandle &h, const float *input, int ticks, float ga
     pool.arena.carve(base, kStaticPoolBytes);
     float *d_in = nullptr, *d_out = nullptr;
     try {
-        build_one_brain(pool, h.brain, d_in, d_out);
+        build_one_brain(pool, h.brain(), d_in, d_out);
     } catch (const CarveFull &) {
-        throw std::runtime_error("Brain of " + std::to_string(h.brain.genes.size()) + " neurons and " +
-                                 std::to_string(h.brain.edges.size()) +
+        throw std::runtime_error("Brain of " + std::to_string(h.brain().genes.size()) + " neurons and " +
+                                 std::to_string(h.brain().edges.size()) +
                                  " synapses exceeds the static device pool of " +
                                  std::to_string(kStaticPoolBytes) + " bytes");
     }
