@@ -310,10 +310,12 @@ __device__ inline float branch_io(float c, const uint64_t *etab) {
 // never -0, so it is skipped.  With adapt_ns == 0, w == +0 and a finite
 // v - rest, the adaptation update adds a signed zero to +0, an identity.
 //
-// still (when not null) receives whether the tick left v and w unchanged in
-// every substep with the conductances, the inhibition and a decaying
-// afterdepolarization all +0 throughout (checked on entry: they then stay
-// +0, and they are not read in any other state).
+// still (when not null) receives whether the tick ended with v and w as it
+// began them, without a spike, with the conductances, the inhibition and a
+// decaying afterdepolarization all +0 throughout (checked on entry: they then
+// stay +0).  The tick is a function of its starting state and its inputs, so
+// the same inputs then give the same tick again, whatever v and w did
+// between its substeps.
 __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float current,
                                 const uint64_t *etab, bool *still = nullptr) {
     const float h = 0.25f;
@@ -321,6 +323,7 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
     const bool adp_on = !(g.flags & (1u << 31));
     int spike = 0;
     bool same = true;
+    const uint32_t v_start = __float_as_uint(s.v), w_start = __float_as_uint(s.w);
     if (still)
         same = (__float_as_uint(s.d0) | __float_as_uint(s.d1) | __float_as_uint(s.d2) |
                 __float_as_uint(s.d3) | __float_as_uint(s.inh) |
@@ -328,7 +331,6 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
         float v = s.v;
-        const uint32_t w_in = __float_as_uint(s.w);
         if (!(g.a == 0.0f && __float_as_uint(s.w) == 0u && __builtin_isfinite(v - g.rest)))
             s.w += h * (g.a * (v - g.rest) - s.w) / g.tauw;
         float x = clamp_range((v - g.thr) / g.slope, -20.0f, 10.0f);
@@ -352,13 +354,13 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
             s.w += g.b;
             s.adp += g.adp_spike;
         }
-        if (still) same = same && __float_as_uint(s.v) == __float_as_uint(v) && __float_as_uint(s.w) == w_in;
         if (adp_on) s.adp *= g.adp_decay;
         s.d0 *= kDecayExcQuarter; s.d1 *= kDecayExcQuarter;
         s.d2 *= kDecayExcQuarter; s.d3 *= kDecayExcQuarter;
         s.inh *= kDecayInhQuarter;
     }
-    if (still) *still = same && !spike;
+    if (still)
+        *still = same && !spike && __float_as_uint(s.v) == v_start && __float_as_uint(s.w) == w_start;
     return spike;
 }
 
