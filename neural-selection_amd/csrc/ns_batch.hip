@@ -1235,6 +1235,9 @@ void DevicePool::upload_state(const std::vector<ns_neuron_state> &in_n,
     up(view.arrived, arrived); up(view.last, last); up(view.ready, ready);
     NS_HIP(hipMemcpy(view.btick, btick.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
     if (stamps) up(view.stamps, st);
+    // the modulator bytes are per-tick outputs; cleared, they carry no
+    // settled mark over to the new state
+    NS_HIP(hipMemset(view.modbits, 0, size_t(N)));
 }
 
 // ---------------------------------------------------------------------------

@@ -972,6 +972,10 @@ constexpr uint32_t kPendMax = kQuietMax;
 // depolarization passes the profile-2 gate: plasticity reads the
 // depolarization only through that comparison.
 constexpr unsigned kModBits = kModNE | kMod5HT | kModDA, kArrivalMark = 0x40u, kDepolGate = 0x80u;
+// Global modulator byte (p.modbits) of a cell without sensory input whose last
+// tick of the previous call changed no state (k_resident_lds; every other
+// writer of the byte stores modulator bits only, which clears it).
+constexpr unsigned kStillCarry = 0x80u;
 // NeuronConst flag bit (k_resident_lds): an AdEx cell without sensory input,
 // whose tick may be skipped when its last one changed no state.
 constexpr uint32_t kSkipCell = 1u << 29;
@@ -1104,6 +1108,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     float nscale[KN], ncur[KN];
     int nbr[KN], nidx[KN];
     bool nlive[KN];
+    uint32_t carried = 0;   // bit k: kStillCarry of slot k's cell
 #pragma unroll
     for (int k = 0; k < KN; ++k) {
         const int n = nid[k];
@@ -1117,6 +1122,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             load_neuron(p, i, nr[k], nc[k], g_exp_tab);
             nscale[k] = p.ns[NS_LAS * N + i];
             nidx[k] = p.ninput[i];
+            carried |= uint32_t((p.modbits[i] & kStillCarry) != 0) << k;
         }
     }
     // The profile-selected traces of each plastic synapse (the flag word
@@ -1148,16 +1154,19 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             nscale[k] = ns_expf(nscale[k], g_exp_tab);
             ncur[k] = nidx[k] >= 0 ? input[nidx[k]] : -1.0f;
             if (nidx[k] < 0) nc[k].flags |= 1u << 30;
-            // an AdEx cell without sensory input: its only inputs are synaptic
+            // an AdEx cell without sensory input: its only inputs are synaptic,
+            // in this call and the next
             if (nidx[k] < 0 && (nc[k].flags & NF_TYPE) != 6) nc[k].flags |= kSkipCell;
+            else carried &= ~(1u << k);
         }
     }
     __syncthreads();
 
     unsigned my_events = 0, my_skipped = 0;
-    // bit k: neuron slot k's last tick changed no state (see phase B); and
-    // whether any tick of this call changed it (else write-back skips it)
-    uint32_t still = 0, dirty = 0;
+    // bit k: neuron slot k's last tick (of this call, or of the previous one
+    // for a cell without input) changed no state (see phase B); and whether
+    // any tick of this call changed it (else write-back skips it)
+    uint32_t still = carried, dirty = 0;
     for (int t = 0; t < ticks; ++t) {
         // Phase A: plasticity of tick t-1, arrivals of tick t; the arrivals
         // that take a short-term release are noted (one bit per round) and
@@ -1330,7 +1339,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         if (nc[k].flags & NF_ACTIVE) {
             if ((dirty >> k) & 1u) store_neuron(p, i, nr[k]);   // else the loaded state, unchanged
             p.spike[i] = uint8_t(hist_l[n] & 1u);
-            p.modbits[i] = mod_l[n] & kModBits;
+            // a cell without input that ends the call settled carries it over
+            const bool carry = ((still >> k) & 1u) && (nc[k].flags & (1u << 30));
+            p.modbits[i] = uint8_t((mod_l[n] & kModBits) | (carry ? kStillCarry : 0u));
         } else if (!(nc[k].flags & NF_WORLD)) {
             p.spike[i] = 0;
         }

@@ -175,3 +175,33 @@ def test_identity_ticks_skipped_bit_exact(tiles, big):
           f"big brains {info['big_brains']}", flush=True)
     assert (info['big_brains'] > 0) == big
     assert info['skipped_neuron_ticks'] > 0.2 * w.total_neurons * w.ticks * 40
+
+
+def test_settled_marks_end_with_an_imported_state():
+    """A cell without input that ends a call settled skips the first tick of
+    the next call too.  Importing a state replaces what that mark was about:
+    after 20 calls (cells settled, marks set) the state right after reset is
+    imported, and the next calls must equal the oracle's run from that state,
+    which computes every tick."""
+    w = W.small_mix(seed=43, brains=12, calls=40, types=(0, 1, 4, 5), lo=100, hi=400, fanout=1.0,
+                    modulator_p=0.05)
+    print(f'[parity] settled marks: brains={w.count} neurons={w.total_neurons}', flush=True)
+    batch = ns_amd.GpuBatch(workload=w)
+    try:
+        batch.set_tiles(1024, 2, 0, 1, 4224)
+        batch.reset(w.seeds())
+        start = batch.export()
+        for call in range(20):
+            batch.step(w.inputs(call), w.ticks, w.gain)
+        skipped = batch.info()['skipped_neuron_ticks']
+        assert skipped > 0.2 * w.total_neurons * w.ticks * 20
+        batch.import_state(*start)
+        brains = oracle_brains(w)
+        for call in range(6):
+            x = w.inputs(call)
+            out, status = batch.step(x, w.ticks, w.gain)
+            oo, orc = oracle_step_all(brains, x, w.ticks, w.gain)
+            assert np.array_equal(out, oo), f'call {call} after the import: motor outputs differ'
+            assert_state_equal(batch.export(), oracle_export_all(brains), f'call {call} after the import')
+    finally:
+        batch.close()
