@@ -343,7 +343,8 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
         } else {
             soma += s.d0; soma += s.d1; soma += s.d2; soma += s.d3;
         }
-        float dv = g.leak * (g.rest - v) + g.leak_slope * ns_expf(x, etab) +
+        // x is clamped to [-20, 10] (a nan v clamps too): ns_expf's range
+        float dv = g.leak * (g.rest - v) + g.leak_slope * ns_expf_finite(x, etab) +
                    soma * (0.0f - v) + s.inh * (-75.0f - v) + current + s.adp - s.w -
                    0.0f * (v - -68.0f);
         s.v = v + h * dv / g.cap;

@@ -55,16 +55,10 @@ NS_HOST_DEVICE inline double u2d(uint64_t u) {
     union { uint64_t u; double d; } c; c.u = u; return c.d;
 }
 
-// `tab` points at NS_EXP2_TABLE (LDS or global).
-NS_HOST_DEVICE inline float ns_expf(float x, const uint64_t *tab) {
-    uint32_t ux = f2u(x);
-    uint32_t abstop = (ux >> 20) & 0x7ff;
-    if (abstop >= 0x42b) {                       // |x| >= 88 or nan
-        if (ux == 0xff800000u) return 0.0f;      // -inf
-        if (abstop >= 0x7f8) return x + x;       // inf / nan
-        if (x > 0x1.62e42ep6f) return __builtin_inff();
-        if (x < -0x1.9fe368p6f) return 0.0f;
-    }
+// ns_expf for |x| < 88 (no overflow, underflow, infinity or nan to handle):
+// the same evaluation as ns_expf on those inputs.  `tab` points at
+// NS_EXP2_TABLE (LDS or global).
+NS_HOST_DEVICE inline float ns_expf_finite(float x, const uint64_t *tab) {
     double xd = (double)x;
     double kd = __builtin_fma(kInvLn2N, xd, kShift);
     uint64_t ki = d2u(kd);
@@ -79,6 +73,19 @@ NS_HOST_DEVICE inline float ns_expf(float x, const uint64_t *tab) {
     y = __builtin_fma(z, r2, y);
     y = y * s;
     return (float)y;
+}
+
+// `tab` points at NS_EXP2_TABLE (LDS or global).
+NS_HOST_DEVICE inline float ns_expf(float x, const uint64_t *tab) {
+    uint32_t ux = f2u(x);
+    uint32_t abstop = (ux >> 20) & 0x7ff;
+    if (abstop >= 0x42b) {                       // |x| >= 88 or nan
+        if (ux == 0xff800000u) return 0.0f;      // -inf
+        if (abstop >= 0x7f8) return x + x;       // inf / nan
+        if (x > 0x1.62e42ep6f) return __builtin_inff();
+        if (x < -0x1.9fe368p6f) return 0.0f;
+    }
+    return ns_expf_finite(x, tab);
 }
 
 // logf: the published ARM optimized-routines logf that glibc ships.  x =
