@@ -303,12 +303,13 @@ __device__ inline float branch_io(float c, const uint64_t *etab) {
 }
 
 // physiology.cuh:290-374, AdEx branch (types 0, 1, 4, 5).  Returns the spike.
-// Two exact shortcuts.  branch_io(c) is exactly +0 for 0 <= c < 2^-37: c - 4
+// One exact shortcut: branch_io(c) is exactly +0 for 0 <= c < 2^-37: c - 4
 // rounds to -4, so the regenerative term equals its value at c = 0, which is
 // the subtracted offset (a float in [2^-13, 2^-12), whose half ulp is 2^-37),
 // and c + offset rounds to offset.  Such a dendrite adds +0 to a sum that is
-// never -0, so it is skipped.  With adapt_ns == 0, w == +0 and a finite
-// v - rest, the adaptation update adds a signed zero to +0, an identity.
+// never -0, so it is skipped.  The adaptation update runs in every substep,
+// as in the reference (a branch around it costs more than the division it
+// would save, and it is what a zero or nan time constant needs).
 //
 // still (when not null) receives whether the tick ended with v and w as it
 // began them, without a spike, with the conductances, the inhibition and a
@@ -331,8 +332,7 @@ __device__ inline int adex_tick(NeuronRegs &s, const NeuronConst &g, float curre
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
         float v = s.v;
-        if (!(g.a == 0.0f && __float_as_uint(s.w) == 0u && __builtin_isfinite(v - g.rest)))
-            s.w += h * (g.a * (v - g.rest) - s.w) / g.tauw;
+        s.w += h * (g.a * (v - g.rest) - s.w) / g.tauw;
         float x = clamp_range((v - g.thr) / g.slope, -20.0f, 10.0f);
         float soma = 0;
         if (cortical) {
