@@ -1167,6 +1167,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     // for a cell without input) changed no state (see phase B); and whether
     // any tick of this call changed it (else write-back skips it)
     uint32_t still = carried, dirty = 0;
+    uint32_t bad = 0;   // bit k: a tick of this call left slot k's v nonfinite
     for (int t = 0; t < ticks; ++t) {
         // Phase A: plasticity of tick t-1, arrivals of tick t; the arrivals
         // that take a short-term release are noted (one bit per round) and
@@ -1299,7 +1300,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             dirty |= uint32_t(!unchanged) << k;
             hist_l[n] = (hist_l[n] << 1) | uint32_t(spike);
             mod_l[n] = uint8_t(mod | (nr[k].depol >= kDepolGateMv ? kDepolGate : 0u));
-            if (!isfinite(nr[k].v)) p.bstatus[pk.b0 + nbr[k]] = -1;
+            bad |= uint32_t(!isfinite(nr[k].v)) << k;
         }
         __syncthreads();
         for (int m = tid; m < pk.nm; m += WG)
@@ -1337,6 +1338,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             p.ns[NS_LAS * N + i] = homeostasis_las(p.ns[NS_LAS * N + i], float(ticks), hz);
         }
         if (nc[k].flags & NF_ACTIVE) {
+            if ((bad >> k) & 1u) p.bstatus[pk.b0 + nbr[k]] = -1;
             if ((dirty >> k) & 1u) store_neuron(p, i, nr[k]);   // else the loaded state, unchanged
             p.spike[i] = uint8_t(hist_l[n] & 1u);
             // a cell without input that ends the call settled carries it over
