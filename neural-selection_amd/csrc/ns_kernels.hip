@@ -1190,26 +1190,27 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
                         const int profile = (f >> SF_PLAST_SHIFT) & 7;
                         const uint32_t word = ai_l[q];
                         pend = word >> kPendShift;
-                        if (t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b]) {
-                            // the step of tick t-1, deferred when it has no input
-                            // and no arrival at t reads the weight next
-                            const unsigned pm = mod_l[post];
-                            const unsigned modrec = (f >> SF_MODREC_SHIFT) & 7;
-                            const bool arrived = (word & kArrMask) != kArrNone;
-                            const bool spiked = hist_l[post] & 1;
-                            if (arrived || spiked || (pm & kModBits & modrec) || arr || pend == kPendMax) {
-                                PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
-                                const float w0 = w0_l[q];
-                                replay_quiet_steps(r, profile, w0, pend);
-                                plasticity_step(r, profile, modrec, w0, arrived, spiked, gate_depol(pm),
-                                                pm & kModBits, etab);
-                                w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
-                                pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
-                                pend = 0;
-                            } else {
-                                ++pend;
-                            }
+                        // the step of tick t-1, deferred when it has no input
+                        // and no arrival at t reads the weight next.  The
+                        // test runs in every lane (a wave holds pair synapses
+                        // almost always); only the step itself is a branch.
+                        const bool pair = t > 0 && (route & R_PAIR) && profile && !bfrozen_l[b];
+                        const unsigned pm = mod_l[post];
+                        const unsigned modrec = (f >> SF_MODREC_SHIFT) & 7;
+                        const bool arrived = (word & kArrMask) != kArrNone;
+                        const bool spiked = hist_l[post] & 1;
+                        const bool input = arrived || spiked || (pm & kModBits & modrec) || arr ||
+                                           pend == kPendMax;
+                        if (pair && input) {
+                            PlastRegs r{w_l[q], x_l[q], y_l[q], pa_l[q], qa_l[q]};
+                            const float w0 = w0_l[q];
+                            replay_quiet_steps(r, profile, w0, pend);
+                            plasticity_step(r, profile, modrec, w0, arrived, spiked, gate_depol(pm),
+                                            pm & kModBits, etab);
+                            w_l[q] = r.weight; x_l[q] = r.x; y_l[q] = r.y;
+                            pa_l[q] = r.pre_age; qa_l[q] = r.post_age;
                         }
+                        pend = pair ? (input ? 0u : pend + 1) : pend;
                     }
                     // short-term arrivals take their code below
                     ai_l[q] = uint16_t((pend << kPendShift) | (arr ? kArrDirect : kArrNone));
