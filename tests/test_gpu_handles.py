@@ -84,3 +84,40 @@ def test_reference_two_step_batch_path_on_stateful_handles():
             assert np.array_equal(out, oo), c
     finally:
         L.ns_gpu_destroy(batch)
+
+
+def test_batch_keeps_the_genomes_it_was_made_from():
+    """A batch shares its genomes with the handles it was made from; a handle
+    that then changes its genome (its original indices, which seed the
+    initial states) takes its own copy, so the batch still steps the genomes
+    it was made from: outputs and states equal a batch made directly from
+    the same genomes."""
+    w = W.small_mix(seed=5, brains=6, calls=4, types=(0, 1, 4, 5), lo=20, hi=60)
+    print(f'[handles] shared genomes: brains={w.count}', flush=True)
+    hs = ns_amd.GenomeHandles([w.brain(i) for i in range(w.count)], w.channels, max_delay=w.max_delay)
+    ref = ns_amd.GpuBatch(workload=w)
+    b = ns_amd.GpuBatch(workload=w)
+    try:
+        L = b.lib
+        L.ns_gpu_destroy(b.ptr)
+        b.ptr = L.ns_gpu_create((ct.c_void_p * w.count)(*hs.ptrs), w.count)
+        assert b.ptr, ns_amd.error()
+        n0, s0 = int(w.neuron_counts[0]), int(w.edge_counts[0])
+        perm_n = np.ascontiguousarray(np.arange(n0)[::-1] + 1000, np.uint32)
+        perm_s = np.ascontiguousarray(np.arange(max(s0, 1))[::-1] + 5000, np.uint32)
+        L.ns_original_indices(hs.ptrs[0], perm_n.ctypes.data_as(ns_amd._u32),
+                              perm_s.ctypes.data_as(ns_amd._u32))
+        for batch in (ref, b):
+            batch.reset(w.seeds())
+        for call in range(4):
+            x = w.inputs(call)
+            o_ref, st_ref = ref.step(x, w.ticks, w.gain)
+            o_b, st_b = b.step(x, w.ticks, w.gain)
+            assert np.array_equal(o_ref, o_b), f'call {call}: outputs differ'
+        for a_ref, a_b in zip(ref.export(), b.export()):
+            if a_ref is not None:
+                assert a_ref.tobytes() == a_b.tobytes(), 'batch state differs'
+    finally:
+        ref.close()
+        b.close()
+        hs.close()
