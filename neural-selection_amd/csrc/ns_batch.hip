@@ -982,7 +982,7 @@ bool DevicePool::append_slots(const std::vector<std::pair<int, int>> &caps,
 }
 
 void DevicePool::order_touched(hipStream_t st) {
-    launch_pack_order(view.nflags, d_packs, d_touched, int(touched.size()), d_perm, st);
+    launch_pack_order(view.nflags, view.modbits, d_packs, d_touched, int(touched.size()), d_perm, st);
     NS_HIP(hipGetLastError());
     touched.clear();
 }
@@ -1490,8 +1490,9 @@ void Batch::replace(int k, const int *idx, std::vector<std::shared_ptr<const Hos
     stage.add(pool.view.brun, next.data(), size_t(pool.nb));
     pool.upload_stage(stage, st);
     pool.scatter_range(0, int(stage.entries.size()), st);
-    pool.order_touched(st);
+    // the reset clears the newborns' settled marks, which the slot order reads
     launch_reset_slots(pool.view, pool.d_reset, k, pool.d_norig, pool.d_sorig, st);
+    pool.order_touched(st);
     NS_HIP(hipGetLastError());
     h_brun = next;
     brun_valid = true;

@@ -67,6 +67,10 @@ constexpr uint32_t R_BRANCH = 3u, R_INH = 1u << 2, R_MOD_SHIFT = 3, R_MOD = 3u <
 constexpr uint32_t R_VALID = 1u << 5, R_STP = 1u << 6, R_PAIR = 1u << 7;
 
 constexpr unsigned kModNE = 1, kMod5HT = 2, kModDA = 4;
+// Global modulator byte (Pool::modbits) of a cell without sensory input whose
+// last tick of the previous call changed no state (k_resident_lds; every other
+// writer of the byte stores modulator bits only, which clears it).
+constexpr unsigned kStillCarry = 0x80u;
 constexpr float kDepolGateMv = 2.3f;   // profile-2 LTP gate on the post's recent depolarization
 
 // World-rule ring length (src/main.cu:45-46 HISTORY = D + 1): 9 slots for the

@@ -61,7 +61,8 @@ struct ScatterEntry {
 void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *base, hipStream_t st);
 // Stable class order of the neuron slots of packs[list[0..count)] into perm
 // (pack-relative indices at perm[pack.n0 ...]), as the pool build computes it.
-void launch_pack_order(const uint32_t *nflags, const Pack *packs, const int32_t *list, int count,
+void launch_pack_order(const uint32_t *nflags, const uint8_t *modbits, const Pack *packs,
+                       const int32_t *list, int count,
                        uint16_t *perm, hipStream_t st);
 void launch_fill(float *x, int64_t n, float v, hipStream_t st);
 void launch_clear_step(float *out, int64_t nout, int32_t *status, int64_t nstatus, hipStream_t st);

@@ -128,32 +128,38 @@ __global__ void k_scatter(const ScatterEntry *entries, int n, const unsigned cha
     }
 }
 
-// Stable counting sort of one pack's neuron slots by slot_class: per chunk of
-// 1024 slots, each class's ballot gives a slot its rank within its wavefront
-// and the wavefront's count; earlier wavefronts and chunks add their counts.
+// Stable counting sort of one pack's neuron slots by slot_class, the cells
+// that carry a settled mark (kStillCarry) after the others, as k_resident_lds
+// orders them after a launch: per chunk of 1024 slots, each key's ballot gives
+// a slot its rank within its wavefront and the wavefront's count; earlier
+// wavefronts and chunks add their counts.
 constexpr int kOrderThreads = 1024;
+__device__ inline int order_key(const uint32_t *nflags, const uint8_t *modbits, int64_t i) {
+    return slot_class(nflags[i]) + ((modbits[i] & kStillCarry) ? kSlotClasses : 0);
+}
 __global__ __launch_bounds__(kOrderThreads) void k_pack_order(const uint32_t *nflags,
+                                                              const uint8_t *modbits,
                                                               const Pack *packs,
                                                               const int32_t *list,
                                                               uint16_t *perm) {
-    constexpr int W = kOrderThreads / 64;
-    __shared__ int total[kSlotClasses], base[kSlotClasses], wcnt[W][kSlotClasses];
+    constexpr int W = kOrderThreads / 64, K = 2 * kSlotClasses;
+    __shared__ int total[K], base[K], wcnt[W][K];
     const Pack pk = packs[list[blockIdx.x]];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid < kSlotClasses) total[tid] = 0;
+    if (tid < K) total[tid] = 0;
     __syncthreads();
-    for (int j = tid; j < pk.nn; j += kOrderThreads) atomicAdd(&total[slot_class(nflags[pk.n0 + j])], 1);
+    for (int j = tid; j < pk.nn; j += kOrderThreads) atomicAdd(&total[order_key(nflags, modbits, pk.n0 + j)], 1);
     __syncthreads();
     if (tid == 0) {
         int s = 0;
-        for (int c = 0; c < kSlotClasses; ++c) { base[c] = s; s += total[c]; }
+        for (int c = 0; c < K; ++c) { base[c] = s; s += total[c]; }
     }
     __syncthreads();
     for (int j0 = 0; j0 < pk.nn; j0 += kOrderThreads) {   // uniform trip count
         const int j = j0 + tid;
-        const int c = j < pk.nn ? slot_class(nflags[pk.n0 + j]) : -1;
+        const int c = j < pk.nn ? order_key(nflags, modbits, pk.n0 + j) : -1;
         int rank = 0;
-        for (int k = 0; k < kSlotClasses; ++k) {
+        for (int k = 0; k < K; ++k) {
             const uint64_t m = __ballot(c == k);
             if (lane == 0) wcnt[wave][k] = __popcll(m);
             if (c == k) rank = __popcll(m & ((uint64_t(1) << lane) - 1));
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_pack_order(const uint32_t *nf
             perm[pk.n0 + pos] = uint16_t(j);
         }
         __syncthreads();
-        if (tid < kSlotClasses) {
+        if (tid < K) {
             int s = 0;
             for (int w = 0; w < W; ++w) s += wcnt[w][tid];
             base[tid] += s;
@@ -972,10 +978,6 @@ constexpr uint32_t kPendMax = kQuietMax;
 // depolarization passes the profile-2 gate: plasticity reads the
 // depolarization only through that comparison.
 constexpr unsigned kModBits = kModNE | kMod5HT | kModDA, kArrivalMark = 0x40u, kDepolGate = 0x80u;
-// Global modulator byte (p.modbits) of a cell without sensory input whose last
-// tick of the previous call changed no state (k_resident_lds; every other
-// writer of the byte stores modulator bits only, which clears it).
-constexpr unsigned kStillCarry = 0x80u;
 // NeuronConst flag bit (k_resident_lds): an AdEx cell without sensory input,
 // whose tick may be skipped when its last one changed no state.
 constexpr uint32_t kSkipCell = 1u << 29;
@@ -1473,10 +1475,11 @@ void launch_scatter(const ScatterEntry *entries, int n, const unsigned char *bas
     if (n) hipLaunchKernelGGL(k_scatter, dim3(n), dim3(256), 0, st, entries, n, base);
 }
 
-void launch_pack_order(const uint32_t *nflags, const Pack *packs, const int32_t *list, int count,
-                       uint16_t *perm, hipStream_t st) {
+void launch_pack_order(const uint32_t *nflags, const uint8_t *modbits, const Pack *packs,
+                       const int32_t *list, int count, uint16_t *perm, hipStream_t st) {
     if (count > 0)
-        hipLaunchKernelGGL(k_pack_order, dim3(count), dim3(kOrderThreads), 0, st, nflags, packs, list, perm);
+        hipLaunchKernelGGL(k_pack_order, dim3(count), dim3(kOrderThreads), 0, st, nflags, modbits,
+                           packs, list, perm);
 }
 
 void launch_fill(float *x, int64_t n, float v, hipStream_t st) {

@@ -433,8 +433,9 @@ struct Server {
         // the staged vectors are copied into pinned memory by upload_stage
         pool.upload_stage(stage, stream);
         pool.scatter_range(0, int(stage.entries.size()), stream);
-        pool.order_touched(stream);
+        // the reset clears the newborns' settled marks, which the slot order reads
         launch_reset_slots(pool.view, pool.d_reset, k, pool.d_norig, pool.d_sorig, stream);
+        pool.order_touched(stream);
         NS_HIP(hipGetLastError());
         h_run = next;
     }
