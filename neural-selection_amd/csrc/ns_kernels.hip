@@ -1381,12 +1381,13 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     if ((tid & 63) == 0 && my_events) atomicAdd(&events_l, my_events);
     if ((tid & 63) == 0 && my_skipped) atomicAdd(&skipped_l, my_skipped);
     // Slot order for the next launch: by cell class as the build orders it,
-    // with the slots whose last tick changed no state (and those of brains
-    // that sat out) after all the others, so that they fill whole
-    // wavefronts, whose ticks then skip together.  The order only assigns
-    // neurons to threads.
+    // cells with sensory input (never skipped) first, then the unsettled
+    // cells without input, then the slots whose last tick changed no state
+    // (and those of brains that sat out), so that cells that skip share
+    // wavefronts, now and once the unsettled ones settle.  The order only
+    // assigns neurons to threads.
     {
-        constexpr int W = WG / 64, K = 2 * kSlotClasses;
+        constexpr int W = WG / 64, K = 3 * kSlotClasses;
         const int lane = tid & 63, wave = tid >> 6;
         int *cnt = reinterpret_cast<int *>(lds_syn);   // [KN][W][K], then totals and bases
         int *tot = cnt + KN * W * K, *base = tot + K;
@@ -1397,7 +1398,9 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
             key[k] = -1;
             rank[k] = 0;
             if (nid[k] < pk.nn)
-                key[k] = slot_class(nc[k].flags) + ((((still >> k) & 1u) || !nlive[k]) ? kSlotClasses : 0);
+                key[k] = slot_class(nc[k].flags) +
+                         ((((still >> k) & 1u) || !nlive[k]) ? 2 * kSlotClasses
+                          : (nc[k].flags & (1u << 30)) ? kSlotClasses : 0);
             for (int c = 0; c < K; ++c) {
                 const uint64_t m = __ballot(key[k] == c);
                 if (lane == 0) cnt[(k * W + wave) * K + c] = __popcll(m);
