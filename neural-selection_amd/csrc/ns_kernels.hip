@@ -214,7 +214,8 @@ __global__ void k_add_counts(int32_t *total, const float *counts, int64_t n) {
 
 // Delivery (src/main.cu:659-708; gym_adapter/core.cuh:37-47): arrival test on
 // the spike shift register, short-term release on arrival.  Returns arrival.
-__device__ inline unsigned deliver_one(const Pool &p, int64_t q, int32_t b, int t) {
+__device__ inline unsigned deliver_one(const Pool &p, int64_t q, int32_t b, int t,
+                                       const uint64_t *etab = g_exp_tab) {
     uint32_t flags = p.sflags[q];
     uint32_t route = flags >> SF_ROUTE_SHIFT;
     float rel = 0;
@@ -225,7 +226,7 @@ __device__ inline unsigned deliver_one(const Pool &p, int64_t q, int32_t b, int 
         if (tick >= d) arr = (p.hist[p.spre[q]] >> (d - 1)) & 1u;
         if (arr) {
             float w = p.ss[SS_WEIGHT * p.scap + q];
-            rel = (route & R_STP) ? release_at(p, q, flags, w, tick, g_exp_tab) : w;
+            rel = (route & R_STP) ? release_at(p, q, flags, w, tick, etab) : w;
         }
     }
     p.arrived[q] = uint8_t(arr);
@@ -302,7 +303,8 @@ __device__ inline float sensory_current(const Pool &p, int64_t i, NeuronRegs &s,
 // AdEx cell without sensory input (neuron_tick), so that its next tick with
 // no arrival is the identity.
 __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, const float *input,
-                                  float gain, bool arrived = true, bool *still = nullptr) {
+                                  float gain, bool arrived = true, bool *still = nullptr,
+                                  const uint64_t *etab = g_exp_tab) {
     uint32_t flags = p.nflags[i];
     if (!(flags & NF_ACTIVE)) {
         uint32_t h = p.hist[i];
@@ -318,11 +320,11 @@ __device__ inline void neuron_one(const Pool &p, int64_t i, int32_t b, int t, co
     // the state and gene loads go out before the input gather's chain
     NeuronRegs s;
     NeuronConst g;
-    load_neuron(p, i, s, g, g_exp_tab);
+    load_neuron(p, i, s, g, etab);
     float e0 = 0, e1 = 0, e2 = 0, e3 = 0, inh = 0;
-    const unsigned mod = arrived ? gather_inputs(p, i, tick, e0, e1, e2, e3, inh, g_exp_tab) : 0u;
+    const unsigned mod = arrived ? gather_inputs(p, i, tick, e0, e1, e2, e3, inh, etab) : 0u;
     float current = sensory_current(p, i, s, g, input, gain);
-    int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, g_exp_tab, still);
+    int spike = neuron_tick(s, g, current, e0, e1, e2, e3, inh, p.cell, p.ncap, i, etab, still);
     if (still) *still = *still && p.ninput[i] < 0 && (flags & NF_TYPE) != 6;
     store_neuron(p, i, s);
     p.spike[i] = uint8_t(spike);
@@ -403,7 +405,8 @@ __global__ void k_neuron_slow(Pool p, const int32_t *list, int count, int t, uin
 }
 
 // Plasticity (src/main.cu:748-782; core.cuh:60-66) of synapse q of brain b.
-__device__ inline void plasticity_one(const Pool &p, int64_t q, int32_t b) {
+__device__ inline void plasticity_one(const Pool &p, int64_t q, int32_t b,
+                                      const uint64_t *etab = g_exp_tab) {
     if (p.bfrozen[b]) return;
     uint32_t flags = p.sflags[q];
     uint32_t route = flags >> SF_ROUTE_SHIFT;
@@ -414,7 +417,7 @@ __device__ inline void plasticity_one(const Pool &p, int64_t q, int32_t b) {
     load_plast(p, q, profile, r);
     plasticity_step(r, profile, (flags >> SF_MODREC_SHIFT) & 7, p.sg[SG_W0 * p.scap + q],
                     p.arrived[q] != 0, p.spike[post] != 0, p.ns[NS_DEPOL * p.ncap + post],
-                    p.modbits[post], g_exp_tab);
+                    p.modbits[post], etab);
     store_plast(p, q, profile, r);
 }
 
@@ -525,7 +528,7 @@ __device__ inline void replay_quiet_steps(PlastRegs &r, int profile, float w0, u
 // with an input, the 127th deferred step and the call's last step replay the
 // deferred ones first.
 __device__ inline void plasticity_deferred(const Pool &p, int64_t q, int32_t b, bool arr_now, bool last,
-                                           uint8_t &pend) {
+                                           uint8_t &pend, const uint64_t *etab = g_exp_tab) {
     if (p.bfrozen[b]) return;
     const uint32_t flags = p.sflags[q];
     const uint32_t route = flags >> SF_ROUTE_SHIFT;
@@ -542,7 +545,7 @@ __device__ inline void plasticity_deferred(const Pool &p, int64_t q, int32_t b, 
     load_plast(p, q, profile, r);
     const float w0 = p.sg[SG_W0 * p.scap + q];
     replay_quiet_steps(r, profile, w0, pend);
-    plasticity_step(r, profile, modrec, w0, arrived, spiked, p.ns[NS_DEPOL * p.ncap + post], pm, g_exp_tab);
+    plasticity_step(r, profile, modrec, w0, arrived, spiked, p.ns[NS_DEPOL * p.ncap + post], pm, etab);
     store_plast(p, q, profile, r);
     pend = 0;
 }
@@ -568,7 +571,8 @@ __device__ inline bool arrives(const Pool &p, int64_t q, int32_t b, int t) {
 // register.  The synapse byte counts its deferred quiet plasticity steps.
 template <int WG>
 __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float *input,
-                                float gain, float *out, int homeostasis, uint8_t *lds, int64_t hit_cap) {
+                                float gain, float *out, int homeostasis, uint8_t *lds, int64_t hit_cap,
+                                const uint64_t *etab) {
     __shared__ unsigned big_events, big_skipped;
     if (!(p.brun[b] & 1)) return;
     const int tid = threadIdx.x;
@@ -587,18 +591,18 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
     for (int t = 0; t < ticks; ++t) {
         for (int64_t q = s0 + tid; q < s1; q += WG) {
             if (lds) {
-                if (t > 0) plasticity_deferred(p, q, b, arrives(p, q, b, t), false, pend[q - s0]);
+                if (t > 0) plasticity_deferred(p, q, b, arrives(p, q, b, t), false, pend[q - s0], etab);
             } else if (t > 0) {
-                plasticity_one(p, q, b);
+                plasticity_one(p, q, b, etab);
             }
-            const unsigned arr = deliver_one(p, q, b, t);
+            const unsigned arr = deliver_one(p, q, b, t, etab);
             if (arr && hit) hit[p.spost[q] - n0] |= 1;
             arrivals += arr;
         }
         __syncthreads();
         for (int64_t i = n0 + tid; i < n1; i += WG) {
             if (!hit) {
-                neuron_one(p, i, b, t, input, gain, true);
+                neuron_one(p, i, b, t, input, gain, true, nullptr, etab);
                 continue;
             }
             const uint8_t h = hit[i - n0];
@@ -610,7 +614,7 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
                 continue;
             }
             bool still = false;
-            neuron_one(p, i, b, t, input, gain, h & 1, &still);
+            neuron_one(p, i, b, t, input, gain, h & 1, &still, etab);
             hit[i - n0] = still ? 2 : 0;
         }
         __syncthreads();
@@ -621,8 +625,8 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
     }
     if (ticks > 0)
         for (int64_t q = s0 + tid; q < s1; q += WG) {
-            if (lds) plasticity_deferred(p, q, b, false, true, pend[q - s0]);
-            else plasticity_one(p, q, b);
+            if (lds) plasticity_deferred(p, q, b, false, true, pend[q - s0], etab);
+            else plasticity_one(p, q, b, etab);
         }
     __syncthreads();
     if (homeostasis) {
@@ -667,7 +671,8 @@ __global__ __launch_bounds__(WG, MINW) void k_resident(Pool p, const Pack *packs
     __shared__ unsigned events_l;
 
     if (int(blockIdx.x) < nbig) {
-        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis, nullptr, 0);
+        if (threadIdx.x < 32) etab[threadIdx.x] = g_exp_tab[threadIdx.x];   // ordered by the first barrier
+        big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis, nullptr, 0, etab);
         return;
     }
     const Pack pk = packs[blockIdx.x - nbig];
@@ -1005,9 +1010,11 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     __shared__ float crel_l[kRelCap];          // releases of the tick's short-term arrivals
 
     if (int(blockIdx.x) < nbig) {
-        // the pack's synapse arrays are free in a big brain's workgroup
+        // the pack's synapse arrays are free in a big brain's workgroup; the
+        // exponential's table is read from LDS, ordered by the first barrier
+        if (threadIdx.x < 32) etab[threadIdx.x] = g_exp_tab[threadIdx.x];
         big_brain_ticks<WG>(p, big[blockIdx.x], ticks, input, gain, out, homeostasis, lds_syn,
-                            int64_t(smax) * lds_synapse_bytes<PLASTIC>());
+                            int64_t(smax) * lds_synapse_bytes<PLASTIC>(), etab);
         return;
     }
     const Pack pk = packs[blockIdx.x - nbig];
