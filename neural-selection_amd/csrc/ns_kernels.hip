@@ -531,11 +531,12 @@ __device__ inline void plasticity_deferred(const Pool &p, int64_t q, int32_t b, 
                                            uint8_t &pend, const uint64_t *etab = g_exp_tab) {
     if (p.bfrozen[b]) return;
     const uint32_t flags = p.sflags[q];
+    const int32_t post = p.spost[q];           // loaded with the flags word (q is in the pool)
+    const bool arrived = p.arrived[q] != 0;
     const uint32_t route = flags >> SF_ROUTE_SHIFT;
     const int profile = (flags >> SF_PLAST_SHIFT) & 7;
     if (!(route & R_VALID) || !(route & R_PAIR) || !profile) return;
-    const int32_t post = p.spost[q];
-    const bool arrived = p.arrived[q] != 0, spiked = p.spike[post] != 0;
+    const bool spiked = p.spike[post] != 0;
     const unsigned pm = p.modbits[post], modrec = (flags >> SF_MODREC_SHIFT) & 7;
     if (!last && !arrived && !spiked && !(pm & modrec) && !arr_now && pend < kQuietMax) {
         ++pend;
@@ -553,9 +554,27 @@ __device__ inline void plasticity_deferred(const Pool &p, int64_t q, int32_t b, 
 // Whether synapse q of brain b arrives at tick t of the call.
 __device__ inline bool arrives(const Pool &p, int64_t q, int32_t b, int t) {
     const uint32_t flags = p.sflags[q];
+    const int32_t pre = p.spre[q];   // loaded with the flags word (q is in the pool)
     if (!((flags >> SF_ROUTE_SHIFT) & R_VALID)) return false;
     const int d = flags & SF_DELAY;
-    return p.btick[b] + t >= d && ((p.hist[p.spre[q]] >> (d - 1)) & 1u);
+    return p.btick[b] + t >= d && ((p.hist[pre] >> (d - 1)) & 1u);
+}
+
+// deliver_one for a synapse whose arrival at tick t is already known (the
+// arrival depends only on the flags word and the pre neuron's spike register,
+// which plasticity does not change); the weight is read after the synapse's
+// plasticity step of tick t-1, as in deliver_one.
+__device__ inline unsigned deliver_known(const Pool &p, int64_t q, int32_t b, int t, bool arr,
+                                         const uint64_t *etab) {
+    float rel = 0;
+    if (arr) {
+        const uint32_t flags = p.sflags[q];
+        const float w = p.ss[SS_WEIGHT * p.scap + q];
+        rel = ((flags >> SF_ROUTE_SHIFT) & R_STP) ? release_at(p, q, flags, w, p.btick[b] + t, etab) : w;
+    }
+    p.arrived[q] = uint8_t(arr);
+    p.released[q] = rel;
+    return arr;
 }
 
 // A brain too large for a pack: one workgroup runs all its ticks, with the
@@ -590,12 +609,15 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
     // thread, the reference's order), then the neurons of tick t.
     for (int t = 0; t < ticks; ++t) {
         for (int64_t q = s0 + tid; q < s1; q += WG) {
+            unsigned arr;
             if (lds) {
-                if (t > 0) plasticity_deferred(p, q, b, arrives(p, q, b, t), false, pend[q - s0], etab);
-            } else if (t > 0) {
-                plasticity_one(p, q, b, etab);
+                const bool now = arrives(p, q, b, t);
+                if (t > 0) plasticity_deferred(p, q, b, now, false, pend[q - s0], etab);
+                arr = deliver_known(p, q, b, t, now, etab);
+            } else {
+                if (t > 0) plasticity_one(p, q, b, etab);
+                arr = deliver_one(p, q, b, t, etab);
             }
-            const unsigned arr = deliver_one(p, q, b, t, etab);
             if (arr && hit) hit[p.spost[q] - n0] |= 1;
             arrivals += arr;
         }
