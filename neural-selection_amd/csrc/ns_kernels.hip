@@ -552,8 +552,8 @@ __device__ inline void plasticity_deferred(const Pool &p, int64_t q, int32_t b, 
 }
 
 // Whether synapse q of brain b arrives at tick t of the call.
-__device__ inline bool arrives(const Pool &p, int64_t q, int32_t b, int t) {
-    const uint32_t flags = p.sflags[q];
+__device__ inline bool arrives(const Pool &p, int64_t q, int32_t b, int t, uint32_t &flags) {
+    flags = p.sflags[q];
     const int32_t pre = p.spre[q];   // loaded with the flags word (q is in the pool)
     if (!((flags >> SF_ROUTE_SHIFT) & R_VALID)) return false;
     const int d = flags & SF_DELAY;
@@ -565,10 +565,9 @@ __device__ inline bool arrives(const Pool &p, int64_t q, int32_t b, int t) {
 // which plasticity does not change); the weight is read after the synapse's
 // plasticity step of tick t-1, as in deliver_one.
 __device__ inline unsigned deliver_known(const Pool &p, int64_t q, int32_t b, int t, bool arr,
-                                         const uint64_t *etab) {
+                                         uint32_t flags, const uint64_t *etab) {
     float rel = 0;
     if (arr) {
-        const uint32_t flags = p.sflags[q];
         const float w = p.ss[SS_WEIGHT * p.scap + q];
         rel = ((flags >> SF_ROUTE_SHIFT) & R_STP) ? release_at(p, q, flags, w, p.btick[b] + t, etab) : w;
     }
@@ -611,9 +610,10 @@ __device__ void big_brain_ticks(const Pool &p, int32_t b, int ticks, const float
         for (int64_t q = s0 + tid; q < s1; q += WG) {
             unsigned arr;
             if (lds) {
-                const bool now = arrives(p, q, b, t);
+                uint32_t flags;
+                const bool now = arrives(p, q, b, t, flags);
                 if (t > 0) plasticity_deferred(p, q, b, now, false, pend[q - s0], etab);
-                arr = deliver_known(p, q, b, t, now, etab);
+                arr = deliver_known(p, q, b, t, now, flags, etab);
             } else {
                 if (t > 0) plasticity_one(p, q, b, etab);
                 arr = deliver_one(p, q, b, t, etab);
@@ -1041,6 +1041,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
     }
     const Pack pk = packs[blockIdx.x - nbig];
     const int tid = threadIdx.x;
+    if (tid < 32) etab[tid] = g_exp_tab[tid];   // ordered by the barrier below
     {   // the pack's brains; a pack whose brains all sit out this call (free
         // slots, masks) does nothing
         int any = 0;
@@ -1078,7 +1079,6 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         const int slot = k * WG + tid;
         nid[k] = slot < pk.nn ? int(perm[n0 + slot]) : pk.nn;
     }
-    if (tid < 32) etab[tid] = g_exp_tab[tid];
     for (int n = tid; n < pk.nn; n += WG) {
         hist_l[n] = p.hist[n0 + n];
         in_l[n] = uint16_t(p.nin[n0 + n] - s0);
@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         if (n < pk.nn) {
             const int64_t i = n0 + n;
             nbr[k] = p.nbrain[i] - pk.b0;
-            load_neuron(p, i, nr[k], nc[k], g_exp_tab);
+            load_neuron(p, i, nr[k], nc[k], etab);
             nscale[k] = p.ns[NS_LAS * N + i];
             nidx[k] = p.ninput[i];
             carried |= uint32_t((p.modbits[i] & kStillCarry) != 0) << k;
@@ -1182,7 +1182,7 @@ __global__ __launch_bounds__(WG, MINW) void k_resident_lds(Pool p, const Pack *p
         ncur[k] = 0;
         if (nid[k] < pk.nn) {
             nlive[k] = brun_l[nbr[k]] != 0;
-            nscale[k] = ns_expf(nscale[k], g_exp_tab);
+            nscale[k] = ns_expf(nscale[k], etab);
             ncur[k] = nidx[k] >= 0 ? input[nidx[k]] : -1.0f;
             if (nidx[k] < 0) nc[k].flags |= 1u << 30;
             // an AdEx cell without sensory input: its only inputs are synaptic,
